@@ -1,0 +1,341 @@
+// rmq_api.hip — the C ABI of librmq.so (include/rmq.h): plan, build, query, free, GPU BASE.
+//
+// Host-side restatement of the reference's per-algorithm drivers xxx_rmq (src/xxx.h:203-298),
+// xxx_rmq_multi_load (src/xxx.h:530-627) and hierarchical_vector_load (src/xxx.h:932-1040),
+// split into build / query / free, stream-ordered and returning status codes.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <new>
+
+#include "../../include/rmq.h"
+#include "rmq_kernels.hpp"
+
+using rmq::HierView;
+using rmq::kBlock;
+
+struct rmq_hierarchy {
+  int alg;
+  int device;
+  uint32_t R_log;
+  uint32_t group_lanes;
+  uint64_t n;
+  uint64_t t;
+  uint64_t buffer_floats;
+  float* lv;
+  uint32_t* li;
+  HierView view;
+};
+
+namespace {
+
+constexpr uint64_t kMaxN = 0x7FFFFFFFull;  // int32 n, as the reference's CmdArgs (src/tools.h:30)
+constexpr uint64_t kMaxGrid = 1ull << 20;
+
+struct Plan {
+  uint32_t R_log = 0;
+  uint32_t G = 0;
+  uint32_t num_levels = 0;
+  uint64_t size[RMQ_MAX_LEVELS] = {};
+  uint64_t off[RMQ_MAX_LEVELS] = {};
+  uint64_t buffer_floats = 0;
+};
+
+bool alg_supported(int alg) {
+  return alg == RMQ_ALG_XXX || alg == RMQ_ALG_VECTOR_LOAD || alg == RMQ_ALG_XXX_MULTI_LOAD;
+}
+
+// Metadata recurrence of src/xxx.h:225-243 and the t checks of src/xxx.h:211-217 / 938-944.
+rmq_status make_plan(uint64_t n, uint64_t t, int alg, const rmq_build_options* opt, Plan* P) {
+  if (!alg_supported(alg)) return RMQ_EINVAL;
+  if (n == 0 || n > kMaxN || t == 0) return RMQ_EINVAL;
+  uint32_t g = 3, c = 2;  // XXX_CG_SIZE_LOG, XXX_CG_AMOUNT_LOG defaults (CMakeLists.txt:61-67)
+  uint32_t G = 0;
+  if (opt) {
+    const bool literal = (opt->flags & RMQ_BUILD_DEFAULT_CG) != 0;
+    if (opt->cg_size_log || literal) g = opt->cg_size_log;
+    if (opt->cg_amount_log || literal) c = opt->cg_amount_log;
+    G = opt->group_lanes;
+  }
+  uint32_t R_log;
+  if (alg == RMQ_ALG_VECTOR_LOAD) {
+    R_log = c + 2;  // src/xxx.h:934
+    if (R_log < 2 || R_log > 6) return RMQ_EUNSUPPORTED;
+    if (t < (1ull << R_log)) return RMQ_ETHRESHOLD;
+  } else {
+    R_log = g + c;  // src/xxx.h:208
+    if (R_log < 3 || R_log > 7) return RMQ_EUNSUPPORTED;
+    if (t < 2 * (1ull << R_log)) return RMQ_ETHRESHOLD;
+    if (G == 0) {
+      const uint32_t v = (1u << R_log) / 4;  // one 16-byte vector per lane per block
+      G = v < 4 ? 4 : (v > 16 ? 16 : v);
+    }
+    if (G != 4 && G != 8 && G != 16) return RMQ_EUNSUPPORTED;
+  }
+  P->R_log = R_log;
+  P->G = G;
+  const uint64_t R = 1ull << R_log;
+  uint64_t size = n;
+  P->num_levels = 1;
+  P->off[0] = 0xFFFFFFFFull;  // ~uint32_t(0), src/xxx.h:229
+  P->size[0] = size;
+  uint64_t offset = 0;
+  while (true) {
+    if (P->num_levels >= RMQ_MAX_LEVELS || size <= t) break;
+    size = (size + R - 1) >> R_log;
+    offset = (offset + RMQ_LEVEL_ALIGNMENT - 1) & ~static_cast<uint64_t>(RMQ_LEVEL_ALIGNMENT - 1);
+    P->off[P->num_levels] = offset;
+    P->size[P->num_levels] = size;
+    P->num_levels++;
+    offset += size;
+  }
+  P->buffer_floats = offset;
+  return RMQ_OK;
+}
+
+uint32_t grid_for(uint64_t threads) {
+  uint64_t b = (threads + kBlock - 1) / kBlock;
+  if (b == 0) b = 1;
+  if (b > kMaxGrid) b = kMaxGrid;
+  return static_cast<uint32_t>(b);
+}
+
+rmq_status last_error() { return hipGetLastError() == hipSuccess ? RMQ_OK : RMQ_EDEVICE; }
+
+// ---- build dispatch -----------------------------------------------------------------------
+template <int RL>
+void launch_build_level(const float* sv, const uint32_t* si, uint32_t ssize, float* dv,
+                        uint32_t* di, uint32_t dsize, hipStream_t s) {
+  constexpr uint32_t V = (1u << RL) / 4;
+  constexpr uint32_t W = V < 64 ? V : 64;
+  const uint32_t grid = grid_for(static_cast<uint64_t>(dsize) * W);
+  if (si) {
+    rmq::k_build_level<RL, true><<<grid, kBlock, 0, s>>>(sv, si, ssize, dv, di, dsize);
+  } else {
+    rmq::k_build_level<RL, false><<<grid, kBlock, 0, s>>>(sv, si, ssize, dv, di, dsize);
+  }
+}
+
+rmq_status build_level(uint32_t R_log, const float* sv, const uint32_t* si, uint32_t ssize,
+                       float* dv, uint32_t* di, uint32_t dsize, hipStream_t s) {
+  switch (R_log) {
+    case 2: launch_build_level<2>(sv, si, ssize, dv, di, dsize, s); break;
+    case 3: launch_build_level<3>(sv, si, ssize, dv, di, dsize, s); break;
+    case 4: launch_build_level<4>(sv, si, ssize, dv, di, dsize, s); break;
+    case 5: launch_build_level<5>(sv, si, ssize, dv, di, dsize, s); break;
+    case 6: launch_build_level<6>(sv, si, ssize, dv, di, dsize, s); break;
+    case 7: launch_build_level<7>(sv, si, ssize, dv, di, dsize, s); break;
+    default: return RMQ_EUNSUPPORTED;
+  }
+  return last_error();
+}
+
+// ---- query dispatch -----------------------------------------------------------------------
+template <int G, int RL>
+rmq_status launch_cl(int alg, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
+                     const HierView& H, hipStream_t s) {
+  const bool idx = oi != nullptr;
+  if (alg == RMQ_ALG_XXX) {
+    const uint32_t grid = grid_for(q);
+    if (grid == kMaxGrid && static_cast<uint64_t>(q) > kMaxGrid * kBlock) return RMQ_EINVAL;
+    if (idx) rmq::k_query_xxx<G, RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_xxx<G, RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  } else {
+    const uint32_t grid = grid_for(static_cast<uint64_t>(q) * G);
+    if (idx) rmq::k_query_multi<G, RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_multi<G, RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  }
+  return last_error();
+}
+
+template <int RL>
+rmq_status launch_cl_g(uint32_t G, int alg, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
+                       const HierView& H, hipStream_t s) {
+  switch (G) {
+    case 4: return launch_cl<4, RL>(alg, Q, q, ov, oi, H, s);
+    case 8: return launch_cl<8, RL>(alg, Q, q, ov, oi, H, s);
+    case 16: return launch_cl<16, RL>(alg, Q, q, ov, oi, H, s);
+    default: return RMQ_EUNSUPPORTED;
+  }
+}
+
+template <int RL>
+rmq_status launch_vl(const uint2* Q, uint32_t q, float* ov, uint32_t* oi, const HierView& H,
+                     hipStream_t s) {
+  const uint32_t grid = grid_for(q);
+  if (static_cast<uint64_t>(grid) * kBlock < q) return RMQ_EINVAL;
+  if (oi) rmq::k_query_vl<RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  else rmq::k_query_vl<RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  return last_error();
+}
+
+rmq_status query_dispatch(const rmq_hierarchy* h, const uint2* Q, uint32_t q, float* ov,
+                          uint32_t* oi, hipStream_t s) {
+  const HierView& H = h->view;
+  if (h->alg == RMQ_ALG_VECTOR_LOAD) {
+    switch (h->R_log) {
+      case 2: return launch_vl<2>(Q, q, ov, oi, H, s);
+      case 3: return launch_vl<3>(Q, q, ov, oi, H, s);
+      case 4: return launch_vl<4>(Q, q, ov, oi, H, s);
+      case 5: return launch_vl<5>(Q, q, ov, oi, H, s);
+      case 6: return launch_vl<6>(Q, q, ov, oi, H, s);
+      default: return RMQ_EUNSUPPORTED;
+    }
+  }
+  switch (h->R_log) {
+    case 3: return launch_cl_g<3>(h->group_lanes, h->alg, Q, q, ov, oi, H, s);
+    case 4: return launch_cl_g<4>(h->group_lanes, h->alg, Q, q, ov, oi, H, s);
+    case 5: return launch_cl_g<5>(h->group_lanes, h->alg, Q, q, ov, oi, H, s);
+    case 6: return launch_cl_g<6>(h->group_lanes, h->alg, Q, q, ov, oi, H, s);
+    case 7: return launch_cl_g<7>(h->group_lanes, h->alg, Q, q, ov, oi, H, s);
+    default: return RMQ_EUNSUPPORTED;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+rmq_status rmq_plan(uint64_t n, uint64_t t, int alg, const rmq_build_options* opt,
+                    uint32_t* num_levels, uint64_t* sizes, uint64_t* offsets,
+                    uint64_t* buffer_floats, uint32_t* R) {
+  Plan P;
+  const rmq_status st = make_plan(n, t, alg, opt, &P);
+  if (st != RMQ_OK) return st;
+  if (num_levels) *num_levels = P.num_levels;
+  for (uint32_t k = 0; k < P.num_levels; ++k) {
+    if (sizes) sizes[k] = P.size[k];
+    if (offsets) offsets[k] = P.off[k];
+  }
+  if (buffer_floats) *buffer_floats = P.buffer_floats;
+  if (R) *R = 1u << P.R_log;
+  return RMQ_OK;
+}
+
+rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
+                        const rmq_build_options* opt, void* stream, rmq_hierarchy** out) {
+  if (!out) return RMQ_EINVAL;
+  *out = nullptr;
+  if (!d_array || (reinterpret_cast<uintptr_t>(d_array) & 15u) != 0) return RMQ_EINVAL;
+  Plan P;
+  const rmq_status st = make_plan(n, t, alg, opt, &P);
+  if (st != RMQ_OK) return st;
+
+  rmq_hierarchy* h = new (std::nothrow) rmq_hierarchy();
+  if (!h) return RMQ_ENOMEM;
+  h->alg = alg;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return RMQ_EDEVICE;
+  }
+  h->R_log = P.R_log;
+  h->group_lanes = P.G;
+  h->n = n;
+  h->t = t;
+  h->buffer_floats = P.buffer_floats;
+  if (P.num_levels > 1) {
+    const size_t bytes = static_cast<size_t>(P.buffer_floats) * 4;
+    if (hipMalloc(reinterpret_cast<void**>(&h->lv), bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      delete h;
+      return RMQ_ENOMEM;
+    }
+    if (hipMalloc(reinterpret_cast<void**>(&h->li), bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(h->lv);
+      delete h;
+      return RMQ_ENOMEM;
+    }
+  }
+  HierView& H = h->view;
+  H.A = d_array;
+  H.lv = h->lv;
+  H.li = h->li;
+  H.num_levels = P.num_levels;
+  for (uint32_t k = 0; k < rmq::kMaxLevels; ++k) {
+    H.size[k] = k < P.num_levels ? static_cast<uint32_t>(P.size[k]) : 0;
+    H.off[k] = k < P.num_levels ? static_cast<uint32_t>(P.off[k]) : 0;
+  }
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  for (uint32_t k = 0; k + 1 < P.num_levels; ++k) {
+    const float* sv = k == 0 ? d_array : h->lv + P.off[k];
+    const uint32_t* si = k == 0 ? nullptr : h->li + P.off[k];
+    const rmq_status bs =
+        build_level(P.R_log, sv, si, static_cast<uint32_t>(P.size[k]), h->lv + P.off[k + 1],
+                    h->li + P.off[k + 1], static_cast<uint32_t>(P.size[k + 1]), s);
+    if (bs != RMQ_OK) {
+      rmq_free(h);
+      return bs;
+    }
+  }
+  *out = h;
+  return RMQ_OK;
+}
+
+rmq_status rmq_build(const float* d_array, uint64_t n, uint64_t t, int alg, void* stream,
+                     rmq_hierarchy** out) {
+  return rmq_build_ex(d_array, n, t, alg, nullptr, stream, out);
+}
+
+rmq_status rmq_query(const rmq_hierarchy* h, const uint32_t* d_lr, uint64_t q, float* d_val,
+                     uint32_t* d_idx, void* stream) {
+  if (!h) return RMQ_EINVAL;
+  if (q == 0) return RMQ_OK;
+  if (!d_lr || (!d_val && !d_idx)) return RMQ_EINVAL;
+  if ((reinterpret_cast<uintptr_t>(d_lr) & 7u) != 0) return RMQ_EINVAL;
+  if (q > 0xFFFFFFFFull) return RMQ_EINVAL;
+  return query_dispatch(h, reinterpret_cast<const uint2*>(d_lr), static_cast<uint32_t>(q), d_val,
+                        d_idx, static_cast<hipStream_t>(stream));
+}
+
+void rmq_free(rmq_hierarchy* h) {
+  if (!h) return;
+  if (h->lv) (void)hipFree(h->lv);
+  if (h->li) (void)hipFree(h->li);
+  delete h;
+}
+
+rmq_status rmq_get_metadata(const rmq_hierarchy* h, uint32_t* num_levels, uint64_t* sizes,
+                            uint64_t* offsets, uint64_t* buffer_floats, uint32_t* R) {
+  if (!h) return RMQ_EINVAL;
+  const HierView& H = h->view;
+  if (num_levels) *num_levels = H.num_levels;
+  for (uint32_t k = 0; k < H.num_levels; ++k) {
+    if (sizes) sizes[k] = H.size[k];
+    if (offsets) offsets[k] = k == 0 ? 0xFFFFFFFFull : H.off[k];
+  }
+  if (buffer_floats) *buffer_floats = h->buffer_floats;
+  if (R) *R = 1u << h->R_log;
+  return RMQ_OK;
+}
+
+rmq_status rmq_gpu_base(const float* d_array, uint64_t n, const uint32_t* d_lr, uint64_t q,
+                        float* d_val, uint32_t* d_idx, void* stream) {
+  if (!d_array || n == 0 || n > kMaxN) return RMQ_EINVAL;
+  if (q == 0) return RMQ_OK;
+  if (!d_lr || (!d_val && !d_idx) || q > 0xFFFFFFFFull) return RMQ_EINVAL;
+  if ((reinterpret_cast<uintptr_t>(d_lr) & 7u) != 0) return RMQ_EINVAL;
+  const uint32_t grid = grid_for(q);
+  if (static_cast<uint64_t>(grid) * kBlock < q) return RMQ_EINVAL;
+  rmq::k_gpu_base<<<grid, kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      d_array, reinterpret_cast<const uint2*>(d_lr), static_cast<uint32_t>(q), d_val, d_idx);
+  return last_error();
+}
+
+const char* rmq_status_str(rmq_status s) {
+  switch (s) {
+    case RMQ_OK: return "ok";
+    case RMQ_EINVAL: return "invalid argument";
+    case RMQ_ETHRESHOLD: return "scan threshold too small for the reduction factor";
+    case RMQ_ENOMEM: return "out of device memory";
+    case RMQ_EDEVICE: return "HIP runtime error";
+    case RMQ_EUNSUPPORTED: return "unsupported option";
+  }
+  return "unknown status";
+}
+
+const char* rmq_version(void) { return "rmq-mi355x 0.1 gfx950"; }
+
+}  // extern "C"

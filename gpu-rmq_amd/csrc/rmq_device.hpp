@@ -1,0 +1,164 @@
+// rmq_device.hpp — device helpers for the gfx950 RMQ kernels.
+//
+// Wave64 everywhere: group reductions use DPP row operations (quad_perm, row_half_mirror,
+// row_mirror stay inside 4/8/16-lane groups of a 16-lane row), one ds_swizzle for the
+// 32-lane step and one ds_bpermute for the 64-lane step. No 32-lane warp assumptions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rmq {
+
+constexpr uint32_t kMaxLevels = 20;  // XXX_MAX_LEVELS, src/constants.cuh:14
+
+// Level metadata as seen by the kernels (the reference's xxx_metadata_type,
+// src/constants.cuh:34-39, plus the level-buffer pointers). Passed by value.
+struct HierView {
+  const float* A;        // level 0: the caller's array (borrowed)
+  const float* lv;       // level buffer, values  (levels 1..num_levels-1)
+  const uint32_t* li;    // level buffer, leftmost level-0 index of each value
+  uint32_t num_levels;
+  uint32_t size[kMaxLevels];
+  uint32_t off[kMaxLevels];
+};
+
+namespace dev {
+
+constexpr uint32_t kNoIdx = 0xFFFFFFFFu;
+#define RMQ_INF __builtin_huge_valf()
+
+__device__ __forceinline__ uint32_t f2u(float x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ float u2f(uint32_t x) { return __builtin_bit_cast(float, x); }
+
+// DPP control codes (GCN3+ encoding): quad_perm = p0 | p1<<2 | p2<<4 | p3<<6.
+enum : int {
+  kDppXor1 = 0xB1,         // quad_perm [1,0,3,2]
+  kDppXor2 = 0x4E,         // quad_perm [2,3,0,1]
+  kDppHalfMirror = 0x141,  // lane i <- lane 7-i inside each 8-lane half row
+  kDppMirror = 0x140,      // lane i <- lane 15-i inside each 16-lane row
+};
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_mov(uint32_t x) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(x), CTRL, 0xF, 0xF, false));
+}
+
+// Partner value for step `S` of a W-lane all-reduce (S = 1, 2, 4, 8, 16, 32 = xor distance,
+// realised with mirrors, which pair the same halves as xor once the smaller groups agree).
+template <int S>
+__device__ __forceinline__ uint32_t partner(uint32_t x) {
+  if constexpr (S == 1) return dpp_mov<kDppXor1>(x);
+  if constexpr (S == 2) return dpp_mov<kDppXor2>(x);
+  if constexpr (S == 4) return dpp_mov<kDppHalfMirror>(x);
+  if constexpr (S == 8) return dpp_mov<kDppMirror>(x);
+  if constexpr (S == 16)  // ds_swizzle bit mode: and 0x1F, or 0, xor 0x10
+    return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(x), 0x401F));
+  if constexpr (S == 32) return static_cast<uint32_t>(__shfl_xor(static_cast<int>(x), 32));
+  return x;
+}
+
+// Candidate order: smaller value, then smaller index. Scanning left to right with a strict
+// `<` (CPU BASE, src/cpu_methods.h:43-52) selects exactly the minimum of this order.
+__device__ __forceinline__ bool better(float v, uint32_t i, float bv, uint32_t bi) {
+  return (v < bv) || (v == bv && i < bi);
+}
+
+__device__ __forceinline__ float fmin_sel(float a, float b) { return b < a ? b : a; }
+
+template <int S>
+__device__ __forceinline__ void step_min(float& v) {
+  v = fmin_sel(v, u2f(partner<S>(f2u(v))));
+}
+
+template <int S>
+__device__ __forceinline__ void step_argmin(float& v, uint32_t& i) {
+  const float ov = u2f(partner<S>(f2u(v)));
+  const uint32_t oi = partner<S>(i);
+  const bool take = better(ov, oi, v, i);
+  v = take ? ov : v;
+  i = take ? oi : i;
+}
+
+// All-reduce over aligned groups of W lanes; every lane of the group gets the result.
+// All W lanes of a group must be active (callers keep control flow group-uniform).
+template <int W>
+__device__ __forceinline__ float group_min(float v) {
+  if constexpr (W >= 2) step_min<1>(v);
+  if constexpr (W >= 4) step_min<2>(v);
+  if constexpr (W >= 8) step_min<4>(v);
+  if constexpr (W >= 16) step_min<8>(v);
+  if constexpr (W >= 32) step_min<16>(v);
+  if constexpr (W >= 64) step_min<32>(v);
+  return v;
+}
+
+template <int W>
+__device__ __forceinline__ void group_argmin(float& v, uint32_t& i) {
+  if constexpr (W >= 2) step_argmin<1>(v, i);
+  if constexpr (W >= 4) step_argmin<2>(v, i);
+  if constexpr (W >= 8) step_argmin<4>(v, i);
+  if constexpr (W >= 16) step_argmin<8>(v, i);
+  if constexpr (W >= 32) step_argmin<16>(v, i);
+  if constexpr (W >= 64) step_argmin<32>(v, i);
+}
+
+// 16-byte load of base[i..i+4) where i % 4 == 0 and i < size; elements at or past `size`
+// read as +inf and are never touched in memory (no over-read of the caller's array).
+__device__ __forceinline__ float4 load4(const float* __restrict__ base, uint32_t i, uint32_t size) {
+  if (i + 4 <= size) return *reinterpret_cast<const float4*>(base + i);
+  float4 r = make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+  r.x = base[i];
+  if (i + 1 < size) r.y = base[i + 1];
+  if (i + 2 < size) r.z = base[i + 2];
+  return r;
+}
+
+__device__ __forceinline__ uint4 load4u(const uint32_t* __restrict__ base, uint32_t i, uint32_t size) {
+  if (i + 4 <= size) return *reinterpret_cast<const uint4*>(base + i);
+  uint4 r = make_uint4(kNoIdx, kNoIdx, kNoIdx, kNoIdx);
+  r.x = base[i];
+  if (i + 1 < size) r.y = base[i + 1];
+  if (i + 2 < size) r.z = base[i + 2];
+  return r;
+}
+
+__device__ __forceinline__ bool in_range(uint32_t p, uint32_t lo, uint32_t hi) {
+  return p - lo < hi - lo;  // lo <= p < hi, with lo <= hi
+}
+
+// Fold the elements of a float4 at position i that fall in [lo, hi) into a running minimum.
+__device__ __forceinline__ float fold4(float acc, float4 x, uint32_t i, uint32_t lo, uint32_t hi) {
+  acc = in_range(i + 0, lo, hi) ? fmin_sel(acc, x.x) : acc;
+  acc = in_range(i + 1, lo, hi) ? fmin_sel(acc, x.y) : acc;
+  acc = in_range(i + 2, lo, hi) ? fmin_sel(acc, x.z) : acc;
+  acc = in_range(i + 3, lo, hi) ? fmin_sel(acc, x.w) : acc;
+  return acc;
+}
+
+__device__ __forceinline__ void take(float& bv, uint32_t& bi, float v, uint32_t i, bool ok) {
+  const bool t = ok && better(v, i, bv, bi);
+  bv = t ? v : bv;
+  bi = t ? i : bi;
+}
+
+// Same with indices: level 0 (ix == nullptr) uses positions, upper levels read the stored
+// leftmost index of each block.
+__device__ __forceinline__ void fold4_idx(float& bv, uint32_t& bi, float4 x, const uint32_t* ix,
+                                          uint32_t i, uint32_t size, uint32_t lo, uint32_t hi) {
+  uint4 id;
+  if (ix) {
+    id = load4u(ix, i, size);
+  } else {
+    id = make_uint4(i, i + 1, i + 2, i + 3);
+  }
+  take(bv, bi, x.x, id.x, in_range(i + 0, lo, hi));
+  take(bv, bi, x.y, id.y, in_range(i + 1, lo, hi));
+  take(bv, bi, x.z, id.z, in_range(i + 2, lo, hi));
+  take(bv, bi, x.w, id.w, in_range(i + 3, lo, hi));
+}
+
+}  // namespace dev
+}  // namespace rmq

@@ -1,0 +1,224 @@
+"""rmq_amd — Python host API over librmq.so (MI355X hierarchical range-minimum queries).
+
+Device memory and streams come from PyTorch (plumbing only); every computation runs in the
+library's HIP kernels through the C ABI of include/rmq.h. Host generators reproduce the
+reference driver's seeded inputs (include/rmq_gen.h).
+
+    import rmq_amd
+    A = torch.from_numpy(rmq_amd.gen_array(n, seed)).cuda()
+    Q = torch.from_numpy(rmq_amd.gen_queries(q, -2, n, seed)).cuda()
+    with rmq_amd.Hierarchy(A, t=1 << 15, alg=16) as h:
+        val, idx = h.query(Q, want_val=True, want_idx=True)
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import (  # noqa: F401  (re-exported)
+    RMQ_ALG_GPU_BASE,
+    RMQ_ALG_VECTOR_LOAD,
+    RMQ_ALG_XXX,
+    RMQ_ALG_XXX_MULTI_LOAD,
+    RMQ_EDEVICE,
+    RMQ_EINVAL,
+    RMQ_ENOMEM,
+    RMQ_ETHRESHOLD,
+    RMQ_EUNSUPPORTED,
+    RMQ_OK,
+    RMQError,
+    BuildOptions,
+    LIB_PATH,
+    lib,
+    status_str,
+    version,
+)
+
+ALGS = (RMQ_ALG_XXX, RMQ_ALG_VECTOR_LOAD, RMQ_ALG_XXX_MULTI_LOAD)
+
+
+@dataclass
+class Plan:
+    """Level metadata (the reference's xxx_metadata_type, src/constants.cuh:34-39)."""
+
+    num_levels: int
+    sizes: List[int]
+    offsets: List[int]
+    buffer_floats: int
+    R: int
+
+
+def _options(cg_size_log, cg_amount_log, group_lanes) -> Optional[BuildOptions]:
+    if cg_size_log is None and cg_amount_log is None and not group_lanes:
+        return None
+    o = BuildOptions()
+    o.cg_size_log = 3 if cg_size_log is None else int(cg_size_log)
+    o.cg_amount_log = 2 if cg_amount_log is None else int(cg_amount_log)
+    o.group_lanes = int(group_lanes or 0)
+    o.flags = _lib.RMQ_BUILD_DEFAULT_CG
+    return o
+
+
+def _opt_ptr(o: Optional[BuildOptions]):
+    return ctypes.byref(o) if o is not None else None
+
+
+def plan(n: int, t: int, alg: int = RMQ_ALG_XXX, cg_size_log=None, cg_amount_log=None,
+         group_lanes: int = 0) -> Plan:
+    """Host-only level plan (src/xxx.h:225-243). Raises RMQError on invalid input."""
+    L = _lib.lib()
+    nl = ctypes.c_uint32()
+    sizes = (ctypes.c_uint64 * _lib.RMQ_MAX_LEVELS)()
+    offs = (ctypes.c_uint64 * _lib.RMQ_MAX_LEVELS)()
+    bf = ctypes.c_uint64()
+    R = ctypes.c_uint32()
+    o = _options(cg_size_log, cg_amount_log, group_lanes)
+    _lib.check(L.rmq_plan(n, t, alg, _opt_ptr(o), ctypes.byref(nl), sizes, offs, ctypes.byref(bf),
+                          ctypes.byref(R)), "rmq_plan")
+    k = nl.value
+    return Plan(k, list(sizes[:k]), list(offs[:k]), bf.value, R.value)
+
+
+# ---- host generators (include/rmq_gen.h) ---------------------------------------------------
+
+def gen_array(n: int, seed: int) -> np.ndarray:
+    """float32[n] from std::mt19937(seed) + uniform_real_distribution<float>(0,1)."""
+    out = np.empty(n, dtype=np.float32)
+    _lib.lib().rmq_gen_array(n, seed, out.ctypes.data)
+    return out
+
+
+def gen_queries(q: int, lr: int, n: int, seed: int, trivial_check: bool = False) -> np.ndarray:
+    """uint32[q, 2] inclusive (l, r) pairs exactly as the reference driver makes them."""
+    out = np.empty((q, 2), dtype=np.uint32)
+    st = _lib.lib().rmq_gen_queries(q, lr, n, seed, int(bool(trivial_check)), out.ctypes.data)
+    if st != 0:
+        raise ValueError(f"invalid query generator arguments: lr={lr}, n={n}")
+    return out
+
+
+def gen_sample_indices(original_size: int, sample_size: int, seed: int) -> np.ndarray:
+    out = np.empty(min(original_size, sample_size), dtype=np.uint32)
+    k = _lib.lib().rmq_gen_sample_indices(original_size, sample_size, seed, out.ctypes.data)
+    return out[:k]
+
+
+# ---- device side -----------------------------------------------------------------------------
+
+def _torch():
+    import torch  # deferred: host-only users do not need it
+
+    return torch
+
+
+def _stream_handle(stream) -> Optional[int]:
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+
+
+def _check_device_tensor(x, name: str, dtypes) -> None:
+    torch = _torch()
+    if not isinstance(x, torch.Tensor) or not x.is_cuda:
+        raise ValueError(f"{name} must be a CUDA (HIP) tensor")
+    if x.dtype not in dtypes:
+        raise ValueError(f"{name} has dtype {x.dtype}, expected one of {dtypes}")
+    if not x.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+class Hierarchy:
+    """A built hierarchy resident in HBM (rmq_build / rmq_query / rmq_free).
+
+    The array tensor is borrowed for the lifetime of the object, as the C ABI requires.
+    """
+
+    def __init__(self, array, t: int, alg: int = RMQ_ALG_XXX, cg_size_log=None,
+                 cg_amount_log=None, group_lanes: int = 0, stream=None):
+        torch = _torch()
+        _check_device_tensor(array, "array", (torch.float32,))
+        if array.dim() != 1:
+            raise ValueError("array must be 1-D")
+        self.array = array
+        self.n = array.numel()
+        self.t = int(t)
+        self.alg = int(alg)
+        self._h = ctypes.c_void_p()
+        o = _options(cg_size_log, cg_amount_log, group_lanes)
+        st = _lib.lib().rmq_build_ex(array.data_ptr(), self.n, self.t, self.alg, _opt_ptr(o),
+                                     _stream_handle(stream), ctypes.byref(self._h))
+        _lib.check(st, "rmq_build")
+
+    def query(self, lr, want_val: bool = True, want_idx: bool = False, out_val=None, out_idx=None,
+              stream=None) -> Tuple[Optional["object"], Optional["object"]]:
+        """Answer the queries lr (int32/uint32 tensor [q, 2], inclusive l, r) on the device.
+
+        Returns (values float32[q] or None, leftmost indices int32[q] or None). Indices are
+        stored as int32 tensors holding the uint32 values (n < 2^31).
+        """
+        torch = _torch()
+        if self._h.value is None:
+            raise RuntimeError("hierarchy already freed")
+        _check_device_tensor(lr, "lr", (torch.int32, torch.uint32))
+        q = lr.numel() // 2
+        if lr.numel() != 2 * q:
+            raise ValueError("lr must hold q pairs")
+        if want_val and out_val is None:
+            out_val = torch.empty(q, dtype=torch.float32, device=lr.device)
+        if want_idx and out_idx is None:
+            out_idx = torch.empty(q, dtype=torch.int32, device=lr.device)
+        vp = out_val.data_ptr() if want_val else None
+        ip = out_idx.data_ptr() if want_idx else None
+        st = _lib.lib().rmq_query(self._h, lr.data_ptr(), q, vp, ip, _stream_handle(stream))
+        _lib.check(st, "rmq_query")
+        return (out_val if want_val else None, out_idx if want_idx else None)
+
+    def metadata(self) -> Plan:
+        nl = ctypes.c_uint32()
+        sizes = (ctypes.c_uint64 * _lib.RMQ_MAX_LEVELS)()
+        offs = (ctypes.c_uint64 * _lib.RMQ_MAX_LEVELS)()
+        bf = ctypes.c_uint64()
+        R = ctypes.c_uint32()
+        _lib.check(_lib.lib().rmq_get_metadata(self._h, ctypes.byref(nl), sizes, offs,
+                                               ctypes.byref(bf), ctypes.byref(R)),
+                   "rmq_get_metadata")
+        k = nl.value
+        return Plan(k, list(sizes[:k]), list(offs[:k]), bf.value, R.value)
+
+    def close(self) -> None:
+        if self._h.value is not None:
+            _lib.lib().rmq_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gpu_base(array, lr, want_val: bool = True, want_idx: bool = False, stream=None):
+    """GPU BASE (algs 2 / 102, src/cuda_methods.cuh:4-68) through rmq_gpu_base."""
+    torch = _torch()
+    _check_device_tensor(array, "array", (torch.float32,))
+    _check_device_tensor(lr, "lr", (torch.int32, torch.uint32))
+    q = lr.numel() // 2
+    val = torch.empty(q, dtype=torch.float32, device=lr.device) if want_val else None
+    idx = torch.empty(q, dtype=torch.int32, device=lr.device) if want_idx else None
+    st = _lib.lib().rmq_gpu_base(array.data_ptr(), array.numel(), lr.data_ptr(), q,
+                                 val.data_ptr() if want_val else None,
+                                 idx.data_ptr() if want_idx else None, _stream_handle(stream))
+    _lib.check(st, "rmq_gpu_base")
+    return val, idx

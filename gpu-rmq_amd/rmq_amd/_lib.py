@@ -1,0 +1,108 @@
+"""ctypes binding of librmq.so (include/rmq.h, include/rmq_gen.h).
+
+The shared library is built in-tree by ``make -C gpu-rmq_amd`` (``__graft_entry__.build()``).
+Loading fails loudly when it is missing: there is no fallback path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "librmq.so")
+
+RMQ_OK = 0
+RMQ_EINVAL = -1
+RMQ_ETHRESHOLD = -2
+RMQ_ENOMEM = -3
+RMQ_EDEVICE = -4
+RMQ_EUNSUPPORTED = -5
+
+RMQ_ALG_GPU_BASE = 2
+RMQ_ALG_XXX = 16
+RMQ_ALG_VECTOR_LOAD = 20
+RMQ_ALG_XXX_MULTI_LOAD = 24
+RMQ_MAX_LEVELS = 20
+RMQ_BUILD_DEFAULT_CG = 0x1
+
+# Every symbol the public headers declare, with its ctypes signature.
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_i32 = ctypes.c_int32
+_vp = ctypes.c_void_p
+_pu32 = ctypes.POINTER(ctypes.c_uint32)
+_pu64 = ctypes.POINTER(ctypes.c_uint64)
+
+
+class BuildOptions(ctypes.Structure):
+    """rmq_build_options (include/rmq.h)."""
+
+    _fields_ = [
+        ("cg_size_log", ctypes.c_uint32),
+        ("cg_amount_log", ctypes.c_uint32),
+        ("group_lanes", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
+    ]
+
+
+SIGNATURES = {
+    "rmq_build": (ctypes.c_int, [_vp, _u64, _u64, ctypes.c_int, _vp, ctypes.POINTER(_vp)]),
+    "rmq_build_ex": (
+        ctypes.c_int,
+        [_vp, _u64, _u64, ctypes.c_int, ctypes.POINTER(BuildOptions), _vp, ctypes.POINTER(_vp)],
+    ),
+    "rmq_query": (ctypes.c_int, [_vp, _vp, _u64, _vp, _vp, _vp]),
+    "rmq_free": (None, [_vp]),
+    "rmq_get_metadata": (ctypes.c_int, [_vp, _pu32, _pu64, _pu64, _pu64, _pu32]),
+    "rmq_plan": (
+        ctypes.c_int,
+        [_u64, _u64, ctypes.c_int, ctypes.POINTER(BuildOptions), _pu32, _pu64, _pu64, _pu64, _pu32],
+    ),
+    "rmq_gpu_base": (ctypes.c_int, [_vp, _u64, _vp, _u64, _vp, _vp, _vp]),
+    "rmq_status_str": (ctypes.c_char_p, [ctypes.c_int]),
+    "rmq_version": (ctypes.c_char_p, []),
+    "rmq_gen_array": (None, [_u64, _i32, _vp]),
+    "rmq_gen_queries": (ctypes.c_int, [_u64, _i32, _u64, _i32, ctypes.c_int, _vp]),
+    "rmq_gen_sample_indices": (_u64, [_u64, _u64, _i32, _vp]),
+}
+
+_lib = None
+
+
+class RMQError(RuntimeError):
+    """A non-zero rmq_status from the library."""
+
+    def __init__(self, status: int, where: str):
+        self.status = status
+        super().__init__(f"{where}: {status_str(status)} (status {status})")
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"librmq.so not found at {LIB_PATH}; build it with `make -C gpu-rmq_amd` "
+                "(or __graft_entry__.build())"
+            )
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def status_str(status: int) -> str:
+    return lib().rmq_status_str(status).decode()
+
+
+def check(status: int, where: str) -> None:
+    if status != RMQ_OK:
+        raise RMQError(status, where)
+
+
+def version() -> str:
+    return lib().rmq_version().decode()

@@ -1,0 +1,52 @@
+"""Algorithmic bytes per query and the HBM roofline of the query kernel.
+
+The figure is fixed by the reference traversal at R0 = 32 (algs 16/24 at both the default
+g*c and the paper g*c), independent of the R this library runs with:
+  e = r + 1; sec(a, b) = floor(b/16) - floor(a/16) + 1 if b >= a else 0   (64-byte sectors)
+  if e - l <= 2*R0: S0 = sec(l, r)
+  else:             S0 = sec(l, ceil32(l) - 1) + sec(floor32(e), e - 1)
+  B_q = 8 (the {l, r} pair) + 4 * k_out (value or index: 1, both: 2) + 64 * S0
+These are the level-0 pieces the reference reads (src/xxx.h:163-176, 185-186). Upper levels
+are <= n/32 floats and are left out: they stay in the 256 MB Infinity Cache / L2.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+R0 = 32
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+
+
+def _sec(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    s = (b // 16) - (a // 16) + 1
+    return np.where(b >= a, s, 0)
+
+
+def level0_sectors(lr: np.ndarray) -> np.ndarray:
+    l = lr[:, 0].astype(np.int64)
+    r = lr[:, 1].astype(np.int64)
+    e = r + 1
+    short = (e - l) <= 2 * R0
+    nl = ((l + R0 - 1) // R0) * R0
+    pr = (e // R0) * R0
+    long_s = _sec(l, nl - 1) + _sec(pr, e - 1)
+    return np.where(short, _sec(l, r), long_s)
+
+
+def bytes_per_query(lr: np.ndarray, k_out: int) -> np.ndarray:
+    return 8 + 4 * k_out + 64 * level0_sectors(lr)
+
+
+def total_bytes(lr: np.ndarray, k_out: int, chunk: int = 1 << 22) -> int:
+    tot = 0
+    for s in range(0, lr.shape[0], chunk):
+        tot += int(bytes_per_query(lr[s:s + chunk], k_out).sum())
+    return tot
+
+
+def build_bytes(sizes) -> int:
+    """Each non-top level read once, each upper level written once (value + index arrays)."""
+    read = 4 * sum(sizes[:-1])
+    write = 8 * sum(sizes[1:])
+    return read + write

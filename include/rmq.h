@@ -1,0 +1,125 @@
+/*
+ * rmq.h — C ABI of the MI355X hierarchical range-minimum-query library (librmq.so).
+ *
+ * This is the drop-in boundary for GPU-RMQ's hierarchical RMQ path (algs 16, 20, 24).
+ * In the reference there is no plugin layer: `main` calls a C++ template per algorithm
+ * that allocates, builds, queries and frees in one call and returns a host buffer:
+ *   xxx_rmq              (src/xxx.h:203-298)   alg 16, "GPU-RMQ (CL)"
+ *   hierarchical_vector_load (src/xxx.h:932-1040) alg 20, "GPU-RMQ (VL)"
+ *   xxx_rmq_multi_load   (src/xxx.h:530-627)   alg 24, "CL multi load"
+ * Here build and query are split so a built hierarchy can stay resident in HBM:
+ *   rmq_build  replaces the metadata loop + build launches  (src/xxx.h:211-271)
+ *   rmq_query  replaces the query launch                    (src/xxx.h:273-285)
+ *   rmq_free   replaces the frees                           (src/xxx.h:294-296)
+ *
+ * Conventions
+ *  - All pointers named d_* are device pointers (hipMalloc / torch tensors).
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream). Every call is
+ *    stream-ordered and asynchronous; nothing on the query path allocates memory.
+ *  - Queries are `uint32_t[2*q]` = q pairs {l, r}, inclusive, 0 <= l <= r < n, exactly the
+ *    reference's uint2 layout (src/main.cu:25-26, src/rand.cuh:79-80).
+ *  - Results are exact: value = A[leftmost argmin], index = leftmost argmin under a strict
+ *    `<` scan from l to r, identical to CPU BASE (src/cpu_methods.h:7-16, 42-53).
+ *  - Errors are status codes instead of the reference's exit() (src/xxx.h:211-217).
+ *  - A built hierarchy is immutable; concurrent rmq_query calls on different streams are
+ *    allowed. The hierarchy borrows d_array: it must stay valid and unchanged until rmq_free.
+ */
+#ifndef RMQ_AMD_RMQ_H
+#define RMQ_AMD_RMQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  RMQ_OK = 0,
+  RMQ_EINVAL = -1,        /* bad pointer, size, alignment, alg or option */
+  RMQ_ETHRESHOLD = -2,    /* t < 2R (algs 16/24) or t < R (alg 20): src/xxx.h:211-217,938-944 */
+  RMQ_ENOMEM = -3,        /* device allocation failed */
+  RMQ_EDEVICE = -4,       /* HIP runtime error (launch or API) */
+  RMQ_EUNSUPPORTED = -5   /* option combination not compiled in */
+} rmq_status;
+
+/* Algorithm ids, the reference's own numbering (src/main.cu:36-53). */
+enum {
+  RMQ_ALG_GPU_BASE = 2,             /* brute-force scan per query, src/cuda_methods.cuh:4-68 */
+  RMQ_ALG_XXX = 16,                 /* GPU-RMQ (CL) */
+  RMQ_ALG_VECTOR_LOAD = 20,         /* GPU-RMQ (VL), Hierarchical_vector_load */
+  RMQ_ALG_XXX_MULTI_LOAD = 24       /* CL multi load */
+};
+
+#define RMQ_MAX_LEVELS 20           /* XXX_MAX_LEVELS, src/constants.cuh:14 */
+#define RMQ_LEVEL_ALIGNMENT 32      /* XXX_LEVEL_ALIGNMENT, src/constants.cuh:15 */
+
+typedef struct rmq_hierarchy rmq_hierarchy;
+
+/* Build options. Zero-initialise and set what you need; 0 means "reference default".
+ * cg_size_log / cg_amount_log are the reference's compile-time XXX_CG_SIZE_LOG /
+ * XXX_CG_AMOUNT_LOG (CMakeLists.txt:61-67, defaults 3 and 2). They fix the reduction
+ * factor R of the hierarchy exactly as in the reference:
+ *   algs 16/24: R = 2^(cg_size_log + cg_amount_log)      (src/xxx.h:208-209)
+ *   alg 20:     R = 2^(cg_amount_log + 2)                 (src/xxx.h:934-935)
+ * group_lanes is this library's own execution choice (lanes that cooperate on one
+ * query in algs 16/24; 0 = auto). It never changes results. */
+typedef struct {
+  uint32_t cg_size_log;     /* 0 -> 3 */
+  uint32_t cg_amount_log;   /* 0 -> 2 */
+  uint32_t group_lanes;     /* 0 -> auto; else 2, 4, 8 or 16 */
+  uint32_t flags;           /* RMQ_BUILD_* below */
+} rmq_build_options;
+
+#define RMQ_BUILD_DEFAULT_CG 0x1u   /* use cg_size_log = cg_amount_log = 0 literally */
+
+/* Build the hierarchy of block minima over d_array[0..n) on `stream`.
+ * Reference counterpart: metadata loop + per-level xxx_build_step launches,
+ * src/xxx.h:225-271 (alg 16), 552-599 (alg 24), 959-1010 (alg 20).
+ * Requires n >= 1, n <= 2^32-1, d_array 16-byte aligned, t >= 1. */
+rmq_status rmq_build(const float* d_array, uint64_t n, uint64_t t, int alg,
+                     void* stream, rmq_hierarchy** out);
+
+rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
+                        const rmq_build_options* opt, void* stream, rmq_hierarchy** out);
+
+/* Answer q queries. d_val and/or d_idx may be NULL (not both).
+ * Reference counterpart: xxx_query (src/xxx.h:124-200), xxx_query_multi_load
+ * (src/xxx.h:470-528), kernel_hierarchical_vector_load (src/xxx.h:883-930); the
+ * index output has no counterpart there and follows CPU BASE IDX (src/cpu_methods.h:42-53).
+ * d_lr must be 8-byte aligned. q = 0 is a no-op. */
+rmq_status rmq_query(const rmq_hierarchy* h, const uint32_t* d_lr, uint64_t q,
+                     float* d_val, uint32_t* d_idx, void* stream);
+
+/* Release the level buffers and the handle. Synchronises the device. NULL is a no-op. */
+void rmq_free(rmq_hierarchy* h);
+
+/* Level metadata of a built hierarchy, the reference's xxx_metadata_type
+ * (src/constants.cuh:34-39): sizes[k], offsets[k] for k < *num_levels (offsets[0] is the
+ * ~0u sentinel of src/xxx.h:229), *buffer_floats = floats in the level buffer, *R = the
+ * reduction factor. Any out pointer may be NULL. */
+rmq_status rmq_get_metadata(const rmq_hierarchy* h, uint32_t* num_levels, uint64_t* sizes,
+                            uint64_t* offsets, uint64_t* buffer_floats, uint32_t* R);
+
+/* Same metadata without building anything (host only, no device use). */
+rmq_status rmq_plan(uint64_t n, uint64_t t, int alg, const rmq_build_options* opt,
+                    uint32_t* num_levels, uint64_t* sizes, uint64_t* offsets,
+                    uint64_t* buffer_floats, uint32_t* R);
+
+/* GPU BASE (algs 2 / 102): one lane per query scans [l, r] serially with strict `<`.
+ * Reference counterpart: kernel_rmq_basic / kernel_rmq_basic_idx / value+index variant,
+ * src/cuda_methods.cuh:4-68. Used by the rtxrmq driver's --check, as in src/main.cu:194-214. */
+rmq_status rmq_gpu_base(const float* d_array, uint64_t n, const uint32_t* d_lr, uint64_t q,
+                        float* d_val, uint32_t* d_idx, void* stream);
+
+/* Human-readable name of a status code. */
+const char* rmq_status_str(rmq_status s);
+
+/* Library version string, e.g. "rmq-mi355x 0.1 gfx950". */
+const char* rmq_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RMQ_AMD_RMQ_H */

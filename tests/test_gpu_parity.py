@@ -1,0 +1,288 @@
+"""GPU parity: librmq.so's HIP kernels against the CPU oracle, bit-exact (value bit pattern and
+leftmost index), through the C ABI. Inputs come from the reference generator restatement.
+
+Sizes: every test compares the full query set against the oracle where the oracle finishes in
+seconds (CPU BASE for short ranges, the oracle's CPU hierarchy — itself pinned equal to CPU BASE
+in test_oracle.py — for long ones) plus a CPU BASE sample; the full-size runs add
+size-independent properties (index inside [l, r], value == A[index], all algorithms agree).
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import rmq_amd
+
+pytestmark = pytest.mark.gpu
+
+SEED = 27722  # SEEDS[0], scripts/perf-benchmark-exp.sh:20
+
+# (alg, extra build options): reference defaults and paper settings plus execution variants
+VARIANTS = [
+    (16, {}),
+    (24, {}),
+    (20, {}),
+    (16, dict(cg_size_log=4, cg_amount_log=1)),             # paper CL: g=16, c=2 -> R=32
+    (20, dict(cg_amount_log=1)),                             # paper VL: R=8
+    (16, dict(group_lanes=4)),
+    (16, dict(group_lanes=16)),
+    (24, dict(group_lanes=16)),
+    (16, dict(cg_size_log=3, cg_amount_log=0)),              # R=8
+    (16, dict(cg_size_log=3, cg_amount_log=3)),              # R=64
+    (24, dict(cg_size_log=4, cg_amount_log=3)),              # R=128
+    (20, dict(cg_amount_log=0)),                             # R=4
+    (20, dict(cg_amount_log=4)),                             # R=64
+]
+
+
+def _say(msg):
+    sys.stderr.write(f"[gpu-case] {msg}\n")
+    sys.stderr.flush()
+
+
+def _R(alg, opts):
+    g = opts.get("cg_size_log", 3)
+    c = opts.get("cg_amount_log", 2)
+    return 1 << (c + 2) if alg == 20 else 1 << (g + c)
+
+
+def _min_t(alg, opts):
+    return _R(alg, opts) * (1 if alg == 20 else 2)
+
+
+def run_gpu(A, lr, alg, t, want_val=True, want_idx=True, **opts):
+    import torch
+
+    dA = torch.from_numpy(np.ascontiguousarray(A)).cuda()
+    dQ = torch.from_numpy(np.ascontiguousarray(lr).view(np.int32)).cuda()
+    with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
+        v, i = h.query(dQ, want_val=want_val, want_idx=want_idx)
+        torch.cuda.synchronize()
+    v = v.cpu().numpy() if v is not None else None
+    i = i.cpu().numpy().view(np.uint32) if i is not None else None
+    return v, i
+
+
+def assert_exact(A, lr, v, i, ref_i, what):
+    if i is not None:
+        bad = np.nonzero(i != ref_i)[0]
+        assert bad.size == 0, f"{what}: {bad.size} index mismatches, first q={bad[:5]} " \
+                              f"lr={lr[bad[:3]].tolist()} got={i[bad[:3]]} want={ref_i[bad[:3]]}"
+    if v is not None:
+        want = A[ref_i].view(np.uint32)
+        bad = np.nonzero(v.view(np.uint32) != want)[0]
+        assert bad.size == 0, f"{what}: {bad.size} value mismatches, first q={bad[:5]}"
+
+
+def edge_queries(n, R):
+    pairs = {(0, n - 1), (0, 0), (n - 1, n - 1)}
+    for L in {1, 2, 3, 4, 5, R - 1, R, R + 1, 2 * R - 1, 2 * R, 2 * R + 1, 2 * R + 2, 3 * R,
+              4 * R + 3, R * R, R * R + 1, 2 * R * R + 5}:
+        if 1 <= L <= n:
+            for l in {0, 1, 3, R - 1, R, R + 1, n - L, max(0, n - L - 1), (n - L) // 2}:
+                if 0 <= l <= n - L:
+                    pairs.add((l, l + L - 1))
+    return np.array(sorted(pairs), dtype=np.uint32)
+
+
+@pytest.mark.parametrize("alg,opts", VARIANTS, ids=lambda x: str(x))
+def test_small_edge_cases(device, alg, opts):
+    rng = np.random.default_rng(1)
+    R = _R(alg, opts)
+    tmin = _min_t(alg, opts)
+    for n in (1, 2, 3, 5, 33, 1001, 4099, 20000, 65536 + 7):
+        for t in sorted({tmin, 4 * tmin, 1 << 15}):
+            for kind in ("uniform", "ties", "zeros"):
+                if kind == "uniform":
+                    A = rng.random(n, dtype=np.float32)
+                elif kind == "ties":
+                    A = rng.integers(0, 3, size=n).astype(np.float32)
+                else:
+                    A = rng.integers(0, 2, size=n).astype(np.float32)
+                    A[rng.random(n) < 0.5] *= np.float32(-1.0)  # mixes +0 and -0
+                rnd = np.sort(rng.integers(0, n, size=(300, 2)), axis=1).astype(np.uint32)
+                lr = np.concatenate([edge_queries(n, R), rnd])
+                ref_i = oracle.cpu_base_index(A, lr)
+                _say(f"small alg={alg} opts={opts} n={n} t={t} kind={kind} q={len(lr)}")
+                v, i = run_gpu(A, lr, alg, t, True, True, **opts)
+                assert_exact(A, lr, v, i, ref_i, f"alg={alg} n={n} t={t} {kind} both")
+                v, _ = run_gpu(A, lr, alg, t, True, False, **opts)
+                assert_exact(A, lr, v, None, ref_i, f"alg={alg} n={n} t={t} {kind} value")
+                _, i = run_gpu(A, lr, alg, t, False, True, **opts)
+                assert_exact(A, lr, None, i, ref_i, f"alg={alg} n={n} t={t} {kind} index")
+
+
+def test_tie_vectors(device):
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tie_vectors.json")
+    with open(path) as f:
+        cases = json.load(f)
+    for case in cases:
+        A = np.array(case["array_bits"], dtype=np.uint32).view(np.float32)
+        lr = np.array(case["lr"], dtype=np.uint32)
+        for alg, opts in VARIANTS:
+            t = _min_t(alg, opts)
+            _say(f"tie_vectors {case['name']} alg={alg} opts={opts}")
+            v, i = run_gpu(A, lr, alg, t, True, True, **opts)
+            assert i.tolist() == case["index"], (case["name"], alg, opts)
+            assert v.view(np.uint32).tolist() == case["value_bits"], (case["name"], alg, opts)
+            v, _ = run_gpu(A, lr, alg, t, True, False, **opts)
+            assert v.view(np.uint32).tolist() == case["value_bits"], (case["name"], alg, opts)
+
+
+def test_gpu_base_matches_cpu_base(device):
+    rng = np.random.default_rng(4)
+    import torch
+
+    for n in (1, 77, 5000):
+        A = rng.integers(0, 5, size=n).astype(np.float32)
+        lr = np.sort(rng.integers(0, n, size=(2000, 2)), axis=1).astype(np.uint32)
+        _say(f"gpu_base n={n}")
+        v, i = rmq_amd.gpu_base(torch.from_numpy(A).cuda(), torch.from_numpy(lr.view(np.int32)).cuda(),
+                                True, True)
+        torch.cuda.synchronize()
+        ref_i = oracle.cpu_base_index(A, lr)
+        assert np.array_equal(i.cpu().numpy().view(np.uint32), ref_i)
+        assert np.array_equal(v.cpu().numpy().view(np.uint32), A[ref_i].view(np.uint32))
+
+
+def test_config1_plain_value(device):
+    """n = 2^20, q = 2^20, lr = -1 (uniform large), t = 2^15, value mode."""
+    n = q = 1 << 20
+    A = rmq_amd.gen_array(n, SEED)
+    lr = rmq_amd.gen_queries(q, -1, n, SEED)
+    ref_v, ref_i = oracle.CpuHierarchy(A, 1 << 15, 5, 16).query(lr)
+    sample = rmq_amd.gen_sample_indices(q, 1 << 12, SEED)
+    assert np.array_equal(oracle.cpu_base_index(A, lr[sample]), ref_i[sample])
+    for alg, opts in VARIANTS[:5]:
+        _say(f"config1 alg={alg} opts={opts}")
+        v, _ = run_gpu(A, lr, alg, 1 << 15, True, False, **opts)
+        assert_exact(A, lr, v, None, ref_i, f"config1 alg={alg} {opts}")
+
+
+@pytest.mark.parametrize("t_log", [10, 15, 18])
+def test_config2_index_sampled_full_n(device, t_log):
+    """n = 2^26, lr = -2 (lognormal medium), index mode, t swept; 2^22 queries vs the oracle."""
+    n = 1 << 26
+    A = _array_2_26()
+    lr = _queries_2_26()[: 1 << 22]
+    ref_v, ref_i = oracle.CpuHierarchy(A, 1 << 15, 5, 16).query(lr)
+    for alg, opts in ((16, {}), (24, {}), (20, {})):
+        _say(f"config2 alg={alg} t=2^{t_log}")
+        v, i = run_gpu(A, lr, alg, 1 << t_log, True, True, **opts)
+        assert_exact(A, lr, v, i, ref_i, f"config2 alg={alg} t=2^{t_log}")
+    sample = rmq_amd.gen_sample_indices(lr.shape[0], 1 << 13, SEED)
+    assert np.array_equal(oracle.cpu_base_index(A, lr[sample]), ref_i[sample])
+    assert n == A.size
+
+
+def test_config2_full_q_properties(device):
+    """Full n = q = 2^26: every algorithm gives the same answer; answers are inside [l, r]."""
+    import torch
+
+    A = _array_2_26()
+    lr = _queries_2_26()
+    dA = torch.from_numpy(A).cuda()
+    dQ = torch.from_numpy(lr.view(np.int32)).cuda()
+    outs = {}
+    for alg in (16, 24, 20):
+        _say(f"config2 full q alg={alg}")
+        with rmq_amd.Hierarchy(dA, 1 << 15, alg) as h:
+            v, i = h.query(dQ, True, True)
+            torch.cuda.synchronize()
+        outs[alg] = (v.cpu().numpy(), i.cpu().numpy().view(np.uint32))
+    v16, i16 = outs[16]
+    for alg in (24, 20):
+        assert np.array_equal(outs[alg][1], i16)
+        assert np.array_equal(outs[alg][0].view(np.uint32), v16.view(np.uint32))
+    assert (i16 >= lr[:, 0]).all() and (i16 <= lr[:, 1]).all()
+    assert np.array_equal(A[i16].view(np.uint32), v16.view(np.uint32))
+    sample = rmq_amd.gen_sample_indices(lr.shape[0], 1 << 13, SEED + 1)
+    assert np.array_equal(oracle.cpu_base_index(A, lr[sample]), i16[sample])
+
+
+_CACHE = {}
+
+
+def _array_2_26():
+    if "A26" not in _CACHE:
+        _CACHE["A26"] = rmq_amd.gen_array(1 << 26, SEED)
+    return _CACHE["A26"]
+
+
+def _queries_2_26():
+    if "Q26" not in _CACHE:
+        _CACHE["Q26"] = rmq_amd.gen_queries(1 << 26, -2, 1 << 26, SEED)
+    return _CACHE["Q26"]
+
+
+@pytest.mark.parametrize("lr_mode", [-3, -6])
+def test_config3_large_array(device, lr_mode):
+    """n = 2^30 (4 GiB), lr = -3 and -6, value + index; 2^20 queries vs the oracle hierarchy."""
+    import torch
+
+    n = 1 << 30
+    if "A30" not in _CACHE:
+        _CACHE.clear()
+        _CACHE["A30"] = rmq_amd.gen_array(n, SEED)
+    A = _CACHE["A30"]
+    lr = rmq_amd.gen_queries(1 << 20, lr_mode, n, SEED)
+    ref_v, ref_i = oracle.CpuHierarchy(A, 64, 5, 16).query(lr)
+    dA = torch.from_numpy(A).cuda()
+    dQ = torch.from_numpy(lr.view(np.int32)).cuda()
+    for alg, t in ((16, 64), (24, 1 << 15), (20, 32)):
+        _say(f"config3 lr={lr_mode} alg={alg} t={t}")
+        opts = dict(cg_amount_log=1) if alg == 20 else {}
+        with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
+            v, i = h.query(dQ, True, True)
+            torch.cuda.synchronize()
+        assert_exact(A, lr, v.cpu().numpy(), i.cpu().numpy().view(np.uint32), ref_i,
+                     f"config3 lr={lr_mode} alg={alg}")
+    del dA
+    torch.cuda.empty_cache()
+
+
+def test_streams_and_reuse(device):
+    """One resident hierarchy, queried concurrently on two streams and repeatedly."""
+    import torch
+
+    n = 1 << 22
+    A = rmq_amd.gen_array(n, 5)
+    lr1 = rmq_amd.gen_queries(1 << 18, -6, n, 5)
+    lr2 = rmq_amd.gen_queries(1 << 18, -1, n, 5)
+    ref1 = oracle.CpuHierarchy(A, 64, 5, 16).query(lr1)[1]
+    ref2 = oracle.CpuHierarchy(A, 64, 5, 16).query(lr2)[1]
+    dA = torch.from_numpy(A).cuda()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    with rmq_amd.Hierarchy(dA, 64, 16) as h:
+        torch.cuda.synchronize()
+        q1 = torch.from_numpy(lr1.view(np.int32)).cuda()
+        q2 = torch.from_numpy(lr2.view(np.int32)).cuda()
+        torch.cuda.synchronize()
+        _say("streams: concurrent queries")
+        for _ in range(3):
+            _, i1 = h.query(q1, False, True, stream=s1)
+            _, i2 = h.query(q2, False, True, stream=s2)
+        torch.cuda.synchronize()
+        assert np.array_equal(i1.cpu().numpy().view(np.uint32), ref1)
+        assert np.array_equal(i2.cpu().numpy().view(np.uint32), ref2)
+
+
+def test_error_paths_on_device(device):
+    import torch
+
+    dA = torch.rand(1000, device="cuda")
+    with pytest.raises(rmq_amd.RMQError) as e:
+        rmq_amd.Hierarchy(dA, 63, 16)
+    assert e.value.status == rmq_amd.RMQ_ETHRESHOLD
+    with rmq_amd.Hierarchy(dA, 64, 16) as h:
+        empty = torch.empty((0, 2), dtype=torch.int32, device="cuda")
+        v, i = h.query(empty, True, True)
+        assert v.numel() == 0 and i.numel() == 0
+        L = rmq_amd.lib()
+        q = torch.zeros((4, 2), dtype=torch.int32, device="cuda")
+        assert L.rmq_query(h._h, q.data_ptr(), 4, None, None, None) == rmq_amd.RMQ_EINVAL
