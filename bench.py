@@ -44,7 +44,24 @@ def parse():
     ap.add_argument("--mode", choices=["index", "value", "both"], default="index")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-sweep", action="store_true", help="time the default alg 16, t=2^15")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (calibrates rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE against known bytes)")
     return ap.parse_args()
+
+
+def copy_probe():
+    import torch
+
+    nbytes = 1 << 30
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
+                      "launches": 5}), flush=True)
 
 
 def log(msg):
@@ -114,6 +131,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "design is described in DESIGN.md and is not run.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import numpy as np
     import torch
