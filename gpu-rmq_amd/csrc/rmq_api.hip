@@ -16,11 +16,38 @@
 using rmq::HierView;
 using rmq::kBlock;
 
+namespace rmq {
+// ---------------------------------------------------------------------------------------
+// GPU BASE (kernel_rmq_basic / _idx / value+index, src/cuda_methods.cuh:4-68): one lane per
+// query, serial strict-`<` scan from l to r. The reference's --check oracle.
+__global__ __launch_bounds__(kBlock) void k_gpu_base(const float* __restrict__ A,
+                                                     const uint2* __restrict__ Q, uint32_t q,
+                                                     float* __restrict__ out_v,
+                                                     uint32_t* __restrict__ out_i) {
+  const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (qi >= q) return;
+  const uint2 p = Q[qi];
+  float m = A[p.x];
+  uint32_t mi = p.x;
+  for (uint32_t i = p.x; i <= p.y; ++i) {
+    const float x = A[i];
+    if (x < m) {
+      m = x;
+      mi = i;
+    }
+  }
+  if (out_v) out_v[qi] = m;
+  if (out_i) out_i[qi] = mi;
+}
+
+}  // namespace rmq
+
 struct rmq_hierarchy {
   int alg;
   int device;
   uint32_t R_log;
   uint32_t group_lanes;
+  uint32_t flags;
   uint64_t n;
   uint64_t t;
   uint64_t buffer_floats;
@@ -133,41 +160,50 @@ rmq_status build_level(uint32_t R_log, const float* sv, const uint32_t* si, uint
 }
 
 // ---- query dispatch -----------------------------------------------------------------------
-template <int G, int RL>
+// P = pruned evaluation order (default) or every piece read (RMQ_BUILD_FULL).
+template <int G, int RL, bool P>
 rmq_status launch_cl(int alg, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
                      const HierView& H, hipStream_t s) {
   const bool idx = oi != nullptr;
   if (alg == RMQ_ALG_XXX) {
     const uint32_t grid = grid_for(q);
-    if (grid == kMaxGrid && static_cast<uint64_t>(q) > kMaxGrid * kBlock) return RMQ_EINVAL;
-    if (idx) rmq::k_query_xxx<G, RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-    else rmq::k_query_xxx<G, RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    if (static_cast<uint64_t>(grid) * kBlock < q) return RMQ_EINVAL;
+    if (idx) rmq::k_query_xxx<G, RL, P, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_xxx<G, RL, P, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
   } else {
     const uint32_t grid = grid_for(static_cast<uint64_t>(q) * G);
-    if (idx) rmq::k_query_multi<G, RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-    else rmq::k_query_multi<G, RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    if (idx) rmq::k_query_multi<G, RL, P, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_multi<G, RL, P, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
   }
   return last_error();
 }
 
 template <int RL>
-rmq_status launch_cl_g(uint32_t G, int alg, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
-                       const HierView& H, hipStream_t s) {
+rmq_status launch_cl_g(uint32_t G, bool full, int alg, const uint2* Q, uint32_t q, float* ov,
+                       uint32_t* oi, const HierView& H, hipStream_t s) {
   switch (G) {
-    case 4: return launch_cl<4, RL>(alg, Q, q, ov, oi, H, s);
-    case 8: return launch_cl<8, RL>(alg, Q, q, ov, oi, H, s);
-    case 16: return launch_cl<16, RL>(alg, Q, q, ov, oi, H, s);
+    case 4: return full ? launch_cl<4, RL, false>(alg, Q, q, ov, oi, H, s)
+                        : launch_cl<4, RL, true>(alg, Q, q, ov, oi, H, s);
+    case 8: return full ? launch_cl<8, RL, false>(alg, Q, q, ov, oi, H, s)
+                        : launch_cl<8, RL, true>(alg, Q, q, ov, oi, H, s);
+    case 16: return full ? launch_cl<16, RL, false>(alg, Q, q, ov, oi, H, s)
+                         : launch_cl<16, RL, true>(alg, Q, q, ov, oi, H, s);
     default: return RMQ_EUNSUPPORTED;
   }
 }
 
 template <int RL>
-rmq_status launch_vl(const uint2* Q, uint32_t q, float* ov, uint32_t* oi, const HierView& H,
-                     hipStream_t s) {
+rmq_status launch_vl(bool full, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
+                     const HierView& H, hipStream_t s) {
   const uint32_t grid = grid_for(q);
   if (static_cast<uint64_t>(grid) * kBlock < q) return RMQ_EINVAL;
-  if (oi) rmq::k_query_vl<RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-  else rmq::k_query_vl<RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  if (full) {
+    if (oi) rmq::k_query_vl<RL, false, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_vl<RL, false, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  } else {
+    if (oi) rmq::k_query_vl<RL, true, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_vl<RL, true, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  }
   return last_error();
 }
 
@@ -176,20 +212,20 @@ rmq_status query_dispatch(const rmq_hierarchy* h, const uint2* Q, uint32_t q, fl
   const HierView& H = h->view;
   if (h->alg == RMQ_ALG_VECTOR_LOAD) {
     switch (h->R_log) {
-      case 2: return launch_vl<2>(Q, q, ov, oi, H, s);
-      case 3: return launch_vl<3>(Q, q, ov, oi, H, s);
-      case 4: return launch_vl<4>(Q, q, ov, oi, H, s);
-      case 5: return launch_vl<5>(Q, q, ov, oi, H, s);
-      case 6: return launch_vl<6>(Q, q, ov, oi, H, s);
+      case 2: return launch_vl<2>((h->flags & RMQ_BUILD_FULL) != 0, Q, q, ov, oi, H, s);
+      case 3: return launch_vl<3>((h->flags & RMQ_BUILD_FULL) != 0, Q, q, ov, oi, H, s);
+      case 4: return launch_vl<4>((h->flags & RMQ_BUILD_FULL) != 0, Q, q, ov, oi, H, s);
+      case 5: return launch_vl<5>((h->flags & RMQ_BUILD_FULL) != 0, Q, q, ov, oi, H, s);
+      case 6: return launch_vl<6>((h->flags & RMQ_BUILD_FULL) != 0, Q, q, ov, oi, H, s);
       default: return RMQ_EUNSUPPORTED;
     }
   }
   switch (h->R_log) {
-    case 3: return launch_cl_g<3>(h->group_lanes, h->alg, Q, q, ov, oi, H, s);
-    case 4: return launch_cl_g<4>(h->group_lanes, h->alg, Q, q, ov, oi, H, s);
-    case 5: return launch_cl_g<5>(h->group_lanes, h->alg, Q, q, ov, oi, H, s);
-    case 6: return launch_cl_g<6>(h->group_lanes, h->alg, Q, q, ov, oi, H, s);
-    case 7: return launch_cl_g<7>(h->group_lanes, h->alg, Q, q, ov, oi, H, s);
+    case 3: return launch_cl_g<3>(h->group_lanes, (h->flags & RMQ_BUILD_FULL) != 0, h->alg, Q, q, ov, oi, H, s);
+    case 4: return launch_cl_g<4>(h->group_lanes, (h->flags & RMQ_BUILD_FULL) != 0, h->alg, Q, q, ov, oi, H, s);
+    case 5: return launch_cl_g<5>(h->group_lanes, (h->flags & RMQ_BUILD_FULL) != 0, h->alg, Q, q, ov, oi, H, s);
+    case 6: return launch_cl_g<6>(h->group_lanes, (h->flags & RMQ_BUILD_FULL) != 0, h->alg, Q, q, ov, oi, H, s);
+    case 7: return launch_cl_g<7>(h->group_lanes, (h->flags & RMQ_BUILD_FULL) != 0, h->alg, Q, q, ov, oi, H, s);
     default: return RMQ_EUNSUPPORTED;
   }
 }
@@ -232,6 +268,7 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   }
   h->R_log = P.R_log;
   h->group_lanes = P.G;
+  h->flags = opt ? opt->flags : 0u;
   h->n = n;
   h->t = t;
   h->buffer_floats = P.buffer_floats;

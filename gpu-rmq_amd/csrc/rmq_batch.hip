@@ -1,0 +1,498 @@
+// rmq_batch.hip — batched form (include/rmq_batch.h): many arrays, one build launch per level,
+// one persistent query launch fed by device-side work queues, HIP-graph capture.
+//
+// Device layout: every array gets a HierView (the same level metadata the plain kernels read,
+// rmq_device.hpp) in a table in global memory; its level buffers are slices of two arenas
+// (values, leftmost indices). Tiles are 256 queries of one array.
+//
+// XCD-aware scheduling: workgroups are dispatched round-robin over the 8 XCDs of an MI355X,
+// so workgroup b runs on XCD b % 8. Arrays are assigned to 8 queues (greedy by query count)
+// and workgroups pull from the queue of their own XCD first, so an array's upper levels are
+// cached in one XCD's L2; an idle workgroup then steals from the other queues. Correctness does
+// not depend on the dispatch order: every tile is taken exactly once through an atomic counter.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "../../include/rmq_batch.h"
+#include "rmq_kernels.hpp"
+
+using rmq::HierView;
+using rmq::kBlock;
+
+namespace {
+
+constexpr uint32_t kRLog = 5;            // R = 32 for every array of a batch
+constexpr uint32_t kR = 1u << kRLog;
+constexpr uint32_t kQueues = 8;          // XCDs of an MI355X
+constexpr uint32_t kTileQueries = kBlock;
+constexpr uint32_t kBuildTile = 1024;    // level-(k+1) entries per build tile
+
+struct BuildTile {
+  uint32_t array;
+  uint32_t level;  // source level k
+  uint32_t j0, j1; // destination entries [j0, j1) of level k+1
+};
+
+struct QueryTile {
+  uint32_t array;
+  uint32_t count;  // queries in the tile (<= 256)
+  uint64_t q0;     // first global query index
+};
+
+}  // namespace
+
+struct rmq_batch {
+  uint32_t count;
+  int device;
+  HierView* d_views;
+  uint32_t* d_variant;
+  float* lv;
+  uint32_t* li;
+  uint64_t level_floats;
+  std::vector<uint32_t> variant;  // host copy
+};
+
+struct rmq_batch_plan {
+  uint32_t ntiles;
+  QueryTile* d_tiles;
+  uint32_t* d_qstart;   // [kQueues + 1] tile index where each queue starts
+  uint32_t* d_counter;  // [kQueues] next tile of each queue
+};
+
+struct rmq_graph {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+namespace rmq {
+
+// ---- synthetic data --------------------------------------------------------------------------
+__device__ __forceinline__ float splitmix_uniform(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + i * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return static_cast<float>(static_cast<uint32_t>(z >> 40)) * (1.0f / 16777216.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_uniform(float* __restrict__ out, uint64_t n,
+                                                         uint64_t seed, uint64_t start) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+    out[i] = splitmix_uniform(seed, start + i);
+  }
+}
+
+// ---- build: one launch per source level, one workgroup per tile -----------------------------
+__global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restrict__ views,
+                                                        const BuildTile* __restrict__ tiles) {
+  constexpr uint32_t W = kR / 4;  // 8 lanes per 32-element block
+  const BuildTile T = tiles[blockIdx.x];
+  const HierView& H = views[T.array];
+  const uint32_t k = T.level;
+  const float* sv = k == 0 ? H.A : H.lv + H.off[k];
+  const uint32_t* si = k == 0 ? nullptr : H.li + H.off[k];
+  const uint32_t ssize = H.size[k];
+  float* dv = const_cast<float*>(H.lv) + H.off[k + 1];
+  uint32_t* di = const_cast<uint32_t*>(H.li) + H.off[k + 1];
+  const uint32_t gl = threadIdx.x & (W - 1);
+  for (uint32_t j = T.j0 + threadIdx.x / W; j < T.j1; j += kBlock / W) {
+    float bv = RMQ_INF;
+    uint32_t bi = dev::kNoIdx;
+    const uint32_t pos = j * kR + 4 * gl;
+    if (pos < ssize) {
+      const float4 x = dev::load4(sv, pos, ssize);
+      dev::fold4_idx(bv, bi, x, si, pos, ssize, pos, ssize);
+    }
+    dev::group_argmin<W>(bv, bi);
+    if (gl == 0) {
+      dv[j] = bv;
+      di[j] = bi;
+    }
+  }
+}
+
+__global__ void k_reset_counters(uint32_t* __restrict__ c) {
+  if (threadIdx.x < kQueues) c[threadIdx.x] = 0;
+}
+
+// ---- query: persistent, tiles pulled from per-XCD queues ----------------------------------------
+template <int G, bool IDX>
+__device__ __forceinline__ void tile_groups(const HierView& H, const uint2* __restrict__ Q,
+                                            uint64_t q0, uint32_t count, float* __restrict__ ov,
+                                            uint32_t* __restrict__ oi) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t gl = lane & (G - 1);
+  const uint32_t local = threadIdx.x;  // one query per lane, 256 per tile
+  const bool valid = local < count;
+  uint32_t l = 0, e = 0;
+  if (valid) {
+    const uint2 p = dev::load_query(Q, q0 + local);
+    l = p.x;
+    e = p.y + 1;
+  }
+  float my_v = RMQ_INF;
+  uint32_t my_i = dev::kNoIdx;
+  const int gbase = static_cast<int>(lane & ~(G - 1));
+  for (int k = 0; k < G; ++k) {
+    const int src = gbase + k;
+    const uint32_t lk = static_cast<uint32_t>(__shfl(static_cast<int>(l), src));
+    const uint32_t ek = static_cast<uint32_t>(__shfl(static_cast<int>(e), src));
+    const int vk = __shfl(static_cast<int>(valid), src);
+    if (vk) {
+      float v;
+      uint32_t i = dev::kNoIdx;
+      if constexpr (IDX) {
+        const GroupAnswer a = solve_cl_pruned<G, kRLog, true>(lk, ek, gl, H);
+        v = a.v;
+        i = a.idx;
+      } else {
+        v = value_exact<G, kRLog, true>(lk, ek, gl, H);
+      }
+      if (gl == static_cast<uint32_t>(k)) {
+        my_v = v;
+        my_i = i;
+      }
+    }
+  }
+  if (valid) {
+    if (ov) ov[q0 + local] = my_v;
+    if (IDX && oi) oi[q0 + local] = my_i;
+  }
+}
+
+template <bool IDX>
+__device__ __forceinline__ void tile_lanes(const HierView& H, const uint2* __restrict__ Q,
+                                           uint64_t q0, uint32_t count, float* __restrict__ ov,
+                                           uint32_t* __restrict__ oi) {
+  const uint32_t local = threadIdx.x;
+  if (local >= count) return;
+  const uint2 p = dev::load_query(Q, q0 + local);
+  GroupAnswer a = solve_vl_pruned<kRLog, IDX>(p.x, p.y + 1, H);
+  if constexpr (!IDX) {
+    if (a.v == 0.0f) a = solve_vl_pruned<kRLog, true>(p.x, p.y + 1, H);
+  }
+  if (ov) ov[q0 + local] = a.v;
+  if (IDX && oi) oi[q0 + local] = a.idx;
+}
+
+template <bool IDX>
+__global__ __launch_bounds__(kBlock) void k_query_batch(
+    const HierView* __restrict__ views, const uint32_t* __restrict__ variant,
+    const QueryTile* __restrict__ tiles, const uint32_t* __restrict__ qstart,
+    uint32_t* __restrict__ counter, const uint2* __restrict__ Q, float* __restrict__ ov,
+    uint32_t* __restrict__ oi) {
+  __shared__ uint32_t s_tile;
+  const uint32_t home = blockIdx.x % kQueues;
+  uint32_t queue = home;
+  uint32_t tried = 0;
+  while (tried < kQueues) {
+    if (threadIdx.x == 0) {
+      const uint32_t len = qstart[queue + 1] - qstart[queue];
+      const uint32_t t = atomicAdd(&counter[queue], 1u);
+      s_tile = t < len ? qstart[queue] + t : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    __syncthreads();
+    if (tile == 0xFFFFFFFFu) {  // this queue is drained: move to the next one
+      queue = (queue + 1) % kQueues;
+      ++tried;
+      continue;
+    }
+    const QueryTile T = tiles[tile];
+    const uint32_t a = __builtin_amdgcn_readfirstlane(T.array);
+    const HierView& H = views[a];
+    switch (variant[a]) {
+      case RMQ_VARIANT_GROUP4: tile_groups<4, IDX>(H, Q, T.q0, T.count, ov, oi); break;
+      case RMQ_VARIANT_GROUP16: tile_groups<16, IDX>(H, Q, T.q0, T.count, ov, oi); break;
+      case RMQ_VARIANT_LANE: tile_lanes<IDX>(H, Q, T.q0, T.count, ov, oi); break;
+      default: tile_groups<8, IDX>(H, Q, T.q0, T.count, ov, oi); break;
+    }
+  }
+}
+
+}  // namespace rmq
+
+namespace {
+
+uint32_t persistent_grid(const void* kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1)
+    per_cu = 2;
+  const uint32_t g = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu);
+  return (g + kQueues - 1) / kQueues * kQueues;
+}
+
+rmq_status err() { return hipGetLastError() == hipSuccess ? RMQ_OK : RMQ_EDEVICE; }
+
+template <typename T>
+rmq_status dmalloc(T** p, size_t count) {
+  if (hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return RMQ_ENOMEM;
+  }
+  return RMQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rmq_status rmq_fill_uniform(float* d_out, uint64_t n, uint64_t seed, uint64_t start, void* stream) {
+  if (!d_out) return RMQ_EINVAL;
+  if (n == 0) return RMQ_OK;
+  uint64_t blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 65536) blocks = 65536;
+  rmq::k_fill_uniform<<<static_cast<uint32_t>(blocks), kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      d_out, n, seed, start);
+  return err();
+}
+
+rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays, uint32_t count,
+                           void* stream, rmq_batch** out) {
+  if (!out) return RMQ_EINVAL;
+  *out = nullptr;
+  if (!d_arrays || !arrays || count == 0) return RMQ_EINVAL;
+  if ((reinterpret_cast<uintptr_t>(d_arrays) & 15u) != 0) return RMQ_EINVAL;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+
+  // per-array plans (src/xxx.h:225-243 at R = 32) and arena offsets
+  std::vector<HierView> views(count);
+  std::vector<uint64_t> lv_off(count);
+  uint64_t arena = 0;
+  uint32_t max_levels = 1;
+  for (uint32_t a = 0; a < count; ++a) {
+    const rmq_batch_array& d = arrays[a];
+    if (d.n == 0 || d.n > 0x7FFFFFFFu || (d.offset & 3u) != 0) return RMQ_EINVAL;
+    if (d.t < 2 * kR) return RMQ_ETHRESHOLD;
+    if (d.variant != RMQ_VARIANT_GROUP4 && d.variant != RMQ_VARIANT_GROUP8 &&
+        d.variant != RMQ_VARIANT_GROUP16 && d.variant != RMQ_VARIANT_LANE)
+      return RMQ_EUNSUPPORTED;
+    uint32_t nl = 0;
+    uint64_t sizes[RMQ_MAX_LEVELS], offs[RMQ_MAX_LEVELS], floats = 0;
+    rmq_build_options o{3, 2, 0, 0};
+    const rmq_status st = rmq_plan(d.n, d.t, RMQ_ALG_XXX, &o, &nl, sizes, offs, &floats, nullptr);
+    if (st != RMQ_OK) return st;
+    HierView& H = views[a];
+    std::memset(&H, 0, sizeof(H));
+    H.num_levels = nl;
+    for (uint32_t k = 0; k < nl; ++k) {
+      H.size[k] = static_cast<uint32_t>(sizes[k]);
+      H.off[k] = k == 0 ? 0u : static_cast<uint32_t>(offs[k]);
+    }
+    lv_off[a] = arena;
+    arena += (floats + 31) / 32 * 32;
+    max_levels = std::max(max_levels, nl);
+  }
+
+  rmq_batch* b = new (std::nothrow) rmq_batch();
+  if (!b) return RMQ_ENOMEM;
+  b->count = count;
+  b->level_floats = arena;
+  (void)hipGetDevice(&b->device);
+  rmq_status st = dmalloc(&b->lv, arena);
+  if (st == RMQ_OK) st = dmalloc(&b->li, arena);
+  if (st == RMQ_OK) st = dmalloc(&b->d_views, count);
+  if (st == RMQ_OK) st = dmalloc(&b->d_variant, count);
+  if (st != RMQ_OK) {
+    rmq_batch_free(b);
+    return st;
+  }
+  b->variant.resize(count);
+  for (uint32_t a = 0; a < count; ++a) {
+    HierView& H = views[a];
+    H.A = d_arrays + arrays[a].offset;
+    H.lv = b->lv + lv_off[a];
+    H.li = b->li + lv_off[a];
+    b->variant[a] = arrays[a].variant;
+  }
+
+  // build tiles, grouped by source level
+  std::vector<BuildTile> tiles;
+  std::vector<uint32_t> level_start(max_levels, 0);
+  for (uint32_t k = 0; k + 1 < max_levels; ++k) {
+    level_start[k] = static_cast<uint32_t>(tiles.size());
+    for (uint32_t a = 0; a < count; ++a) {
+      if (views[a].num_levels <= k + 1) continue;
+      const uint32_t dsize = views[a].size[k + 1];
+      for (uint32_t j = 0; j < dsize; j += kBuildTile)
+        tiles.push_back(BuildTile{a, k, j, std::min(dsize, j + kBuildTile)});
+    }
+  }
+  if (max_levels >= 1) level_start[max_levels - 1] = static_cast<uint32_t>(tiles.size());
+  BuildTile* d_tiles = nullptr;
+  if (!tiles.empty()) {
+    st = dmalloc(&d_tiles, tiles.size());
+    if (st != RMQ_OK) {
+      rmq_batch_free(b);
+      return st;
+    }
+  }
+  bool ok = hipMemcpyAsync(b->d_views, views.data(), count * sizeof(HierView), hipMemcpyHostToDevice, s) == hipSuccess &&
+            hipMemcpyAsync(b->d_variant, b->variant.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice, s) == hipSuccess;
+  if (ok && !tiles.empty())
+    ok = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(BuildTile), hipMemcpyHostToDevice, s) == hipSuccess;
+  for (uint32_t k = 0; ok && k + 1 < max_levels; ++k) {
+    const uint32_t t0 = level_start[k], t1 = level_start[k + 1];
+    if (t1 > t0) rmq::k_build_batch<<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0);
+    ok = hipGetLastError() == hipSuccess;
+  }
+  // the host vectors die here: wait for the copies (and the build) before returning
+  ok = ok && hipStreamSynchronize(s) == hipSuccess;
+  if (d_tiles) (void)hipFree(d_tiles);
+  if (!ok) {
+    rmq_batch_free(b);
+    return RMQ_EDEVICE;
+  }
+  *out = b;
+  return RMQ_OK;
+}
+
+rmq_status rmq_batch_plan_queries(const rmq_batch* b, const uint64_t* qoffsets, void* stream,
+                                  rmq_batch_plan** out) {
+  if (!b || !qoffsets || !out) return RMQ_EINVAL;
+  *out = nullptr;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  // assign arrays to queues greedily by query count (largest first)
+  std::vector<uint32_t> order(b->count);
+  std::iota(order.begin(), order.end(), 0u);
+  std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    return qoffsets[x + 1] - qoffsets[x] > qoffsets[y + 1] - qoffsets[y];
+  });
+  std::vector<uint64_t> load(kQueues, 0);
+  std::vector<std::vector<uint32_t>> members(kQueues);
+  for (uint32_t a : order) {
+    if (qoffsets[a + 1] < qoffsets[a]) return RMQ_EINVAL;
+    const uint32_t qn = static_cast<uint32_t>(std::min_element(load.begin(), load.end()) - load.begin());
+    load[qn] += qoffsets[a + 1] - qoffsets[a];
+    members[qn].push_back(a);
+  }
+  std::vector<QueryTile> tiles;
+  std::vector<uint32_t> qstart(kQueues + 1, 0);
+  for (uint32_t qn = 0; qn < kQueues; ++qn) {
+    qstart[qn] = static_cast<uint32_t>(tiles.size());
+    std::sort(members[qn].begin(), members[qn].end());
+    for (uint32_t a : members[qn]) {
+      for (uint64_t q = qoffsets[a]; q < qoffsets[a + 1]; q += kTileQueries) {
+        const uint64_t c = std::min<uint64_t>(kTileQueries, qoffsets[a + 1] - q);
+        tiles.push_back(QueryTile{a, static_cast<uint32_t>(c), q});
+      }
+    }
+  }
+  qstart[kQueues] = static_cast<uint32_t>(tiles.size());
+  rmq_batch_plan* p = new (std::nothrow) rmq_batch_plan();
+  if (!p) return RMQ_ENOMEM;
+  p->ntiles = static_cast<uint32_t>(tiles.size());
+  rmq_status st = dmalloc(&p->d_tiles, tiles.size());
+  if (st == RMQ_OK) st = dmalloc(&p->d_qstart, kQueues + 1);
+  if (st == RMQ_OK) st = dmalloc(&p->d_counter, kQueues);
+  if (st != RMQ_OK) {
+    rmq_batch_plan_free(p);
+    return st;
+  }
+  bool ok = hipMemcpyAsync(p->d_qstart, qstart.data(), qstart.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess;
+  if (ok && !tiles.empty())
+    ok = hipMemcpyAsync(p->d_tiles, tiles.data(), tiles.size() * sizeof(QueryTile), hipMemcpyHostToDevice, s) == hipSuccess;
+  ok = ok && hipStreamSynchronize(s) == hipSuccess;
+  if (!ok) {
+    rmq_batch_plan_free(p);
+    return RMQ_EDEVICE;
+  }
+  *out = p;
+  return RMQ_OK;
+}
+
+rmq_status rmq_batch_query(const rmq_batch* b, const rmq_batch_plan* p, const uint32_t* d_lr,
+                           float* d_val, uint32_t* d_idx, void* stream) {
+  if (!b || !p) return RMQ_EINVAL;
+  if (p->ntiles == 0) return RMQ_OK;
+  if (!d_lr || (!d_val && !d_idx) || (reinterpret_cast<uintptr_t>(d_lr) & 7u) != 0) return RMQ_EINVAL;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint2* Q = reinterpret_cast<const uint2*>(d_lr);
+  rmq::k_reset_counters<<<1, 64, 0, s>>>(p->d_counter);
+  if (d_idx) {
+    static const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(rmq::k_query_batch<true>));
+    rmq::k_query_batch<true><<<grid, kBlock, 0, s>>>(b->d_views, b->d_variant, p->d_tiles,
+                                                     p->d_qstart, p->d_counter, Q, d_val, d_idx);
+  } else {
+    static const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(rmq::k_query_batch<false>));
+    rmq::k_query_batch<false><<<grid, kBlock, 0, s>>>(b->d_views, b->d_variant, p->d_tiles,
+                                                      p->d_qstart, p->d_counter, Q, d_val, d_idx);
+  }
+  return err();
+}
+
+rmq_status rmq_batch_capture(const rmq_batch* b, const rmq_batch_plan* p, const uint32_t* d_lr,
+                             float* d_val, uint32_t* d_idx, void* stream, rmq_graph** out) {
+  if (!out) return RMQ_EINVAL;
+  *out = nullptr;
+  (void)stream;
+  hipStream_t cs;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return RMQ_EDEVICE;
+  rmq_graph* g = new (std::nothrow) rmq_graph();
+  if (!g) {
+    (void)hipStreamDestroy(cs);
+    return RMQ_ENOMEM;
+  }
+  rmq_status st = RMQ_OK;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) st = RMQ_EDEVICE;
+  if (st == RMQ_OK) {
+    st = rmq_batch_query(b, p, d_lr, d_val, d_idx, cs);
+    hipGraph_t graph = nullptr;
+    if (hipStreamEndCapture(cs, &graph) != hipSuccess) st = RMQ_EDEVICE;
+    g->graph = graph;
+  }
+  if (st == RMQ_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    st = RMQ_EDEVICE;
+  (void)hipStreamDestroy(cs);
+  if (st != RMQ_OK) {
+    rmq_graph_free(g);
+    return st;
+  }
+  *out = g;
+  return RMQ_OK;
+}
+
+rmq_status rmq_graph_launch(const rmq_graph* g, void* stream) {
+  if (!g || !g->exec) return RMQ_EINVAL;
+  return hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) == hipSuccess ? RMQ_OK : RMQ_EDEVICE;
+}
+
+void rmq_graph_free(rmq_graph* g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+}
+
+void rmq_batch_plan_free(rmq_batch_plan* p) {
+  if (!p) return;
+  if (p->d_tiles) (void)hipFree(p->d_tiles);
+  if (p->d_qstart) (void)hipFree(p->d_qstart);
+  if (p->d_counter) (void)hipFree(p->d_counter);
+  delete p;
+}
+
+void rmq_batch_free(rmq_batch* b) {
+  if (!b) return;
+  if (b->lv) (void)hipFree(b->lv);
+  if (b->li) (void)hipFree(b->li);
+  if (b->d_views) (void)hipFree(b->d_views);
+  if (b->d_variant) (void)hipFree(b->d_variant);
+  delete b;
+}
+
+uint64_t rmq_batch_level_bytes(const rmq_batch* b) { return b ? b->level_floats * 8 : 0; }
+
+}  // extern "C"

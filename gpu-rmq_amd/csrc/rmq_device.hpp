@@ -116,6 +116,34 @@ __device__ __forceinline__ float4 load4(const float* __restrict__ base, uint32_t
   return r;
 }
 
+// Streaming form for data read once per query (level-0 blocks, the query pairs): the
+// non-temporal hint keeps it from displacing the upper levels in L2.
+#ifndef RMQ_STREAM_L0
+#define RMQ_STREAM_L0 1
+#endif
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float4 load4_stream(const float* __restrict__ base, uint32_t i,
+                                               uint32_t size) {
+#if RMQ_STREAM_L0
+  if (i + 4 <= size) {
+    const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(base + i));
+    return make_float4(v.x, v.y, v.z, v.w);
+  }
+#endif
+  return load4(base, i, size);
+}
+
+__device__ __forceinline__ uint2 load_query(const uint2* __restrict__ Q, uint64_t qi) {
+#if RMQ_STREAM_L0
+  const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(Q + qi));
+  return make_uint2(v.x, v.y);
+#else
+  return Q[qi];
+#endif
+}
+
 __device__ __forceinline__ uint4 load4u(const uint32_t* __restrict__ base, uint32_t i, uint32_t size) {
   if (i + 4 <= size) return *reinterpret_cast<const uint4*>(base + i);
   uint4 r = make_uint4(kNoIdx, kNoIdx, kNoIdx, kNoIdx);
@@ -144,8 +172,67 @@ __device__ __forceinline__ void take(float& bv, uint32_t& bi, float v, uint32_t 
   bi = t ? i : bi;
 }
 
-// Same with indices: level 0 (ix == nullptr) uses positions, upper levels read the stored
-// leftmost index of each block.
+// ---- index mode without index-array traffic ----------------------------------------------
+// The pieces a query visits stand for disjoint level-0 spans, so candidates are totally ordered
+// by the level-0 start of their span ("key" = position << (R_log * level)). Comparing
+// (value, key) therefore realises the leftmost rule without reading any stored index; only the
+// winner's index is looked up at the end. "pay" says where that index lives: kDirect | position
+// for a level-0 candidate, or the slot of the level-buffer index array.
+constexpr uint32_t kDirect = 0x80000000u;
+
+struct Cand {
+  float v;
+  uint32_t key;
+  uint32_t pay;
+};
+
+__device__ __forceinline__ Cand cand_none() { return Cand{RMQ_INF, kNoIdx, kNoIdx}; }
+
+__device__ __forceinline__ void cand_take(Cand& b, float v, uint32_t key, uint32_t pay, bool ok) {
+  const bool t = ok && better(v, key, b.v, b.key);
+  b.v = t ? v : b.v;
+  b.key = t ? key : b.key;
+  b.pay = t ? pay : b.pay;
+}
+
+// Fold the elements of the vector at level position p that fall in [lo, hi). shift =
+// R_log * level; pay_base = kDirect at level 0, else the level's offset in the level buffer.
+__device__ __forceinline__ void cand_fold4(Cand& b, float4 x, uint32_t p, uint32_t lo, uint32_t hi,
+                                           uint32_t shift, uint32_t pay_base) {
+  cand_take(b, x.x, (p + 0) << shift, pay_base + p + 0, in_range(p + 0, lo, hi));
+  cand_take(b, x.y, (p + 1) << shift, pay_base + p + 1, in_range(p + 1, lo, hi));
+  cand_take(b, x.z, (p + 2) << shift, pay_base + p + 2, in_range(p + 2, lo, hi));
+  cand_take(b, x.w, (p + 3) << shift, pay_base + p + 3, in_range(p + 3, lo, hi));
+}
+
+template <int S>
+__device__ __forceinline__ void step_cand(Cand& b) {
+  const float ov = u2f(partner<S>(f2u(b.v)));
+  const uint32_t ok = partner<S>(b.key);
+  const uint32_t op = partner<S>(b.pay);
+  const bool t = better(ov, ok, b.v, b.key);
+  b.v = t ? ov : b.v;
+  b.key = t ? ok : b.key;
+  b.pay = t ? op : b.pay;
+}
+
+template <int W>
+__device__ __forceinline__ void group_cand(Cand& b) {
+  if constexpr (W >= 2) step_cand<1>(b);
+  if constexpr (W >= 4) step_cand<2>(b);
+  if constexpr (W >= 8) step_cand<4>(b);
+  if constexpr (W >= 16) step_cand<8>(b);
+  if constexpr (W >= 32) step_cand<16>(b);
+  if constexpr (W >= 64) step_cand<32>(b);
+}
+
+// The leftmost level-0 index of the winning candidate.
+__device__ __forceinline__ uint32_t cand_index(const Cand& b, const uint32_t* __restrict__ li) {
+  return (b.pay & kDirect) ? (b.pay & ~kDirect) : li[b.pay];
+}
+
+// Same fold for the level build: positions are level-k slots, idx from the source index
+// array (ix == nullptr at level 0, where the index is the position itself).
 __device__ __forceinline__ void fold4_idx(float& bv, uint32_t& bi, float4 x, const uint32_t* ix,
                                           uint32_t i, uint32_t size, uint32_t lo, uint32_t hi) {
   uint4 id;

@@ -54,22 +54,44 @@ __global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict_
 
 // ---------------------------------------------------------------------------------------
 // Query traversal shared by algs 16 and 24 (src/xxx.h:158-187 and 493-521): r exclusive;
-// while the remaining range at this level is longer than 2R, fold the two partial blocks
-// [l, ceil_R(l)) and [floor_R(r), r) and climb; finally scan [l, r) at the level reached.
-// A group of G lanes works on one query: each lane owns 16-byte vectors of a block, so a
-// 32-float block is one vector load per lane at G = 8. The level loop runs with a
-// wave-uniform counter (metadata stays in scalar registers); groups that stopped climbing
-// sit it out.
+// while the remaining range at this level is longer than 2R, the partial blocks
+// [l, ceil_R(l)) and [floor_R(r), r) belong to the answer and the walk climbs; the range
+// left at the level reached (the "final range") is scanned. A group of G lanes works on one
+// query: each lane owns 16-byte vectors, so a 32-float block is one vector load per lane at
+// G = 8.
+//
+// Two evaluation orders of the same pieces, same answers:
+//  - solve_cl (full): every piece is read, level by level, as the reference does.
+//  - solve_cl_pruned (default): the final range is read first. All left pieces lie inside the
+//    level-K block just left of the final range, all right pieces inside the block just right
+//    of it ("guards", adjacent to the final range, so usually on the same cache lines). A tail
+//    is read only if its guard could still win: left tail if guard <= best (a left element
+//    wins ties), right tail if guard < best. Walking down, the next guard is the level-k entry
+//    just outside the piece just read. For random data the tails are rarely needed, so most
+//    queries touch only the small top of the hierarchy instead of two random level-0 lines.
+//
+// Closed forms of the reference's climb: l_k = ceil(l / R^k), e_k = floor(e / R^k).
+
+// Index of the winner / value-only result of a group, written by all lanes of the group.
+struct GroupAnswer {
+  float v;
+  uint32_t idx;
+};
+
+__device__ __forceinline__ const float* level_values(const HierView& H, uint32_t k) {
+  return k == 0 ? H.A : H.lv + H.off[k];
+}
+
 template <int G, int RL, bool IDX>
-__device__ __forceinline__ void solve_cl(uint32_t l, uint32_t e, uint32_t gl, const HierView& H,
-                                         float& out_v, uint32_t& out_i) {
+__device__ __forceinline__ GroupAnswer solve_cl(uint32_t l, uint32_t e, uint32_t gl,
+                                                const HierView& H) {
   constexpr uint32_t R = 1u << RL;
-  constexpr uint32_t V = R / 4;   // vectors per block
+  constexpr uint32_t V = R / 4;
   float bv = RMQ_INF;
-  uint32_t bi = dev::kNoIdx;
+  dev::Cand b = dev::cand_none();
   const float* cur_v = H.A;
-  const uint32_t* cur_i = nullptr;
   uint32_t cur_size = H.size[0];
+  uint32_t shift = 0, pay = dev::kDirect;
   bool climbing = true;
   for (uint32_t lev = 0; lev + 1 < H.num_levels; ++lev) {
     if (climbing && e - l > 2 * R) {
@@ -81,60 +103,177 @@ __device__ __forceinline__ void solve_cl(uint32_t l, uint32_t e, uint32_t gl, co
         const uint32_t p = lb + 4 * s;
         if (p + 4 > l && p < nl) {
           const float4 x = dev::load4(cur_v, p, cur_size);
-          if constexpr (IDX) dev::fold4_idx(bv, bi, x, cur_i, p, cur_size, l, nl);
+          if constexpr (IDX) dev::cand_fold4(b, x, p, l, nl, shift, pay);
           else bv = dev::fold4(bv, x, p, l, nl);
         }
         const uint32_t p2 = pr + 4 * s;
         if (p2 < e) {
           const float4 x = dev::load4(cur_v, p2, cur_size);
-          if constexpr (IDX) dev::fold4_idx(bv, bi, x, cur_i, p2, cur_size, pr, e);
+          if constexpr (IDX) dev::cand_fold4(b, x, p2, pr, e, shift, pay);
           else bv = dev::fold4(bv, x, p2, pr, e);
         }
       }
       l = nl >> RL;
       e = pr >> RL;
       cur_v = H.lv + H.off[lev + 1];
-      cur_i = H.li + H.off[lev + 1];
       cur_size = H.size[lev + 1];
+      shift += RL;
+      pay = H.off[lev + 1];
     } else {
       climbing = false;
     }
   }
-  // final scan of [l, e) at the level reached, 4*G elements per step
   for (uint32_t p = (l & ~(4 * G - 1)) + 4 * gl; p < e; p += 4 * G) {
     if (p + 4 > l) {
       const float4 x = dev::load4(cur_v, p, cur_size);
-      if constexpr (IDX) dev::fold4_idx(bv, bi, x, cur_i, p, cur_size, l, e);
+      if constexpr (IDX) dev::cand_fold4(b, x, p, l, e, shift, pay);
       else bv = dev::fold4(bv, x, p, l, e);
     }
   }
   if constexpr (IDX) {
-    dev::group_argmin<G>(bv, bi);
+    dev::group_cand<G>(b);
+    return GroupAnswer{b.v, dev::cand_index(b, H.li)};
+  } else {
+    return GroupAnswer{dev::group_min<G>(bv), dev::kNoIdx};
+  }
+}
+
+// Fold the part [lo, hi) of one level-k block (or any range inside one block) into b.
+template <int G, int RL, bool IDX>
+__device__ __forceinline__ void fold_piece(dev::Cand& b, float& bv, const float* vals,
+                                           uint32_t size, uint32_t lo, uint32_t hi, uint32_t gl,
+                                           uint32_t shift, uint32_t pay) {
+  constexpr uint32_t V = (1u << RL) / 4;
+  const uint32_t lb = lo & ~((1u << RL) - 1);
+#pragma unroll
+  for (uint32_t s = gl; s < V; s += G) {
+    const uint32_t p = lb + 4 * s;
+    if (p + 4 > lo && p < hi) {
+      const float4 x = dev::load4(vals, p, size);
+      if constexpr (IDX) dev::cand_fold4(b, x, p, lo, hi, shift, pay);
+      else bv = dev::fold4(bv, x, p, lo, hi);
+    }
+  }
+}
+
+template <int G, int RL, bool IDX>
+__device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, uint32_t gl,
+                                                       const HierView& H) {
+  constexpr uint32_t R = 1u << RL;
+  constexpr uint32_t STEP = 4 * G;
+  // climb with the reference's stopping rule; K = level of the final range
+  uint32_t K = 0, lk = l, ek = e;
+  for (uint32_t lev = 0; lev + 1 < H.num_levels; ++lev) {
+    if (K == lev && ek - lk > 2 * R) {
+      lk = (lk + R - 1) >> RL;
+      ek >>= RL;
+      K = lev + 1;
+    }
+  }
+  const uint32_t shK = RL * K;
+  const float* vK = level_values(H, K);
+  const uint32_t szK = H.size[K];
+  const uint32_t payK = K == 0 ? dev::kDirect : H.off[K];
+  const bool left_tail = K > 0 && l < (lk << shK);
+  const bool right_tail = K > 0 && (ek << shK) < e;
+  // guards are issued together with the final range
+  const float gL = left_tail ? vK[lk - 1] : RMQ_INF;
+  const float gR = right_tail ? vK[ek] : RMQ_INF;
+
+  // final range: per lane in increasing position, strict `<` keeps the leftmost
+  float bv = RMQ_INF;
+  uint32_t bpos = dev::kNoIdx;
+  for (uint32_t p = (lk & ~(STEP - 1)) + 4 * gl; p < ek; p += STEP) {
+    if (p + 4 > lk) {
+      const float4 x = dev::load4(vK, p, szK);
+      if constexpr (IDX) {
+        const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const bool t = dev::in_range(p + j, lk, ek) && (xs[j] < bv || bpos == dev::kNoIdx);
+          bv = t ? xs[j] : bv;
+          bpos = t ? p + j : bpos;
+        }
+      } else {
+        bv = dev::fold4(bv, x, p, lk, ek);
+      }
+    }
+  }
+  dev::Cand b;
+  if constexpr (IDX) {
+    b.v = bv;
+    b.key = bpos == dev::kNoIdx ? dev::kNoIdx : bpos << shK;
+    b.pay = bpos == dev::kNoIdx ? dev::kNoIdx : payK + bpos;
+    dev::group_cand<G>(b);
+    bv = b.v;
   } else {
     bv = dev::group_min<G>(bv);
   }
-  out_v = bv;
-  out_i = bi;
+
+  // left tail, walking down (rare for random data)
+  if (left_tail && !(gL > bv)) {
+    for (int k = static_cast<int>(K) - 1; k >= 0; --k) {
+      const uint32_t sk = RL * static_cast<uint32_t>(k);
+      const uint32_t lo = static_cast<uint32_t>((static_cast<uint64_t>(l) + (1ull << sk) - 1) >> sk);
+      const uint32_t hi = (lo + R - 1) & ~(R - 1);
+      const float* vk = level_values(H, k);
+      fold_piece<G, RL, IDX>(b, bv, vk, H.size[k], lo, hi, gl, sk, k == 0 ? dev::kDirect : H.off[k]);
+      if constexpr (IDX) {
+        dev::group_cand<G>(b);
+        bv = b.v;
+      } else {
+        bv = dev::group_min<G>(bv);
+      }
+      if (k == 0 || !(l < (lo << sk))) break;  // nothing deeper
+      if (vk[lo - 1] > bv) break;              // deeper guard cannot win
+    }
+  }
+  // right tail
+  if (right_tail && gR < bv) {
+    for (int k = static_cast<int>(K) - 1; k >= 0; --k) {
+      const uint32_t sk = RL * static_cast<uint32_t>(k);
+      const uint32_t hi = e >> sk;
+      const uint32_t lo = hi & ~(R - 1);
+      const float* vk = level_values(H, k);
+      fold_piece<G, RL, IDX>(b, bv, vk, H.size[k], lo, hi, gl, sk, k == 0 ? dev::kDirect : H.off[k]);
+      if constexpr (IDX) {
+        dev::group_cand<G>(b);
+        bv = b.v;
+      } else {
+        bv = dev::group_min<G>(bv);
+      }
+      if (k == 0 || !((hi << sk) < e)) break;
+      if (!(vk[hi] < bv)) break;
+    }
+  }
+  if constexpr (IDX) return GroupAnswer{b.v, dev::cand_index(b, H.li)};
+  else return GroupAnswer{bv, dev::kNoIdx};
+}
+
+// P selects the pruned evaluation order (default) or the full one.
+template <int G, int RL, bool P, bool IDX>
+__device__ __forceinline__ GroupAnswer solve_group(uint32_t l, uint32_t e, uint32_t gl,
+                                                   const HierView& H) {
+  if constexpr (P) return solve_cl_pruned<G, RL, IDX>(l, e, gl, H);
+  else return solve_cl<G, RL, IDX>(l, e, gl, H);
 }
 
 // Value-only answers must carry the bit pattern of A[leftmost argmin]. A plain minimum can
 // only differ from that when the minimum is zero (+0 and -0 compare equal); those queries
 // are answered again with index tracking.
-template <int G, int RL>
-__device__ __forceinline__ float value_exact_cl(uint32_t l, uint32_t e, uint32_t gl,
-                                                const HierView& H) {
-  float v;
-  uint32_t i;
-  solve_cl<G, RL, false>(l, e, gl, H, v, i);
-  if (v == 0.0f) solve_cl<G, RL, true>(l, e, gl, H, v, i);
-  return v;
+template <int G, int RL, bool P>
+__device__ __forceinline__ float value_exact(uint32_t l, uint32_t e, uint32_t gl,
+                                             const HierView& H) {
+  GroupAnswer a = solve_group<G, RL, P, false>(l, e, gl, H);
+  if (a.v == 0.0f) a = solve_group<G, RL, P, true>(l, e, gl, H);
+  return a.v;
 }
 
 // Alg 16 (xxx_query, src/xxx.h:124-200): every lane owns one query; a G-lane group serves
 // its members' queries one after another (the reference's ballot work queue, here a fixed
 // round over the group's lanes with a ds_bpermute broadcast). Queries are read and answers
 // written with one coalesced access per wave.
-template <int G, int RL, bool IDX>
+template <int G, int RL, bool P, bool IDX>
 __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ Q, uint32_t q,
                                                       float* __restrict__ out_v,
                                                       uint32_t* __restrict__ out_i, HierView H) {
@@ -144,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ 
   const bool valid = qi < q;
   uint32_t l = 0, e = 0;
   if (valid) {
-    const uint2 p = Q[qi];
+    const uint2 p = dev::load_query(Q, qi);
     l = p.x;
     e = p.y + 1;
   }
@@ -160,9 +299,11 @@ __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ 
       float v;
       uint32_t i = dev::kNoIdx;
       if constexpr (IDX) {
-        solve_cl<G, RL, true>(lk, ek, gl, H, v, i);
+        const GroupAnswer a = solve_group<G, RL, P, true>(lk, ek, gl, H);
+        v = a.v;
+        i = a.idx;
       } else {
-        v = value_exact_cl<G, RL>(lk, ek, gl, H);
+        v = value_exact<G, RL, P>(lk, ek, gl, H);
       }
       if (gl == static_cast<uint32_t>(k)) {
         my_v = v;
@@ -179,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ 
 // Alg 24 (xxx_query_multi_load, src/xxx.h:470-528): one G-lane group per query, every lane of
 // the group reads the same query; grid-stride over queries in units of groups (the
 // reference strides by the thread count, Appendix-A quirk not kept).
-template <int G, int RL, bool IDX>
+template <int G, int RL, bool P, bool IDX>
 __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict__ Q, uint32_t q,
                                                         float* __restrict__ out_v,
                                                         uint32_t* __restrict__ out_i,
@@ -188,13 +329,15 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
   const uint64_t grp = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / G;
   const uint64_t ngrp = static_cast<uint64_t>(gridDim.x) * kBlock / G;
   for (uint64_t qi = grp; qi < q; qi += ngrp) {
-    const uint2 p = Q[qi];
+    const uint2 p = dev::load_query(Q, qi);
     float v;
     uint32_t i = dev::kNoIdx;
     if constexpr (IDX) {
-      solve_cl<G, RL, true>(p.x, p.y + 1, gl, H, v, i);
+      const GroupAnswer a = solve_group<G, RL, P, true>(p.x, p.y + 1, gl, H);
+      v = a.v;
+      i = a.idx;
     } else {
-      v = value_exact_cl<G, RL>(p.x, p.y + 1, gl, H);
+      v = value_exact<G, RL, P>(p.x, p.y + 1, gl, H);
     }
     if (gl == 0) {
       if (out_v) out_v[qi] = v;
@@ -210,15 +353,14 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
 // also re-reads the whole block when l is already aligned; that block is covered by the
 // next level, so it is skipped here — same answer, fewer loads.)
 template <int RL, bool IDX>
-__device__ __forceinline__ void solve_vl(uint32_t l, uint32_t e, const HierView& H, float& out_v,
-                                         uint32_t& out_i) {
+__device__ __forceinline__ GroupAnswer solve_vl(uint32_t l, uint32_t e, const HierView& H) {
   constexpr uint32_t R = 1u << RL;
   constexpr uint32_t V = R / 4;
   float bv = RMQ_INF;
-  uint32_t bi = dev::kNoIdx;
+  dev::Cand b = dev::cand_none();
   const float* cur_v = H.A;
-  const uint32_t* cur_i = nullptr;
   uint32_t cur_size = H.size[0];
+  uint32_t shift = 0, pay = dev::kDirect;
   bool climbing = true;
   for (uint32_t lev = 0; lev + 1 < H.num_levels; ++lev) {
     if (climbing && e - l > R) {
@@ -226,29 +368,32 @@ __device__ __forceinline__ void solve_vl(uint32_t l, uint32_t e, const HierView&
       const uint32_t pr = e & ~(R - 1);
       if (pr >= nl) {
         const uint32_t lb = l & ~(R - 1);
+        float4 xl[V], xr[V];
 #pragma unroll
-        for (uint32_t s = 0; s < V; ++s) {
+        for (uint32_t s = 0; s < V; ++s) {  // both blocks' loads in flight together
           const uint32_t p = lb + 4 * s;
-          if (p + 4 > l && p < nl) {
-            const float4 x = dev::load4(cur_v, p, cur_size);
-            if constexpr (IDX) dev::fold4_idx(bv, bi, x, cur_i, p, cur_size, l, nl);
-            else bv = dev::fold4(bv, x, p, l, nl);
-          }
+          xl[s] = (p + 4 > l && p < nl) ? dev::load4(cur_v, p, cur_size)
+                                        : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+          const uint32_t p2 = pr + 4 * s;
+          xr[s] = p2 < e ? dev::load4(cur_v, p2, cur_size)
+                         : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
         }
 #pragma unroll
         for (uint32_t s = 0; s < V; ++s) {
-          const uint32_t p = pr + 4 * s;
-          if (p < e) {
-            const float4 x = dev::load4(cur_v, p, cur_size);
-            if constexpr (IDX) dev::fold4_idx(bv, bi, x, cur_i, p, cur_size, pr, e);
-            else bv = dev::fold4(bv, x, p, pr, e);
+          if constexpr (IDX) {
+            dev::cand_fold4(b, xl[s], lb + 4 * s, l, nl, shift, pay);
+            dev::cand_fold4(b, xr[s], pr + 4 * s, pr, e, shift, pay);
+          } else {
+            bv = dev::fold4(bv, xl[s], lb + 4 * s, l, nl);
+            bv = dev::fold4(bv, xr[s], pr + 4 * s, pr, e);
           }
         }
         l = nl >> RL;
         e = pr >> RL;
         cur_v = H.lv + H.off[lev + 1];
-        cur_i = H.li + H.off[lev + 1];
         cur_size = H.size[lev + 1];
+        shift += RL;
+        pay = H.off[lev + 1];
       } else {
         climbing = false;
       }
@@ -258,53 +403,123 @@ __device__ __forceinline__ void solve_vl(uint32_t l, uint32_t e, const HierView&
   }
   for (uint32_t p = l & ~3u; p < e; p += 4) {
     const float4 x = dev::load4(cur_v, p, cur_size);
-    if constexpr (IDX) dev::fold4_idx(bv, bi, x, cur_i, p, cur_size, l, e);
+    if constexpr (IDX) dev::cand_fold4(b, x, p, l, e, shift, pay);
     else bv = dev::fold4(bv, x, p, l, e);
   }
-  out_v = bv;
-  out_i = bi;
+  if constexpr (IDX) return GroupAnswer{b.v, dev::cand_index(b, H.li)};
+  else return GroupAnswer{bv, dev::kNoIdx};
 }
 
+// Pruned evaluation order for alg 20 (same rules as solve_cl_pruned, one lane per query; the
+// climb stops when r - l <= R or the aligned bounds cross, src/xxx.h:903-913).
 template <int RL, bool IDX>
+__device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H) {
+  constexpr uint32_t R = 1u << RL;
+  constexpr uint32_t V = R / 4;
+  uint32_t K = 0, lk = l, ek = e;
+  for (uint32_t lev = 0; lev + 1 < H.num_levels; ++lev) {
+    if (K == lev && ek - lk > R) {
+      const uint32_t nl = (lk + R - 1) & ~(R - 1);
+      const uint32_t pr = ek & ~(R - 1);
+      if (pr >= nl) {
+        lk = nl >> RL;
+        ek = pr >> RL;
+        K = lev + 1;
+      }
+    }
+  }
+  const uint32_t shK = RL * K;
+  const float* vK = level_values(H, K);
+  const uint32_t szK = H.size[K];
+  const bool left_tail = K > 0 && l < (lk << shK);
+  const bool right_tail = K > 0 && (ek << shK) < e;
+  const float gL = left_tail ? vK[lk - 1] : RMQ_INF;
+  const float gR = right_tail ? vK[ek] : RMQ_INF;
+  float bv = RMQ_INF;
+  uint32_t bpos = dev::kNoIdx;
+  for (uint32_t p = lk & ~3u; p < ek; p += 4) {
+    const float4 x = dev::load4(vK, p, szK);
+    if constexpr (IDX) {
+      const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const bool t = dev::in_range(p + j, lk, ek) && (xs[j] < bv || bpos == dev::kNoIdx);
+        bv = t ? xs[j] : bv;
+        bpos = t ? p + j : bpos;
+      }
+    } else {
+      bv = dev::fold4(bv, x, p, lk, ek);
+    }
+  }
+  dev::Cand b;
+  b.v = bv;
+  b.key = bpos == dev::kNoIdx ? dev::kNoIdx : bpos << shK;
+  b.pay = bpos == dev::kNoIdx ? dev::kNoIdx : (K == 0 ? dev::kDirect : H.off[K]) + bpos;
+  if (left_tail && !(gL > bv)) {
+    for (int k = static_cast<int>(K) - 1; k >= 0; --k) {
+      const uint32_t sk = RL * static_cast<uint32_t>(k);
+      const uint32_t lo = static_cast<uint32_t>((static_cast<uint64_t>(l) + (1ull << sk) - 1) >> sk);
+      const uint32_t hi = (lo + R - 1) & ~(R - 1);
+      const float* vk = level_values(H, k);
+      const uint32_t pay = k == 0 ? dev::kDirect : H.off[k];
+#pragma unroll
+      for (uint32_t s = 0; s < V; ++s) {
+        const uint32_t p = (lo & ~(R - 1)) + 4 * s;
+        if (p + 4 > lo && p < hi) {
+          const float4 x = dev::load4(vk, p, H.size[k]);
+          if constexpr (IDX) dev::cand_fold4(b, x, p, lo, hi, sk, pay);
+          else b.v = dev::fold4(b.v, x, p, lo, hi);
+        }
+      }
+      bv = b.v;
+      if (k == 0 || !(l < (lo << sk))) break;
+      if (vk[lo - 1] > bv) break;
+    }
+  }
+  if (right_tail && gR < bv) {
+    for (int k = static_cast<int>(K) - 1; k >= 0; --k) {
+      const uint32_t sk = RL * static_cast<uint32_t>(k);
+      const uint32_t hi = e >> sk;
+      const uint32_t lo = hi & ~(R - 1);
+      const float* vk = level_values(H, k);
+      const uint32_t pay = k == 0 ? dev::kDirect : H.off[k];
+#pragma unroll
+      for (uint32_t s = 0; s < V; ++s) {
+        const uint32_t p = lo + 4 * s;
+        if (p < hi) {
+          const float4 x = dev::load4(vk, p, H.size[k]);
+          if constexpr (IDX) dev::cand_fold4(b, x, p, lo, hi, sk, pay);
+          else b.v = dev::fold4(b.v, x, p, lo, hi);
+        }
+      }
+      bv = b.v;
+      if (k == 0 || !((hi << sk) < e)) break;
+      if (!(vk[hi] < bv)) break;
+    }
+  }
+  if constexpr (IDX) return GroupAnswer{b.v, dev::cand_index(b, H.li)};
+  else return GroupAnswer{b.v, dev::kNoIdx};
+}
+
+template <int RL, bool P, bool IDX>
+__device__ __forceinline__ GroupAnswer solve_lane(uint32_t l, uint32_t e, const HierView& H) {
+  if constexpr (P) return solve_vl_pruned<RL, IDX>(l, e, H);
+  else return solve_vl<RL, IDX>(l, e, H);
+}
+
+template <int RL, bool P, bool IDX>
 __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q, uint32_t q,
                                                      float* __restrict__ out_v,
                                                      uint32_t* __restrict__ out_i, HierView H) {
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (qi >= q) return;
-  const uint2 p = Q[qi];
-  float v;
-  uint32_t i = dev::kNoIdx;
-  if constexpr (IDX) {
-    solve_vl<RL, true>(p.x, p.y + 1, H, v, i);
-  } else {
-    solve_vl<RL, false>(p.x, p.y + 1, H, v, i);
-    if (v == 0.0f) solve_vl<RL, true>(p.x, p.y + 1, H, v, i);
+  const uint2 p = dev::load_query(Q, qi);
+  GroupAnswer a = solve_lane<RL, P, IDX>(p.x, p.y + 1, H);
+  if constexpr (!IDX) {
+    if (a.v == 0.0f) a = solve_lane<RL, P, true>(p.x, p.y + 1, H);
   }
-  if (out_v) out_v[qi] = v;
-  if (IDX && out_i) out_i[qi] = i;
-}
-
-// ---------------------------------------------------------------------------------------
-// GPU BASE (kernel_rmq_basic / _idx / value+index, src/cuda_methods.cuh:4-68): one lane per
-// query, serial strict-`<` scan from l to r. The reference's --check oracle.
-__global__ __launch_bounds__(kBlock) void k_gpu_base(const float* __restrict__ A,
-                                                     const uint2* __restrict__ Q, uint32_t q,
-                                                     float* __restrict__ out_v,
-                                                     uint32_t* __restrict__ out_i) {
-  const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (qi >= q) return;
-  const uint2 p = Q[qi];
-  float m = A[p.x];
-  uint32_t mi = p.x;
-  for (uint32_t i = p.x; i <= p.y; ++i) {
-    const float x = A[i];
-    if (x < m) {
-      m = x;
-      mi = i;
-    }
-  }
-  if (out_v) out_v[qi] = m;
-  if (out_i) out_i[qi] = mi;
+  if (out_v) out_v[qi] = a.v;
+  if (IDX && out_i) out_i[qi] = a.idx;
 }
 
 }  // namespace rmq

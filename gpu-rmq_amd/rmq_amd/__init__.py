@@ -53,14 +53,14 @@ class Plan:
     R: int
 
 
-def _options(cg_size_log, cg_amount_log, group_lanes) -> Optional[BuildOptions]:
-    if cg_size_log is None and cg_amount_log is None and not group_lanes:
+def _options(cg_size_log, cg_amount_log, group_lanes, full=False) -> Optional[BuildOptions]:
+    if cg_size_log is None and cg_amount_log is None and not group_lanes and not full:
         return None
     o = BuildOptions()
     o.cg_size_log = 3 if cg_size_log is None else int(cg_size_log)
     o.cg_amount_log = 2 if cg_amount_log is None else int(cg_amount_log)
     o.group_lanes = int(group_lanes or 0)
-    o.flags = _lib.RMQ_BUILD_DEFAULT_CG
+    o.flags = _lib.RMQ_BUILD_DEFAULT_CG | (_lib.RMQ_BUILD_FULL if full else 0)
     return o
 
 
@@ -140,7 +140,7 @@ class Hierarchy:
     """
 
     def __init__(self, array, t: int, alg: int = RMQ_ALG_XXX, cg_size_log=None,
-                 cg_amount_log=None, group_lanes: int = 0, stream=None):
+                 cg_amount_log=None, group_lanes: int = 0, full: bool = False, stream=None):
         torch = _torch()
         _check_device_tensor(array, "array", (torch.float32,))
         if array.dim() != 1:
@@ -150,7 +150,7 @@ class Hierarchy:
         self.t = int(t)
         self.alg = int(alg)
         self._h = ctypes.c_void_p()
-        o = _options(cg_size_log, cg_amount_log, group_lanes)
+        o = _options(cg_size_log, cg_amount_log, group_lanes, full)
         st = _lib.lib().rmq_build_ex(array.data_ptr(), self.n, self.t, self.alg, _opt_ptr(o),
                                      _stream_handle(stream), ctypes.byref(self._h))
         _lib.check(st, "rmq_build")
@@ -222,3 +222,135 @@ def gpu_base(array, lr, want_val: bool = True, want_idx: bool = False, stream=No
                                  idx.data_ptr() if want_idx else None, _stream_handle(stream))
     _lib.check(st, "rmq_gpu_base")
     return val, idx
+
+
+# ---- batched form (include/rmq_batch.h) ------------------------------------------------------
+
+VARIANTS = {"lane": _lib.RMQ_VARIANT_LANE, "group4": _lib.RMQ_VARIANT_GROUP4,
+            "group8": _lib.RMQ_VARIANT_GROUP8, "group16": _lib.RMQ_VARIANT_GROUP16}
+
+
+def fill_uniform(out, seed: int, start: int = 0, stream=None) -> None:
+    """Fill a float32 CUDA tensor with the library's counter-based uniform [0, 1) numbers."""
+    torch = _torch()
+    _check_device_tensor(out, "out", (torch.float32,))
+    _lib.check(_lib.lib().rmq_fill_uniform(out.data_ptr(), out.numel(), seed, start,
+                                           _stream_handle(stream)), "rmq_fill_uniform")
+
+
+class Batch:
+    """Many independent arrays, one build launch per level, one persistent query launch.
+
+    arrays: float32 CUDA tensor holding every array; offsets/sizes (element units, offsets
+    multiples of 4); t and variant per array (ints / names in VARIANTS).
+    """
+
+    def __init__(self, arrays, offsets, sizes, t, variant, stream=None):
+        torch = _torch()
+        _check_device_tensor(arrays, "arrays", (torch.float32,))
+        count = len(sizes)
+        descs = (_lib.BatchArray * count)()
+        for a in range(count):
+            v = variant[a]
+            descs[a].offset = int(offsets[a])
+            descs[a].n = int(sizes[a])
+            descs[a].t = int(t[a])
+            descs[a].variant = VARIANTS[v] if isinstance(v, str) else int(v)
+        self.arrays = arrays
+        self.count = count
+        self._b = ctypes.c_void_p()
+        self._plans = []
+        _lib.check(_lib.lib().rmq_batch_build(arrays.data_ptr(), descs, count,
+                                              _stream_handle(stream), ctypes.byref(self._b)),
+                   "rmq_batch_build")
+
+    def plan(self, qoffsets, stream=None) -> "BatchPlan":
+        q = np.ascontiguousarray(qoffsets, dtype=np.uint64)
+        if q.size != self.count + 1:
+            raise ValueError("qoffsets needs count + 1 entries")
+        p = ctypes.c_void_p()
+        _lib.check(_lib.lib().rmq_batch_plan_queries(self._b, q.ctypes.data, _stream_handle(stream),
+                                                     ctypes.byref(p)), "rmq_batch_plan_queries")
+        plan = BatchPlan(self, p, int(q[-1]))
+        self._plans.append(plan)
+        return plan
+
+    def level_bytes(self) -> int:
+        return int(_lib.lib().rmq_batch_level_bytes(self._b))
+
+    def close(self) -> None:
+        for p in self._plans:
+            p.close()
+        self._plans = []
+        if self._b.value is not None:
+            _lib.lib().rmq_batch_free(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchPlan:
+    """Device work queue for one query layout of a Batch."""
+
+    def __init__(self, batch: Batch, handle, total: int):
+        self.batch, self._p, self.total = batch, handle, total
+        self._graphs = []
+
+    def query(self, lr, want_val=True, want_idx=False, out_val=None, out_idx=None, stream=None):
+        torch = _torch()
+        _check_device_tensor(lr, "lr", (torch.int32, torch.uint32))
+        if lr.numel() != 2 * self.total:
+            raise ValueError("lr must hold the planned number of pairs")
+        if want_val and out_val is None:
+            out_val = torch.empty(self.total, dtype=torch.float32, device=lr.device)
+        if want_idx and out_idx is None:
+            out_idx = torch.empty(self.total, dtype=torch.int32, device=lr.device)
+        _lib.check(_lib.lib().rmq_batch_query(self.batch._b, self._p, lr.data_ptr(),
+                                              out_val.data_ptr() if want_val else None,
+                                              out_idx.data_ptr() if want_idx else None,
+                                              _stream_handle(stream)), "rmq_batch_query")
+        return (out_val if want_val else None, out_idx if want_idx else None)
+
+    def capture(self, lr, out_val=None, out_idx=None) -> "Graph":
+        """Capture the query launch (counter reset + persistent kernel) into a HIP graph."""
+        g = ctypes.c_void_p()
+        _lib.check(_lib.lib().rmq_batch_capture(self.batch._b, self._p, lr.data_ptr(),
+                                                out_val.data_ptr() if out_val is not None else None,
+                                                out_idx.data_ptr() if out_idx is not None else None,
+                                                None, ctypes.byref(g)), "rmq_batch_capture")
+        graph = Graph(g, (lr, out_val, out_idx))
+        self._graphs.append(graph)
+        return graph
+
+    def close(self) -> None:
+        for g in self._graphs:
+            g.close()
+        self._graphs = []
+        if self._p.value is not None:
+            _lib.lib().rmq_batch_plan_free(self._p)
+            self._p = ctypes.c_void_p()
+
+
+class Graph:
+    """An instantiated HIP graph of a batched query launch (buffers bound at capture)."""
+
+    def __init__(self, handle, keep):
+        self._g, self._keep = handle, keep
+
+    def launch(self, stream=None) -> None:
+        _lib.check(_lib.lib().rmq_graph_launch(self._g, _stream_handle(stream)), "rmq_graph_launch")
+
+    def close(self) -> None:
+        if self._g.value is not None:
+            _lib.lib().rmq_graph_free(self._g)
+            self._g = ctypes.c_void_p()

@@ -10,7 +10,7 @@ import ctypes
 import os
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "librmq.so")
+LIB_PATH = os.environ.get("RMQ_LIB") or os.path.join(PKG_DIR, "librmq.so")
 
 RMQ_OK = 0
 RMQ_EINVAL = -1
@@ -25,6 +25,23 @@ RMQ_ALG_VECTOR_LOAD = 20
 RMQ_ALG_XXX_MULTI_LOAD = 24
 RMQ_MAX_LEVELS = 20
 RMQ_BUILD_DEFAULT_CG = 0x1
+RMQ_BUILD_FULL = 0x2
+RMQ_VARIANT_LANE = 1
+RMQ_VARIANT_GROUP4 = 4
+RMQ_VARIANT_GROUP8 = 8
+RMQ_VARIANT_GROUP16 = 16
+
+
+class BatchArray(ctypes.Structure):
+    """rmq_batch_array (include/rmq_batch.h)."""
+
+    _fields_ = [
+        ("offset", ctypes.c_uint64),
+        ("n", ctypes.c_uint32),
+        ("t", ctypes.c_uint32),
+        ("variant", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
 
 # Every symbol the public headers declare, with its ctypes signature.
 _u32 = ctypes.c_uint32
@@ -62,6 +79,16 @@ SIGNATURES = {
     "rmq_gpu_base": (ctypes.c_int, [_vp, _u64, _vp, _u64, _vp, _vp, _vp]),
     "rmq_status_str": (ctypes.c_char_p, [ctypes.c_int]),
     "rmq_version": (ctypes.c_char_p, []),
+    "rmq_batch_build": (ctypes.c_int, [_vp, _vp, _u32, _vp, ctypes.POINTER(_vp)]),
+    "rmq_batch_plan_queries": (ctypes.c_int, [_vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "rmq_batch_query": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "rmq_batch_capture": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "rmq_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "rmq_graph_free": (None, [_vp]),
+    "rmq_batch_plan_free": (None, [_vp]),
+    "rmq_batch_free": (None, [_vp]),
+    "rmq_batch_level_bytes": (_u64, [_vp]),
+    "rmq_fill_uniform": (ctypes.c_int, [_vp, _u64, _u64, _u64, _vp]),
     "rmq_gen_array": (None, [_u64, _i32, _vp]),
     "rmq_gen_queries": (ctypes.c_int, [_u64, _i32, _u64, _i32, ctypes.c_int, _vp]),
     "rmq_gen_sample_indices": (_u64, [_u64, _u64, _i32, _vp]),

@@ -1,0 +1,104 @@
+"""Summarise rocprofv3 CSV output of the RMQ kernels into a short Markdown report.
+
+    python -m rmq_amd.profsum --kernel k_query_xxx --queries 67108864 --bytes-per-query 195.99 \
+        --stats DIR/kt_kernel_stats.csv --fetch DIR/fetch_counter_collection.csv \
+        --write DIR/write_counter_collection.csv --copy-fetch DIR/copyf_counter_collection.csv \
+        --copy-write DIR/copyw_counter_collection.csv [--l2 DIR/l2_counter_collection.csv] \
+        --command "..." > profiles/<kernel>_<workload>.md
+
+FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what the same counters read for a
+plain device copy of known size (bench.py --copy-probe: 1 GiB read + 1 GiB written per launch).
+"""
+
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+
+
+def _per_kernel(path, counter):
+    vals = collections.defaultdict(list)
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] == counter:
+                vals[row["Kernel_Name"]].append(float(row["Counter_Value"]))
+    return vals
+
+
+def _match(vals, kernel):
+    hits = {k: v for k, v in vals.items() if kernel in k}
+    if not hits:
+        raise SystemExit(f"kernel {kernel!r} not found")
+    name = max(hits, key=lambda k: len(hits[k]))
+    v = hits[name]
+    return name, sum(v) / len(v), len(v)
+
+
+def _stats(path, kernel):
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if kernel in row["Name"]:
+                return row["Name"], int(row["Calls"]), float(row["AverageNs"]), float(row["MinNs"]), \
+                    float(row["MaxNs"])
+    raise SystemExit(f"kernel {kernel!r} not in stats")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--kernel", required=True)
+    ap.add_argument("--queries", type=int, required=True)
+    ap.add_argument("--bytes-per-query", type=float, required=True)
+    ap.add_argument("--stats", required=True)
+    ap.add_argument("--fetch", required=True)
+    ap.add_argument("--write", required=True)
+    ap.add_argument("--copy-fetch", required=True)
+    ap.add_argument("--copy-write", required=True)
+    ap.add_argument("--l2")
+    ap.add_argument("--command", action="append", default=[])
+    ap.add_argument("--title", default="")
+    a = ap.parse_args()
+
+    name, calls, avg_ns, min_ns, max_ns = _stats(a.stats, a.kernel)
+    _, fetch_kib, _ = _match(_per_kernel(a.fetch, "FETCH_SIZE"), a.kernel)
+    _, write_kib, _ = _match(_per_kernel(a.write, "WRITE_SIZE"), a.kernel)
+    _, cf, _ = _match(_per_kernel(a.copy_fetch, "FETCH_SIZE"), "copyBuffer")
+    _, cw, _ = _match(_per_kernel(a.copy_write, "WRITE_SIZE"), "copyBuffer")
+    gib_kib = float(1 << 20)
+    f_fac, w_fac = gib_kib / cf, gib_kib / cw
+    fetch_b = fetch_kib * 1024 * f_fac
+    write_b = write_kib * 1024 * w_fac
+    q = a.queries
+    algo = a.bytes_per_query * q
+    out = []
+    out.append(f"# {a.title or name}\n")
+    out.append("Commands (run from /tmp with TMPDIR=/tmp, repo at $R):\n")
+    for c in a.command:
+        out.append(f"    {c}")
+    out.append("")
+    out.append(f"Kernel: `{name}`\n")
+    out.append("| quantity | value |")
+    out.append("|---|---|")
+    out.append(f"| dispatches (kernel trace) | {calls} |")
+    out.append(f"| average duration | {avg_ns / 1e6:.4f} ms (min {min_ns / 1e6:.4f}, max {max_ns / 1e6:.4f}) |")
+    out.append(f"| queries per dispatch | {q} |")
+    out.append(f"| throughput at average duration | {q / avg_ns:.3f} Gqueries/s |")
+    out.append(f"| algorithmic bytes per query (R0=32 traversal, 64-B sectors) | {a.bytes_per_query:.3f} |")
+    out.append(f"| algorithmic GB/s at average duration | {algo / avg_ns:.1f} ({algo / avg_ns / 8000:.3f} of 8 TB/s) |")
+    out.append(f"| FETCH_SIZE per dispatch (raw) | {fetch_kib:.1f} KiB |")
+    out.append(f"| WRITE_SIZE per dispatch (raw) | {write_kib:.1f} KiB |")
+    out.append(f"| copy-probe calibration (1 GiB copy) | FETCH x{f_fac:.4f}, WRITE x{w_fac:.4f} |")
+    out.append(f"| measured traffic per dispatch | {(fetch_b + write_b) / 1e9:.3f} GB "
+               f"(fetch {fetch_b / 1e9:.3f} + write {write_b / 1e9:.3f}) |")
+    out.append(f"| measured bytes per query | {(fetch_b + write_b) / q:.1f} |")
+    out.append(f"| measured L2-to-fabric GB/s | {(fetch_b + write_b) / avg_ns:.1f} |")
+    if a.l2:
+        _, hit, _ = _match(_per_kernel(a.l2, "TCC_HIT_sum"), a.kernel)
+        _, miss, _ = _match(_per_kernel(a.l2, "TCC_MISS_sum"), a.kernel)
+        out.append(f"| L2 hits / misses per dispatch | {hit:.4g} / {miss:.4g} "
+                   f"(hit rate {hit / (hit + miss):.3f}, {miss / q:.2f} misses per query) |")
+    print("\n".join(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -67,16 +67,18 @@ typedef struct rmq_hierarchy rmq_hierarchy;
 typedef struct {
   uint32_t cg_size_log;     /* 0 -> 3 */
   uint32_t cg_amount_log;   /* 0 -> 2 */
-  uint32_t group_lanes;     /* 0 -> auto; else 2, 4, 8 or 16 */
+  uint32_t group_lanes;     /* 0 -> auto; else 4, 8 or 16 */
   uint32_t flags;           /* RMQ_BUILD_* below */
 } rmq_build_options;
 
 #define RMQ_BUILD_DEFAULT_CG 0x1u   /* use cg_size_log = cg_amount_log = 0 literally */
+#define RMQ_BUILD_FULL 0x2u         /* read every piece of the traversal in the reference's
+                                       order instead of the pruned order (same answers) */
 
 /* Build the hierarchy of block minima over d_array[0..n) on `stream`.
  * Reference counterpart: metadata loop + per-level xxx_build_step launches,
  * src/xxx.h:225-271 (alg 16), 552-599 (alg 24), 959-1010 (alg 20).
- * Requires n >= 1, n <= 2^32-1, d_array 16-byte aligned, t >= 1. */
+ * Requires 1 <= n <= 2^31-1 (the reference's int32 n), d_array 16-byte aligned, t >= 1. */
 rmq_status rmq_build(const float* d_array, uint64_t n, uint64_t t, int alg,
                      void* stream, rmq_hierarchy** out);
 

@@ -1,0 +1,135 @@
+"""Batched form (include/rmq_batch.h): many arrays, one persistent launch, HIP-graph replay.
+
+Batched answers must equal per-array plain launches (rmq_build + rmq_query) and the oracle's
+CPU BASE bit-exactly, for every execution variant and scan threshold."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+import rmq_amd
+from rmq_amd import _lib, workloads
+
+
+def test_batch_abi_rejects_bad_arguments():
+    L = rmq_amd.lib()
+    out = ctypes.c_void_p()
+    arr = (_lib.BatchArray * 1)()
+    arr[0].offset, arr[0].n, arr[0].t, arr[0].variant = 0, 100, 64, 8
+    assert L.rmq_batch_build(None, arr, 1, None, ctypes.byref(out)) == rmq_amd.RMQ_EINVAL
+    assert L.rmq_batch_build(0x1000, arr, 0, None, ctypes.byref(out)) == rmq_amd.RMQ_EINVAL
+    arr[0].t = 63
+    assert L.rmq_batch_build(0x1000, arr, 1, None, ctypes.byref(out)) == rmq_amd.RMQ_ETHRESHOLD
+    arr[0].t, arr[0].variant = 64, 3
+    assert L.rmq_batch_build(0x1000, arr, 1, None, ctypes.byref(out)) == rmq_amd.RMQ_EUNSUPPORTED
+    arr[0].variant, arr[0].offset = 8, 2
+    assert L.rmq_batch_build(0x1000, arr, 1, None, ctypes.byref(out)) == rmq_amd.RMQ_EINVAL
+    assert L.rmq_batch_query(None, None, 0x1000, 0x1000, None, None) == rmq_amd.RMQ_EINVAL
+    assert L.rmq_fill_uniform(None, 10, 1, 0, None) == rmq_amd.RMQ_EINVAL
+    L.rmq_batch_free(None)
+    L.rmq_batch_plan_free(None)
+    L.rmq_graph_free(None)
+
+
+def test_workload_generator_shapes():
+    w = workloads.batch_workload(arrays=50, log_n=(6, 14), log_q=(3, 9), seed=3)
+    assert len(w.sizes) == 50 and w.qoffsets[-1] == w.queries.shape[0]
+    for a in range(50):
+        q = w.queries[w.qoffsets[a]:w.qoffsets[a + 1]]
+        assert (q[:, 0] <= q[:, 1]).all() and (q[:, 1] < w.sizes[a]).all()
+    assert (w.offsets % 64 == 0).all()
+
+
+@pytest.mark.gpu
+def test_fill_uniform_matches_host_restatement(device):
+    import torch
+
+    out = torch.empty(100003, dtype=torch.float32, device="cuda")
+    rmq_amd.fill_uniform(out, seed=12345, start=777)
+    torch.cuda.synchronize()
+    want = workloads.splitmix_uniform(12345, 777, 100003)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    assert 0.0 <= float(out.min()) and float(out.max()) < 1.0
+
+
+def _device_workload(arrays, log_n, log_q, seed, kind="per-array"):
+    import torch
+
+    w = workloads.batch_workload(arrays=arrays, log_n=log_n, log_q=log_q, kind=kind, seed=seed)
+    total = int(w.offsets[-1] + w.sizes[-1] + 64)
+    dA = torch.empty(total, dtype=torch.float32, device="cuda")
+    rmq_amd.fill_uniform(dA, seed=seed)
+    dQ = torch.from_numpy(w.queries.view(np.int32)).cuda()
+    torch.cuda.synchronize()
+    return w, dA, dQ
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("want", ["index", "value"])
+def test_batch_equals_plain_and_oracle(device, want):
+    import torch
+
+    w, dA, dQ = _device_workload(96, (4, 17), (2, 11), seed=11)
+    A_host = dA.cpu().numpy()
+    # plant ties and signed zeros in a few arrays
+    rng = np.random.default_rng(5)
+    for a in range(0, 96, 7):
+        o, n = int(w.offsets[a]), int(w.sizes[a])
+        A_host[o:o + n] = rng.integers(0, 3, size=n).astype(np.float32) * \
+            np.where(rng.random(n) < 0.5, np.float32(-1), np.float32(1))
+    dA.copy_(torch.from_numpy(A_host))
+    variants = ["group8", "group4", "group16", "lane"]
+    t = [64 << (a % 5) for a in range(96)]
+    var = [variants[a % 4] for a in range(96)]
+    wv, wi = want in ("value",), want in ("index",)
+    with rmq_amd.Batch(dA, w.offsets, w.sizes, t, var) as b:
+        plan = b.plan(w.qoffsets)
+        bv, bi = plan.query(dQ, want_val=True, want_idx=wi)
+        torch.cuda.synchronize()
+    bv = bv.cpu().numpy()
+    bi = bi.cpu().numpy().view(np.uint32) if wi else None
+    for a in range(96):
+        o, n = int(w.offsets[a]), int(w.sizes[a])
+        q0, q1 = int(w.qoffsets[a]), int(w.qoffsets[a + 1])
+        Aa = A_host[o:o + n]
+        lr = w.queries[q0:q1]
+        ref_i = oracle.cpu_base_index(Aa, lr)
+        assert np.array_equal(bv[q0:q1].view(np.uint32), Aa[ref_i].view(np.uint32)), (a, var[a])
+        if wi:
+            assert np.array_equal(bi[q0:q1], ref_i), (a, var[a])
+        # per-array plain launch through the C ABI gives the same answers
+        with rmq_amd.Hierarchy(dA[o:o + n], t[a], 16) as h:
+            pv, pi = h.query(dQ[q0:q1], True, True)
+            torch.cuda.synchronize()
+        assert np.array_equal(pv.cpu().numpy().view(np.uint32), bv[q0:q1].view(np.uint32))
+        if wi:
+            assert np.array_equal(pi.cpu().numpy().view(np.uint32), bi[q0:q1])
+
+
+@pytest.mark.gpu
+def test_batch_graph_replays(device):
+    """The captured launch replays correctly several times, with eager work in between."""
+    import torch
+
+    w, dA, dQ = _device_workload(200, (8, 18), (4, 12), seed=21, kind="mixed")
+    with rmq_amd.Batch(dA, w.offsets, w.sizes, [1 << 10] * 200, ["group8"] * 200) as b:
+        plan = b.plan(w.qoffsets)
+        ev, ei = plan.query(dQ, True, True)
+        torch.cuda.synchronize()
+        eager_v, eager_i = ev.clone(), ei.clone()
+        gv = torch.full_like(ev, -1.0)
+        gi = torch.full_like(ei, -1)
+        graph = plan.capture(dQ, gv, gi)
+        for rep in range(4):
+            gv.fill_(-1.0)
+            gi.fill_(-1)
+            torch.cuda.synchronize()
+            graph.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(gv.view(torch.int32), eager_v.view(torch.int32)), rep
+            assert torch.equal(gi, eager_i), rep
+            junk = torch.rand(1 << 20, device="cuda").sum()  # eager work between replays
+            torch.cuda.synchronize()
+            assert float(junk) > 0

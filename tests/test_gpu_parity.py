@@ -34,6 +34,8 @@ VARIANTS = [
     (24, dict(cg_size_log=4, cg_amount_log=3)),              # R=128
     (20, dict(cg_amount_log=0)),                             # R=4
     (20, dict(cg_amount_log=4)),                             # R=64
+    (16, dict(full=True)),                                 # level loop instead of up-front loads
+    (24, dict(full=True, group_lanes=4)),
 ]
 
 
@@ -43,6 +45,7 @@ def _say(msg):
 
 
 def _R(alg, opts):
+    opts = {k: v for k, v in opts.items() if k in ("cg_size_log", "cg_amount_log")}
     g = opts.get("cg_size_log", 3)
     c = opts.get("cg_amount_log", 2)
     return 1 << (c + 2) if alg == 20 else 1 << (g + c)
@@ -112,6 +115,25 @@ def test_small_edge_cases(device, alg, opts):
                 assert_exact(A, lr, v, None, ref_i, f"alg={alg} n={n} t={t} {kind} value")
                 _, i = run_gpu(A, lr, alg, t, False, True, **opts)
                 assert_exact(A, lr, None, i, ref_i, f"alg={alg} n={n} t={t} {kind} index")
+
+
+@pytest.mark.parametrize("t", [64, 128, 4096, 1 << 17, 1 << 23])
+def test_every_depth(device, t):
+    """n = 2^22 + 3 with t from 64 (5 levels) to 2^23 (1 level): every compiled depth of the
+    up-front form and the rolled loop, value and index, mixed range lengths."""
+    n = (1 << 22) + 3
+    A = rmq_amd.gen_array(n, 77)
+    rep = A[::1000].copy()
+    A[::997][: rep.size] = rep  # plant repeated values
+    lr = rmq_amd.gen_queries(1 << 16, -6, n, 77)
+    ref_v, ref_i = oracle.CpuHierarchy(A, 64, 5, 16).query(lr)
+    for alg, opts in ((16, {}), (24, {}), (16, dict(full=True)), (16, dict(group_lanes=4)),
+                      (24, dict(group_lanes=16)), (20, {})):
+        _say(f"depth t={t} alg={alg} opts={opts}")
+        v, i = run_gpu(A, lr, alg, t, True, True, **opts)
+        assert_exact(A, lr, v, i, ref_i, f"depth t={t} alg={alg} {opts}")
+        v, _ = run_gpu(A, lr, alg, t, True, False, **opts)
+        assert_exact(A, lr, v, None, ref_i, f"depth t={t} alg={alg} {opts} value")
 
 
 def test_tie_vectors(device):
