@@ -44,6 +44,11 @@ def parse():
     ap.add_argument("--mode", choices=["index", "value", "both"], default="index")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-sweep", action="store_true", help="time the default alg 16, t=2^15")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--batch-arrays", type=int, default=4096)
+    ap.add_argument("--serve-requests", type=int, default=3000)
+    ap.add_argument("--serve-rate", type=float, default=4000.0, help="offered requests/s")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against known bytes)")
@@ -86,6 +91,145 @@ def time_queries(h, dQ, want_val, want_idx, out_v, out_i, steps, warmup):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) * 1e3 / steps
     return wall, ev0.elapsed_time(ev1) / steps
+
+
+def batched_leg(arrays: int):
+    """4096 arrays (sizes log-U[2^10, 2^24], mixed lengths, queries log-U[2^6, 2^16] per array):
+    one batched build + one persistent query launch vs per-array plain launches (rmq_build +
+    rmq_query per array); identical answers required."""
+    import numpy as np
+    import torch
+
+    import rmq_amd
+    from rmq_amd import autotune, workloads
+
+    t0 = time.perf_counter()
+    w = workloads.batch_workload(arrays=arrays, log_n=(10, 24), log_q=(6, 16), kind="mixed",
+                                 seed=SEED)
+    total = int(w.offsets[-1] + w.sizes[-1] + 64)
+    dA = torch.empty(total, dtype=torch.float32, device="cuda")
+    rmq_amd.fill_uniform(dA, seed=SEED)
+    dQ = torch.from_numpy(w.queries.view(np.int32)).cuda()
+    choices = []
+    for a in range(arrays):
+        q = w.queries[w.qoffsets[a]:w.qoffsets[a + 1]]
+        choices.append(autotune.choose(int(w.sizes[a]), q[:, 1].astype(np.int64) - q[:, 0] + 1))
+    var = [c[0] for c in choices]
+    ts = [c[1] for c in choices]
+    Q = int(w.qoffsets[-1])
+    torch.cuda.synchronize()
+    log(f"batched workload: {arrays} arrays, {w.total_elements / 1e9:.2f} G elements, {Q} queries "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        best = 1e30
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t)
+        return best * 1e3
+
+    out_b = torch.empty(Q, dtype=torch.int32, device="cuda")
+    state = {}
+
+    def batched_build():
+        if "b" in state:
+            state["b"].close()
+        state["b"] = rmq_amd.Batch(dA, w.offsets, w.sizes, ts, var)
+
+    build_b = timed(batched_build)
+    plan = state["b"].plan(w.qoffsets)
+    query_b = timed(lambda: plan.query(dQ, False, True, None, out_b))
+    graph = plan.capture(dQ, None, out_b)
+    query_g = timed(lambda: graph.launch())
+    # per-array plain launches through the C ABI, issued from C loops (same t per array,
+    # alg 16, R = 32): rmq_build_each / rmq_query_each
+    out_p = torch.empty(Q, dtype=torch.int32, device="cuda")
+    pstate = {}
+
+    def plain_build():
+        if "p" in pstate:
+            pstate["p"].close()
+        pstate["p"] = rmq_amd.PlainMany(dA, w.offsets, w.sizes, ts)
+
+    build_p = timed(plain_build, reps=1)
+    query_p = timed(lambda: pstate["p"].query(dQ, w.qoffsets, out_p))
+    same = bool(torch.equal(out_b, out_p))
+    pstate["p"].close()
+    graph.close()
+    state["b"].close()
+    assert same, "batched answers differ from per-array plain launches"
+    return {
+        "workload": f"{arrays} arrays, sizes log-U[2^10,2^24] ({w.total_elements / 1e9:.2f} G "
+                    f"elements), queries log-U[2^6,2^16] per array ({Q} total), mixed lengths, "
+                    "index mode, autotuned per array",
+        "queries": Q,
+        "batched_query_ms": round(query_b, 3), "batched_graph_ms": round(query_g, 3),
+        "plain_query_ms": round(query_p, 3),
+        "query_speedup": round(query_p / query_g, 2),
+        "batched_build_ms": round(build_b, 3), "plain_build_ms": round(build_p, 3),
+        "end_to_end_speedup": round((build_p + query_p) / (build_b + query_g), 2),
+        "batched_gqs": round(Q / query_g / 1e6, 3),
+        "identical": same,
+    }
+
+
+def serve_leg(requests: int, rate: float):
+    """64 resident arrays (sizes log-U[2^20, 2^28]), Poisson arrivals at `rate` requests/s, array
+    by Zipf(1.1), 1..2^16 queries per request (log-uniform), small/medium/large lengths.
+    Latency = completion (seen by poll) - arrival."""
+    import numpy as np
+    import torch
+
+    import rmq_amd
+    from rmq_amd import autotune, workloads
+
+    rng = np.random.default_rng(SEED)
+    sizes = np.exp2(rng.uniform(20, 28, size=64)).astype(np.int64)
+    with rmq_amd.Server(slots=3, max_batch_queries=1 << 21) as srv:
+        ids = []
+        for a, n in enumerate(sizes):
+            var, t = autotune.choose(int(n), np.array([max(1, int(n ** 0.6))]))
+            ids.append(srv.admit_uniform(int(n), seed=SEED + a, t=t, variant=var))
+        ranks = np.arange(1, 65, dtype=np.float64)
+        p = ranks ** -1.1
+        p /= p.sum()
+        targets = rng.choice(64, size=requests, p=p)
+        counts = np.exp2(rng.uniform(0, 16, size=requests)).astype(np.int64)
+        kinds = rng.choice(["small", "medium", "large"], size=requests)
+        reqs = [workloads.make_queries(rng, int(sizes[targets[i]]), int(counts[i]), str(kinds[i]))
+                for i in range(requests)]
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=requests))
+        torch.cuda.synchronize()
+        log(f"serving: {requests} requests at {rate:.0f}/s over 64 resident arrays "
+            f"({srv.stats()['resident_bytes'] / 1e9:.1f} GB)")
+        done_at = {}
+        ticket_of = []
+        t0 = time.perf_counter()
+        i = 0
+        while len(done_at) < requests:
+            now = time.perf_counter() - t0
+            while i < requests and arrivals[i] <= now:
+                tk, _, _ = srv.submit(ids[targets[i]], reqs[i], False, True)
+                ticket_of.append(tk)
+                i += 1
+            for tk in srv.poll():
+                done_at[tk] = time.perf_counter() - t0
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array([done_at[tk] - arrivals[k] for k, tk in enumerate(ticket_of)]) * 1e3
+    return {
+        "workload": f"64 resident arrays log-U[2^20,2^28], {requests} requests, Poisson "
+                    f"{rate:.0f}/s, Zipf(1.1) targets, 1..2^16 queries/request, index mode",
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "gqs": round(float(counts.sum()) / wall / 1e9, 4),
+        "offered_gqs": round(float(counts.sum()) / float(arrivals[-1]) / 1e9, 4),
+        "batches": int(st["batches"]), "requests": requests,
+    }
 
 
 def cpu_baseline(A, lr, target_s=12.0):
@@ -159,34 +303,39 @@ def main():
     torch.cuda.synchronize()
 
     # --- choose (alg, t): explicit, default, or a short sweep ------------------------------
+    opts_of = {"16": {}, "16/g4": dict(group_lanes=4), "24/g4": dict(group_lanes=4), "20": {},
+               "20/c1": dict(cg_amount_log=1)}
     if args.alg and args.t:
-        candidates = [(args.alg, args.t)]
+        candidates = [(str(args.alg), args.t)]
     elif args.no_sweep:
-        candidates = [(16, 1 << 15)]
+        candidates = [("20/c1", 1 << 15)]
     else:
-        candidates = [(alg, 1 << tl) for alg in (16, 24, 20) for tl in (10, 12, 15, 18)]
+        candidates = [(v, 1 << tl) for v in ("16", "16/g4", "24/g4") for tl in (6, 10, 15, 18)]
+        candidates += [(v, 1 << tl) for v in ("20", "20/c1") for tl in (6, 10, 13, 15)]
     sweep = []
     best = None
-    for alg, t in candidates:
-        with rmq_amd.Hierarchy(dA, t, alg) as h:
+    for var, t in candidates:
+        alg = int(var.split("/")[0])
+        with rmq_amd.Hierarchy(dA, t, alg, **opts_of[var]) as h:
             torch.cuda.synchronize()
             wall, ev = time_queries(h, dQ, want_val, want_idx, out_v, out_i,
                                     steps=3 if len(candidates) > 1 else 1, warmup=1)
-        sweep.append({"alg": alg, "t": t, "ms": round(ev, 4), "gqs": round(q / ev / 1e6, 3)})
-        log(f"sweep alg={alg} t={t}: {ev:.3f} ms/launch, {q / ev / 1e6:.2f} Gq/s")
+        sweep.append({"alg": var, "t": t, "ms": round(ev, 4), "gqs": round(q / ev / 1e6, 3)})
+        log(f"sweep alg={var} t={t}: {ev:.3f} ms/launch, {q / ev / 1e6:.2f} Gq/s")
         if best is None or ev < best[2]:
-            best = (alg, t, ev)
-    alg, t, _ = best
+            best = (var, t, ev)
+    var, t, _ = best
+    alg = int(var.split("/")[0])
 
     # --- timed region --------------------------------------------------------------------------
-    with rmq_amd.Hierarchy(dA, t, alg) as h:
+    with rmq_amd.Hierarchy(dA, t, alg, **opts_of[var]) as h:
         torch.cuda.synchronize()
         tb0 = time.perf_counter()
-        with rmq_amd.Hierarchy(dA, t, alg) as h2:
+        with rmq_amd.Hierarchy(dA, t, alg, **opts_of[var]) as h2:
             torch.cuda.synchronize()
             build_ms = (time.perf_counter() - tb0) * 1e3
             meta = h2.metadata()
-        log(f"timing alg={alg} t={t}: {args.warmup} warmup + {args.steps} steps")
+        log(f"timing alg={var} t={t}: {args.warmup} warmup + {args.steps} steps")
         wall_ms, ev_ms = time_queries(h, dQ, want_val, want_idx, out_v, out_i,
                                       args.steps, args.warmup)
         # parity spot check of the timed output against GPU BASE on a sample
@@ -217,7 +366,7 @@ def main():
         "config": {
             "workload": f"plain hierarchical RMQ, {args.mode}-returning, n=2^{args.log_n}, "
                         f"q=2^{args.log_q}, lr={lr_mode} ({LR_NAMES.get(lr_mode, lr_mode)})",
-            "alg": alg, "t": t, "R": meta.R, "levels": meta.sizes, "n": n, "q": q,
+            "alg": var, "t": t, "R": meta.R, "levels": meta.sizes, "n": n, "q": q,
         },
         "roofline": {
             "bound": "hbm",
@@ -232,6 +381,12 @@ def main():
         "build_ms": round(build_ms, 3),
         "sweep": sweep,
     }
+    if not args.no_batched:
+        log("batched leg")
+        result["batched"] = batched_leg(args.batch_arrays)
+    if not args.no_serve:
+        log("serving leg")
+        result["serve"] = serve_leg(args.serve_requests, args.serve_rate)
     if not args.no_cpu:
         log("CPU baseline")
         cb, ch = cpu_baseline(A, lr)

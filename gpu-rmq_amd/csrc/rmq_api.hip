@@ -11,6 +11,7 @@
 #include <new>
 
 #include "../../include/rmq.h"
+#include "rmq_internal.hpp"
 #include "rmq_kernels.hpp"
 
 using rmq::HierView;
@@ -348,6 +349,34 @@ rmq_status rmq_get_metadata(const rmq_hierarchy* h, uint32_t* num_levels, uint64
   return RMQ_OK;
 }
 
+rmq_status rmq_build_each(const float* const* d_arrays, const uint64_t* n, const uint64_t* t,
+                          int alg, const rmq_build_options* opt, uint32_t count, void* stream,
+                          rmq_hierarchy** out) {
+  if (!d_arrays || !n || !t || !out) return RMQ_EINVAL;
+  for (uint32_t i = 0; i < count; ++i) {
+    const rmq_status st = rmq_build_ex(d_arrays[i], n[i], t[i], alg, opt, stream, &out[i]);
+    if (st != RMQ_OK) {
+      for (uint32_t j = 0; j < i; ++j) {
+        rmq_free(out[j]);
+        out[j] = nullptr;
+      }
+      return st;
+    }
+  }
+  return RMQ_OK;
+}
+
+rmq_status rmq_query_each(rmq_hierarchy* const* hs, const uint32_t* const* d_lr, const uint64_t* q,
+                          float* const* d_val, uint32_t* const* d_idx, uint32_t count, void* stream) {
+  if (!hs || !d_lr || !q) return RMQ_EINVAL;
+  for (uint32_t i = 0; i < count; ++i) {
+    const rmq_status st = rmq_query(hs[i], d_lr[i], q[i], d_val ? d_val[i] : nullptr,
+                                    d_idx ? d_idx[i] : nullptr, stream);
+    if (st != RMQ_OK) return st;
+  }
+  return RMQ_OK;
+}
+
 rmq_status rmq_gpu_base(const float* d_array, uint64_t n, const uint32_t* d_lr, uint64_t q,
                         float* d_val, uint32_t* d_idx, void* stream) {
   if (!d_array || n == 0 || n > kMaxN) return RMQ_EINVAL;
@@ -376,3 +405,7 @@ const char* rmq_status_str(rmq_status s) {
 const char* rmq_version(void) { return "rmq-mi355x 0.1 gfx950"; }
 
 }  // extern "C"
+
+const rmq::HierView* rmq_internal_view(const rmq_hierarchy* h) { return h ? &h->view : nullptr; }
+
+uint64_t rmq_internal_level_bytes(const rmq_hierarchy* h) { return h ? h->buffer_floats * 8 : 0; }

@@ -3,13 +3,7 @@
 //
 // Device layout: every array gets a HierView (the same level metadata the plain kernels read,
 // rmq_device.hpp) in a table in global memory; its level buffers are slices of two arenas
-// (values, leftmost indices). Tiles are 256 queries of one array.
-//
-// XCD-aware scheduling: workgroups are dispatched round-robin over the 8 XCDs of an MI355X,
-// so workgroup b runs on XCD b % 8. Arrays are assigned to 8 queues (greedy by query count)
-// and workgroups pull from the queue of their own XCD first, so an array's upper levels are
-// cached in one XCD's L2; an idle workgroup then steals from the other queues. Correctness does
-// not depend on the dispatch order: every tile is taken exactly once through an atomic counter.
+// (values, leftmost indices). The query side is rmq_multi.hpp.
 
 #include <hip/hip_runtime.h>
 
@@ -21,17 +15,14 @@
 #include <vector>
 
 #include "../../include/rmq_batch.h"
-#include "rmq_kernels.hpp"
+#include "rmq_multi.hpp"
 
 using rmq::HierView;
 using rmq::kBlock;
 
 namespace {
 
-constexpr uint32_t kRLog = 5;            // R = 32 for every array of a batch
-constexpr uint32_t kR = 1u << kRLog;
-constexpr uint32_t kQueues = 8;          // XCDs of an MI355X
-constexpr uint32_t kTileQueries = kBlock;
+constexpr uint32_t kR = rmq::kMultiR;
 constexpr uint32_t kBuildTile = 1024;    // level-(k+1) entries per build tile
 
 struct BuildTile {
@@ -40,11 +31,8 @@ struct BuildTile {
   uint32_t j0, j1; // destination entries [j0, j1) of level k+1
 };
 
-struct QueryTile {
-  uint32_t array;
-  uint32_t count;  // queries in the tile (<= 256)
-  uint64_t q0;     // first global query index
-};
+using rmq::QueryTile;
+using rmq::kQueues;
 
 }  // namespace
 
@@ -119,119 +107,9 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
   }
 }
 
-__global__ void k_reset_counters(uint32_t* __restrict__ c) {
-  if (threadIdx.x < kQueues) c[threadIdx.x] = 0;
-}
-
-// ---- query: persistent, tiles pulled from per-XCD queues ----------------------------------------
-template <int G, bool IDX>
-__device__ __forceinline__ void tile_groups(const HierView& H, const uint2* __restrict__ Q,
-                                            uint64_t q0, uint32_t count, float* __restrict__ ov,
-                                            uint32_t* __restrict__ oi) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t gl = lane & (G - 1);
-  const uint32_t local = threadIdx.x;  // one query per lane, 256 per tile
-  const bool valid = local < count;
-  uint32_t l = 0, e = 0;
-  if (valid) {
-    const uint2 p = dev::load_query(Q, q0 + local);
-    l = p.x;
-    e = p.y + 1;
-  }
-  float my_v = RMQ_INF;
-  uint32_t my_i = dev::kNoIdx;
-  const int gbase = static_cast<int>(lane & ~(G - 1));
-  for (int k = 0; k < G; ++k) {
-    const int src = gbase + k;
-    const uint32_t lk = static_cast<uint32_t>(__shfl(static_cast<int>(l), src));
-    const uint32_t ek = static_cast<uint32_t>(__shfl(static_cast<int>(e), src));
-    const int vk = __shfl(static_cast<int>(valid), src);
-    if (vk) {
-      float v;
-      uint32_t i = dev::kNoIdx;
-      if constexpr (IDX) {
-        const GroupAnswer a = solve_cl_pruned<G, kRLog, true>(lk, ek, gl, H);
-        v = a.v;
-        i = a.idx;
-      } else {
-        v = value_exact<G, kRLog, true>(lk, ek, gl, H);
-      }
-      if (gl == static_cast<uint32_t>(k)) {
-        my_v = v;
-        my_i = i;
-      }
-    }
-  }
-  if (valid) {
-    if (ov) ov[q0 + local] = my_v;
-    if (IDX && oi) oi[q0 + local] = my_i;
-  }
-}
-
-template <bool IDX>
-__device__ __forceinline__ void tile_lanes(const HierView& H, const uint2* __restrict__ Q,
-                                           uint64_t q0, uint32_t count, float* __restrict__ ov,
-                                           uint32_t* __restrict__ oi) {
-  const uint32_t local = threadIdx.x;
-  if (local >= count) return;
-  const uint2 p = dev::load_query(Q, q0 + local);
-  GroupAnswer a = solve_vl_pruned<kRLog, IDX>(p.x, p.y + 1, H);
-  if constexpr (!IDX) {
-    if (a.v == 0.0f) a = solve_vl_pruned<kRLog, true>(p.x, p.y + 1, H);
-  }
-  if (ov) ov[q0 + local] = a.v;
-  if (IDX && oi) oi[q0 + local] = a.idx;
-}
-
-template <bool IDX>
-__global__ __launch_bounds__(kBlock) void k_query_batch(
-    const HierView* __restrict__ views, const uint32_t* __restrict__ variant,
-    const QueryTile* __restrict__ tiles, const uint32_t* __restrict__ qstart,
-    uint32_t* __restrict__ counter, const uint2* __restrict__ Q, float* __restrict__ ov,
-    uint32_t* __restrict__ oi) {
-  __shared__ uint32_t s_tile;
-  const uint32_t home = blockIdx.x % kQueues;
-  uint32_t queue = home;
-  uint32_t tried = 0;
-  while (tried < kQueues) {
-    if (threadIdx.x == 0) {
-      const uint32_t len = qstart[queue + 1] - qstart[queue];
-      const uint32_t t = atomicAdd(&counter[queue], 1u);
-      s_tile = t < len ? qstart[queue] + t : 0xFFFFFFFFu;
-    }
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    __syncthreads();
-    if (tile == 0xFFFFFFFFu) {  // this queue is drained: move to the next one
-      queue = (queue + 1) % kQueues;
-      ++tried;
-      continue;
-    }
-    const QueryTile T = tiles[tile];
-    const uint32_t a = __builtin_amdgcn_readfirstlane(T.array);
-    const HierView& H = views[a];
-    switch (variant[a]) {
-      case RMQ_VARIANT_GROUP4: tile_groups<4, IDX>(H, Q, T.q0, T.count, ov, oi); break;
-      case RMQ_VARIANT_GROUP16: tile_groups<16, IDX>(H, Q, T.q0, T.count, ov, oi); break;
-      case RMQ_VARIANT_LANE: tile_lanes<IDX>(H, Q, T.q0, T.count, ov, oi); break;
-      default: tile_groups<8, IDX>(H, Q, T.q0, T.count, ov, oi); break;
-    }
-  }
-}
-
 }  // namespace rmq
 
 namespace {
-
-uint32_t persistent_grid(const void* kernel) {
-  int dev = 0, cus = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 1024;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1)
-    per_cu = 2;
-  const uint32_t g = static_cast<uint32_t>(cus) * static_cast<uint32_t>(per_cu);
-  return (g + kQueues - 1) / kQueues * kQueues;
-}
 
 rmq_status err() { return hipGetLastError() == hipSuccess ? RMQ_OK : RMQ_EDEVICE; }
 
@@ -364,33 +242,11 @@ rmq_status rmq_batch_plan_queries(const rmq_batch* b, const uint64_t* qoffsets, 
   if (!b || !qoffsets || !out) return RMQ_EINVAL;
   *out = nullptr;
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  // assign arrays to queues greedily by query count (largest first)
-  std::vector<uint32_t> order(b->count);
-  std::iota(order.begin(), order.end(), 0u);
-  std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-    return qoffsets[x + 1] - qoffsets[x] > qoffsets[y + 1] - qoffsets[y];
-  });
-  std::vector<uint64_t> load(kQueues, 0);
-  std::vector<std::vector<uint32_t>> members(kQueues);
-  for (uint32_t a : order) {
+  for (uint32_t a = 0; a < b->count; ++a)
     if (qoffsets[a + 1] < qoffsets[a]) return RMQ_EINVAL;
-    const uint32_t qn = static_cast<uint32_t>(std::min_element(load.begin(), load.end()) - load.begin());
-    load[qn] += qoffsets[a + 1] - qoffsets[a];
-    members[qn].push_back(a);
-  }
   std::vector<QueryTile> tiles;
-  std::vector<uint32_t> qstart(kQueues + 1, 0);
-  for (uint32_t qn = 0; qn < kQueues; ++qn) {
-    qstart[qn] = static_cast<uint32_t>(tiles.size());
-    std::sort(members[qn].begin(), members[qn].end());
-    for (uint32_t a : members[qn]) {
-      for (uint64_t q = qoffsets[a]; q < qoffsets[a + 1]; q += kTileQueries) {
-        const uint64_t c = std::min<uint64_t>(kTileQueries, qoffsets[a + 1] - q);
-        tiles.push_back(QueryTile{a, static_cast<uint32_t>(c), q});
-      }
-    }
-  }
-  qstart[kQueues] = static_cast<uint32_t>(tiles.size());
+  std::vector<uint32_t> qstart;
+  rmq::make_query_tiles(b->count, qoffsets, tiles, qstart);
   rmq_batch_plan* p = new (std::nothrow) rmq_batch_plan();
   if (!p) return RMQ_ENOMEM;
   p->ntiles = static_cast<uint32_t>(tiles.size());
@@ -420,16 +276,8 @@ rmq_status rmq_batch_query(const rmq_batch* b, const rmq_batch_plan* p, const ui
   if (!d_lr || (!d_val && !d_idx) || (reinterpret_cast<uintptr_t>(d_lr) & 7u) != 0) return RMQ_EINVAL;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   const uint2* Q = reinterpret_cast<const uint2*>(d_lr);
-  rmq::k_reset_counters<<<1, 64, 0, s>>>(p->d_counter);
-  if (d_idx) {
-    static const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(rmq::k_query_batch<true>));
-    rmq::k_query_batch<true><<<grid, kBlock, 0, s>>>(b->d_views, b->d_variant, p->d_tiles,
-                                                     p->d_qstart, p->d_counter, Q, d_val, d_idx);
-  } else {
-    static const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(rmq::k_query_batch<false>));
-    rmq::k_query_batch<false><<<grid, kBlock, 0, s>>>(b->d_views, b->d_variant, p->d_tiles,
-                                                      p->d_qstart, p->d_counter, Q, d_val, d_idx);
-  }
+  rmq::launch_multi_query(b->d_views, b->d_variant, p->d_tiles, p->d_qstart, p->d_counter, Q,
+                          d_val, d_idx, s);
   return err();
 }
 

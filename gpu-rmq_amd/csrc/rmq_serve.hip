@@ -1,0 +1,397 @@
+// rmq_serve.hip — serving loop over device-resident hierarchies (include/rmq_serve.h).
+//
+// Resident arrays are ordinary hierarchies (rmq_build_ex, R = 32) whose level views sit in a
+// device table; a batch is a set of pending requests packed into one slot's staging buffers and
+// answered by the multi-array persistent kernel (rmq_multi.hpp), one request = one segment of
+// tiles on its array. Each slot owns a stream, pinned host staging and device buffers, so up to
+// `slots` batches overlap on the device while the host packs the next one.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <new>
+#include <vector>
+
+#include "../../include/rmq_serve.h"
+#include "rmq_internal.hpp"
+#include "rmq_multi.hpp"
+
+namespace {
+
+struct Resident {
+  bool live = false;
+  float* owned = nullptr;  // device array owned by the server (HOST / UNIFORM admission)
+  rmq_hierarchy* h = nullptr;
+  uint64_t bytes = 0;
+  uint32_t users = 0;      // pending + in-flight requests
+};
+
+struct Request {
+  uint64_t ticket = 0;
+  int32_t array = -1;
+  uint64_t q = 0;
+  std::vector<uint32_t> lr;
+  float* h_val = nullptr;
+  uint32_t* h_idx = nullptr;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  bool want_idx = false;
+  uint64_t total = 0;
+  uint32_t* h_q = nullptr;
+  float* h_val = nullptr;
+  uint32_t* h_idx = nullptr;
+  rmq::QueryTile* h_tiles = nullptr;
+  uint32_t* h_qstart = nullptr;
+  uint32_t* d_q = nullptr;
+  float* d_val = nullptr;
+  uint32_t* d_idx = nullptr;
+  rmq::QueryTile* d_tiles = nullptr;
+  uint32_t* d_qstart = nullptr;
+  uint32_t* d_counter = nullptr;
+  std::vector<Request> reqs;
+  std::vector<uint64_t> offs;
+};
+
+bool dalloc(void** p, size_t bytes) {
+  if (hipMalloc(p, bytes ? bytes : 4) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return false;
+  }
+  return true;
+}
+
+bool halloc(void** p, size_t bytes) {
+  if (hipHostMalloc(p, bytes ? bytes : 4, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+struct rmq_server {
+  rmq_server_options opt{};
+  uint32_t max_tiles = 0;
+  std::vector<Resident> arrays;
+  rmq::HierView* d_views = nullptr;
+  uint32_t* d_variant = nullptr;
+  rmq::HierView* h_views = nullptr;  // pinned mirrors of the tables
+  uint32_t* h_variant = nullptr;
+  hipStream_t admin = nullptr;
+  std::deque<Request> pending;
+  std::vector<Slot> slots;
+  uint64_t next_ticket = 1;
+  rmq_server_stats stats{};
+};
+
+namespace {
+
+void free_slot(Slot& sl) {
+  if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+  (void)hipHostFree(sl.h_q);
+  (void)hipHostFree(sl.h_val);
+  (void)hipHostFree(sl.h_idx);
+  (void)hipHostFree(sl.h_tiles);
+  (void)hipHostFree(sl.h_qstart);
+  (void)hipFree(sl.d_q);
+  (void)hipFree(sl.d_val);
+  (void)hipFree(sl.d_idx);
+  (void)hipFree(sl.d_tiles);
+  (void)hipFree(sl.d_qstart);
+  (void)hipFree(sl.d_counter);
+  if (sl.done) (void)hipEventDestroy(sl.done);
+  if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  sl = Slot();
+}
+
+bool init_slot(Slot& sl, uint64_t max_q, uint32_t max_tiles) {
+  bool ok = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess;
+  ok = ok && halloc(reinterpret_cast<void**>(&sl.h_q), max_q * 8) &&
+       halloc(reinterpret_cast<void**>(&sl.h_val), max_q * 4) &&
+       halloc(reinterpret_cast<void**>(&sl.h_idx), max_q * 4) &&
+       halloc(reinterpret_cast<void**>(&sl.h_tiles), max_tiles * sizeof(rmq::QueryTile)) &&
+       halloc(reinterpret_cast<void**>(&sl.h_qstart), (rmq::kQueues + 1) * 4);
+  ok = ok && dalloc(reinterpret_cast<void**>(&sl.d_q), max_q * 8) &&
+       dalloc(reinterpret_cast<void**>(&sl.d_val), max_q * 4) &&
+       dalloc(reinterpret_cast<void**>(&sl.d_idx), max_q * 4) &&
+       dalloc(reinterpret_cast<void**>(&sl.d_tiles), max_tiles * sizeof(rmq::QueryTile)) &&
+       dalloc(reinterpret_cast<void**>(&sl.d_qstart), (rmq::kQueues + 1) * 4) &&
+       dalloc(reinterpret_cast<void**>(&sl.d_counter), rmq::kQueues * 4);
+  return ok;
+}
+
+// Copy a finished slot's answers to its requests' buffers and free the slot.
+void retire(rmq_server* s, Slot& sl, uint64_t* done, uint32_t max_done, uint32_t* n_done) {
+  for (size_t i = 0; i < sl.reqs.size(); ++i) {
+    Request& r = sl.reqs[i];
+    if (r.h_val) std::memcpy(r.h_val, sl.h_val + sl.offs[i], r.q * 4);
+    if (r.h_idx) std::memcpy(r.h_idx, sl.h_idx + sl.offs[i], r.q * 4);
+    s->arrays[r.array].users--;
+    s->stats.completed++;
+    if (done && *n_done < max_done) done[(*n_done)++] = r.ticket;
+  }
+  sl.reqs.clear();
+  sl.offs.clear();
+  sl.busy = false;
+  s->stats.in_flight--;
+}
+
+// Pack pending requests (FIFO) into a free slot and launch the batch.
+rmq_status launch(rmq_server* s, Slot& sl) {
+  uint64_t total = 0;
+  sl.want_idx = false;
+  std::vector<uint64_t> qoff;
+  std::vector<uint32_t> ids;
+  while (!s->pending.empty() && sl.reqs.size() < s->opt.max_batch_requests &&
+         total + s->pending.front().q <= s->opt.max_batch_queries) {
+    Request r = std::move(s->pending.front());
+    s->pending.pop_front();
+    std::memcpy(sl.h_q + 2 * total, r.lr.data(), r.q * 8);
+    sl.offs.push_back(total);
+    qoff.push_back(total);
+    ids.push_back(static_cast<uint32_t>(r.array));
+    sl.want_idx = sl.want_idx || r.h_idx != nullptr;
+    total += r.q;
+    r.lr.clear();
+    r.lr.shrink_to_fit();
+    sl.reqs.push_back(std::move(r));
+  }
+  if (sl.reqs.empty()) return RMQ_OK;
+  qoff.push_back(total);
+  std::vector<rmq::QueryTile> tiles;
+  std::vector<uint32_t> qstart;
+  rmq::make_query_tiles(static_cast<uint32_t>(ids.size()), qoff.data(), tiles, qstart, ids.data());
+  std::memcpy(sl.h_tiles, tiles.data(), tiles.size() * sizeof(rmq::QueryTile));
+  std::memcpy(sl.h_qstart, qstart.data(), qstart.size() * 4);
+  const hipStream_t st = sl.stream;
+  bool ok = hipMemcpyAsync(sl.d_q, sl.h_q, total * 8, hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(sl.d_tiles, sl.h_tiles, tiles.size() * sizeof(rmq::QueryTile),
+                           hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(sl.d_qstart, sl.h_qstart, qstart.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
+  if (ok) {
+    rmq::launch_multi_query(s->d_views, s->d_variant, sl.d_tiles, sl.d_qstart, sl.d_counter,
+                            reinterpret_cast<const uint2*>(sl.d_q), sl.d_val,
+                            sl.want_idx ? sl.d_idx : nullptr, st);
+    ok = hipGetLastError() == hipSuccess;
+  }
+  ok = ok && hipMemcpyAsync(sl.h_val, sl.d_val, total * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+  if (ok && sl.want_idx)
+    ok = hipMemcpyAsync(sl.h_idx, sl.d_idx, total * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+  ok = ok && hipEventRecord(sl.done, st) == hipSuccess;
+  sl.total = total;
+  sl.busy = true;
+  s->stats.in_flight++;
+  s->stats.batches++;
+  s->stats.queries += total;
+  return ok ? RMQ_OK : RMQ_EDEVICE;
+}
+
+bool variant_ok(uint32_t v) {
+  return v == RMQ_VARIANT_GROUP4 || v == RMQ_VARIANT_GROUP8 || v == RMQ_VARIANT_GROUP16 ||
+         v == RMQ_VARIANT_LANE;
+}
+
+}  // namespace
+
+extern "C" {
+
+rmq_status rmq_server_create(const rmq_server_options* opt, rmq_server** out) {
+  if (!out) return RMQ_EINVAL;
+  *out = nullptr;
+  rmq_server* s = new (std::nothrow) rmq_server();
+  if (!s) return RMQ_ENOMEM;
+  if (opt) s->opt = *opt;
+  if (!s->opt.slots) s->opt.slots = 3;
+  if (!s->opt.max_batch_queries) s->opt.max_batch_queries = 1u << 20;
+  if (!s->opt.reserve_bytes) s->opt.reserve_bytes = 2ull << 30;
+  if (!s->opt.max_arrays) s->opt.max_arrays = 1024;
+  if (!s->opt.max_batch_requests) s->opt.max_batch_requests = 4096;
+  s->max_tiles = s->opt.max_batch_queries / rmq::kTileQueries + s->opt.max_batch_requests;
+  bool ok = hipStreamCreateWithFlags(&s->admin, hipStreamNonBlocking) == hipSuccess &&
+            dalloc(reinterpret_cast<void**>(&s->d_views), s->opt.max_arrays * sizeof(rmq::HierView)) &&
+            dalloc(reinterpret_cast<void**>(&s->d_variant), s->opt.max_arrays * 4) &&
+            halloc(reinterpret_cast<void**>(&s->h_views), s->opt.max_arrays * sizeof(rmq::HierView)) &&
+            halloc(reinterpret_cast<void**>(&s->h_variant), s->opt.max_arrays * 4);
+  s->slots.resize(s->opt.slots);
+  for (Slot& sl : s->slots) ok = ok && init_slot(sl, s->opt.max_batch_queries, s->max_tiles);
+  if (!ok) {
+    rmq_server_destroy(s);
+    return RMQ_ENOMEM;
+  }
+  *out = s;
+  return RMQ_OK;
+}
+
+void rmq_server_destroy(rmq_server* s) {
+  if (!s) return;
+  for (Slot& sl : s->slots) free_slot(sl);
+  for (Resident& r : s->arrays) {
+    if (r.h) rmq_free(r.h);
+    if (r.owned) (void)hipFree(r.owned);
+  }
+  (void)hipFree(s->d_views);
+  (void)hipFree(s->d_variant);
+  (void)hipHostFree(s->h_views);
+  (void)hipHostFree(s->h_variant);
+  if (s->admin) (void)hipStreamDestroy(s->admin);
+  delete s;
+}
+
+rmq_status rmq_server_admit(rmq_server* s, int source, const float* data, uint64_t n, uint64_t seed,
+                            uint32_t t, uint32_t variant, int32_t* array_id) {
+  if (!s || !array_id || n == 0 || n > 0x7FFFFFFFull || !variant_ok(variant)) return RMQ_EINVAL;
+  if (source != RMQ_ADMIT_DEVICE && source != RMQ_ADMIT_HOST && source != RMQ_ADMIT_UNIFORM)
+    return RMQ_EINVAL;
+  if (source != RMQ_ADMIT_UNIFORM && !data) return RMQ_EINVAL;
+  rmq_build_options o{3, 2, variant == RMQ_VARIANT_LANE ? 0u : variant, 0};
+  uint64_t floats = 0;
+  rmq_status st = rmq_plan(n, t, RMQ_ALG_XXX, &o, nullptr, nullptr, nullptr, &floats, nullptr);
+  if (st != RMQ_OK) return st;
+  const uint64_t need = floats * 8 + (source == RMQ_ADMIT_DEVICE ? 0 : n * 4) + (1u << 16);
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return RMQ_EDEVICE;
+  if (free_b < need + s->opt.reserve_bytes) return RMQ_ENOMEM;
+  int32_t id = -1;
+  for (size_t i = 0; i < s->arrays.size(); ++i)
+    if (!s->arrays[i].live) {
+      id = static_cast<int32_t>(i);
+      break;
+    }
+  if (id < 0) {
+    if (s->arrays.size() >= s->opt.max_arrays) return RMQ_ENOMEM;
+    s->arrays.emplace_back();
+    id = static_cast<int32_t>(s->arrays.size() - 1);
+  }
+  Resident& r = s->arrays[id];
+  const float* A = data;
+  if (source != RMQ_ADMIT_DEVICE) {
+    if (!dalloc(reinterpret_cast<void**>(&r.owned), n * 4)) return RMQ_ENOMEM;
+    if (source == RMQ_ADMIT_HOST) {
+      if (hipMemcpyAsync(r.owned, data, n * 4, hipMemcpyHostToDevice, s->admin) != hipSuccess) st = RMQ_EDEVICE;
+    } else {
+      st = rmq_fill_uniform(r.owned, n, seed, 0, s->admin);
+    }
+    A = r.owned;
+  }
+  if (st == RMQ_OK) st = rmq_build_ex(A, n, t, RMQ_ALG_XXX, &o, s->admin, &r.h);
+  if (st == RMQ_OK) {
+    s->h_views[id] = *rmq_internal_view(r.h);
+    s->h_variant[id] = variant;
+    if (hipMemcpyAsync(s->d_views + id, s->h_views + id, sizeof(rmq::HierView), hipMemcpyHostToDevice,
+                       s->admin) != hipSuccess ||
+        hipMemcpyAsync(s->d_variant + id, s->h_variant + id, 4, hipMemcpyHostToDevice, s->admin) != hipSuccess ||
+        hipStreamSynchronize(s->admin) != hipSuccess)
+      st = RMQ_EDEVICE;
+  }
+  if (st != RMQ_OK) {
+    if (r.h) rmq_free(r.h);
+    if (r.owned) (void)hipFree(r.owned);
+    r = Resident();
+    return st;
+  }
+  r.live = true;
+  r.bytes = rmq_internal_level_bytes(r.h) + (r.owned ? n * 4 : 0);
+  s->stats.resident++;
+  s->stats.resident_bytes += r.bytes;
+  *array_id = id;
+  return RMQ_OK;
+}
+
+rmq_status rmq_server_evict(rmq_server* s, int32_t array_id) {
+  if (!s || array_id < 0 || static_cast<size_t>(array_id) >= s->arrays.size()) return RMQ_EINVAL;
+  Resident& r = s->arrays[array_id];
+  if (!r.live || r.users) return RMQ_EINVAL;
+  rmq_free(r.h);
+  if (r.owned) (void)hipFree(r.owned);
+  s->stats.resident--;
+  s->stats.resident_bytes -= r.bytes;
+  r = Resident();
+  return RMQ_OK;
+}
+
+rmq_status rmq_server_submit(rmq_server* s, int32_t array_id, const uint32_t* h_lr, uint64_t q,
+                             float* h_val, uint32_t* h_idx, uint64_t* ticket) {
+  if (!s || !h_lr || !ticket || q == 0 || q > s->opt.max_batch_queries || (!h_val && !h_idx))
+    return RMQ_EINVAL;
+  if (array_id < 0 || static_cast<size_t>(array_id) >= s->arrays.size() || !s->arrays[array_id].live)
+    return RMQ_EINVAL;
+  Request r;
+  r.ticket = s->next_ticket++;
+  r.array = array_id;
+  r.q = q;
+  r.lr.assign(h_lr, h_lr + 2 * q);
+  r.h_val = h_val;
+  r.h_idx = h_idx;
+  s->arrays[array_id].users++;
+  s->stats.submitted++;
+  *ticket = r.ticket;
+  s->pending.push_back(std::move(r));
+  return RMQ_OK;
+}
+
+rmq_status rmq_server_poll(rmq_server* s, uint64_t* done, uint32_t max_done, uint32_t* n_done) {
+  if (!s) return RMQ_EINVAL;
+  uint32_t nd = 0;
+  rmq_status st = RMQ_OK;
+  for (Slot& sl : s->slots) {
+    if (!sl.busy) continue;
+    const hipError_t e = hipEventQuery(sl.done);
+    if (e == hipSuccess) {
+      retire(s, sl, done, max_done, &nd);
+    } else if (e != hipErrorNotReady) {
+      st = RMQ_EDEVICE;
+    }
+  }
+  for (Slot& sl : s->slots) {
+    if (s->pending.empty() || st != RMQ_OK) break;
+    if (!sl.busy) st = launch(s, sl);
+  }
+  if (n_done) *n_done = nd;
+  return st;
+}
+
+rmq_status rmq_server_drain(rmq_server* s, uint64_t* done, uint32_t max_done, uint32_t* n_done) {
+  if (!s) return RMQ_EINVAL;
+  uint32_t total = 0;
+  while (true) {
+    uint32_t nd = 0;
+    const rmq_status st = rmq_server_poll(s, done ? done + total : nullptr,
+                                          max_done > total ? max_done - total : 0, &nd);
+    total += nd;
+    if (st != RMQ_OK) {
+      if (n_done) *n_done = total;
+      return st;
+    }
+    Slot* wait = nullptr;
+    for (Slot& sl : s->slots)
+      if (sl.busy) {
+        wait = &sl;
+        break;
+      }
+    if (!wait && s->pending.empty()) break;
+    if (wait && hipEventSynchronize(wait->done) != hipSuccess) {
+      if (n_done) *n_done = total;
+      return RMQ_EDEVICE;
+    }
+  }
+  if (n_done) *n_done = total;
+  return RMQ_OK;
+}
+
+rmq_status rmq_server_get_stats(const rmq_server* s, rmq_server_stats* st) {
+  if (!s || !st) return RMQ_EINVAL;
+  *st = s->stats;
+  return RMQ_OK;
+}
+
+}  // extern "C"

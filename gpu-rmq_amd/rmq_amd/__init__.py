@@ -209,6 +209,54 @@ class Hierarchy:
             pass
 
 
+class PlainMany:
+    """Per-array plain hierarchies built and queried from C loops (rmq_build_each /
+    rmq_query_each): the baseline of the batched form. Offsets/sizes index one device buffer."""
+
+    def __init__(self, arrays, offsets, sizes, t, alg: int = RMQ_ALG_XXX, stream=None):
+        torch = _torch()
+        _check_device_tensor(arrays, "arrays", (torch.float32,))
+        self.count = count = len(sizes)
+        base = arrays.data_ptr()
+        ptrs = (ctypes.c_void_p * count)(*[base + 4 * int(o) for o in offsets])
+        ns = (ctypes.c_uint64 * count)(*[int(x) for x in sizes])
+        ts = (ctypes.c_uint64 * count)(*[int(x) for x in t])
+        self._hs = (ctypes.c_void_p * count)()
+        self.arrays = arrays
+        _lib.check(_lib.lib().rmq_build_each(ptrs, ns, ts, alg, None, count, _stream_handle(stream),
+                                             self._hs), "rmq_build_each")
+
+    def query(self, lr, qoffsets, out_idx, stream=None) -> None:
+        """Index answers of every array's queries (local pairs in lr, array a's pairs at
+        qoffsets[a]..qoffsets[a+1]) into out_idx at the same positions."""
+        c = self.count
+        qb, ob = lr.data_ptr(), out_idx.data_ptr()
+        qs = (ctypes.c_void_p * c)(*[qb + 8 * int(qoffsets[a]) for a in range(c)])
+        os_ = (ctypes.c_void_p * c)(*[ob + 4 * int(qoffsets[a]) for a in range(c)])
+        qn = (ctypes.c_uint64 * c)(*[int(qoffsets[a + 1] - qoffsets[a]) for a in range(c)])
+        _lib.check(_lib.lib().rmq_query_each(self._hs, qs, qn, None, os_, c, _stream_handle(stream)),
+                   "rmq_query_each")
+
+    def close(self) -> None:
+        if getattr(self, "_hs", None) is not None:
+            for h in self._hs:
+                if h:
+                    _lib.lib().rmq_free(h)
+            self._hs = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def gpu_base(array, lr, want_val: bool = True, want_idx: bool = False, stream=None):
     """GPU BASE (algs 2 / 102, src/cuda_methods.cuh:4-68) through rmq_gpu_base."""
     torch = _torch()
@@ -354,3 +402,94 @@ class Graph:
         if self._g.value is not None:
             _lib.lib().rmq_graph_free(self._g)
             self._g = ctypes.c_void_p()
+
+
+# ---- serving loop (include/rmq_serve.h) --------------------------------------------------------
+
+class Server:
+    """Resident hierarchies + continuous batching (single-threaded, driven by poll/drain).
+
+    Requests keep their numpy buffers alive until completion: answers land in them.
+    """
+
+    def __init__(self, slots=0, max_batch_queries=0, reserve_bytes=0, max_arrays=0,
+                 max_batch_requests=0):
+        o = _lib.ServerOptions(slots, max_batch_queries, reserve_bytes, max_arrays,
+                               max_batch_requests)
+        self._s = ctypes.c_void_p()
+        _lib.check(_lib.lib().rmq_server_create(ctypes.byref(o), ctypes.byref(self._s)),
+                   "rmq_server_create")
+        self._live = {}  # ticket -> (lr, val, idx) kept alive until done
+        self._done_buf = np.zeros(1 << 16, dtype=np.uint64)
+
+    def admit_uniform(self, n: int, seed: int, t: int = 64, variant="lane") -> int:
+        return self._admit(_lib.RMQ_ADMIT_UNIFORM, None, n, seed, t, variant)
+
+    def admit_host(self, array: np.ndarray, t: int = 64, variant="lane") -> int:
+        a = np.ascontiguousarray(array, dtype=np.float32)
+        return self._admit(_lib.RMQ_ADMIT_HOST, a.ctypes.data, a.size, 0, t, variant)
+
+    def admit_device(self, array, t: int = 64, variant="lane") -> int:
+        torch = _torch()
+        _check_device_tensor(array, "array", (torch.float32,))
+        return self._admit(_lib.RMQ_ADMIT_DEVICE, array.data_ptr(), array.numel(), 0, t, variant)
+
+    def _admit(self, source, ptr, n, seed, t, variant) -> int:
+        v = VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        aid = ctypes.c_int32()
+        _lib.check(_lib.lib().rmq_server_admit(self._s, source, ptr, n, seed, t, v, ctypes.byref(aid)),
+                   "rmq_server_admit")
+        return aid.value
+
+    def evict(self, array_id: int) -> None:
+        _lib.check(_lib.lib().rmq_server_evict(self._s, array_id), "rmq_server_evict")
+
+    def submit(self, array_id: int, lr: np.ndarray, want_val=True, want_idx=False):
+        lr = np.ascontiguousarray(lr, dtype=np.uint32)
+        q = lr.size // 2
+        val = np.empty(q, dtype=np.float32) if want_val else None
+        idx = np.empty(q, dtype=np.uint32) if want_idx else None
+        t = ctypes.c_uint64()
+        _lib.check(_lib.lib().rmq_server_submit(
+            self._s, array_id, lr.ctypes.data, q, val.ctypes.data if want_val else None,
+            idx.ctypes.data if want_idx else None, ctypes.byref(t)), "rmq_server_submit")
+        self._live[t.value] = (lr, val, idx)
+        return t.value, val, idx
+
+    def _finish(self, fn) -> list:
+        n = ctypes.c_uint32()
+        _lib.check(fn(self._s, self._done_buf.ctypes.data, self._done_buf.size, ctypes.byref(n)),
+                   fn.__name__)
+        done = self._done_buf[: n.value].tolist()
+        for t in done:
+            self._live.pop(t, None)
+        return done
+
+    def poll(self) -> list:
+        return self._finish(_lib.lib().rmq_server_poll)
+
+    def drain(self) -> list:
+        return self._finish(_lib.lib().rmq_server_drain)
+
+    def stats(self) -> dict:
+        st = _lib.ServerStats()
+        _lib.check(_lib.lib().rmq_server_get_stats(self._s, ctypes.byref(st)), "rmq_server_get_stats")
+        return {f: getattr(st, f) for f, _ in _lib.ServerStats._fields_}
+
+    def close(self) -> None:
+        if self._s.value is not None:
+            _lib.lib().rmq_server_destroy(self._s)
+            self._s = ctypes.c_void_p()
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -32,6 +32,37 @@ RMQ_VARIANT_GROUP8 = 8
 RMQ_VARIANT_GROUP16 = 16
 
 
+RMQ_ADMIT_DEVICE = 0
+RMQ_ADMIT_HOST = 1
+RMQ_ADMIT_UNIFORM = 2
+
+
+class ServerOptions(ctypes.Structure):
+    """rmq_server_options (include/rmq_serve.h)."""
+
+    _fields_ = [
+        ("slots", ctypes.c_uint32),
+        ("max_batch_queries", ctypes.c_uint32),
+        ("reserve_bytes", ctypes.c_uint64),
+        ("max_arrays", ctypes.c_uint32),
+        ("max_batch_requests", ctypes.c_uint32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    """rmq_server_stats (include/rmq_serve.h)."""
+
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("batches", ctypes.c_uint64),
+        ("queries", ctypes.c_uint64),
+        ("resident_bytes", ctypes.c_uint64),
+        ("resident", ctypes.c_uint32),
+        ("in_flight", ctypes.c_uint32),
+    ]
+
+
 class BatchArray(ctypes.Structure):
     """rmq_batch_array (include/rmq_batch.h)."""
 
@@ -77,6 +108,9 @@ SIGNATURES = {
         [_u64, _u64, ctypes.c_int, ctypes.POINTER(BuildOptions), _pu32, _pu64, _pu64, _pu64, _pu32],
     ),
     "rmq_gpu_base": (ctypes.c_int, [_vp, _u64, _vp, _u64, _vp, _vp, _vp]),
+    "rmq_build_each": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(BuildOptions),
+                                      _u32, _vp, _vp]),
+    "rmq_query_each": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "rmq_status_str": (ctypes.c_char_p, [ctypes.c_int]),
     "rmq_version": (ctypes.c_char_p, []),
     "rmq_batch_build": (ctypes.c_int, [_vp, _vp, _u32, _vp, ctypes.POINTER(_vp)]),
@@ -89,6 +123,16 @@ SIGNATURES = {
     "rmq_batch_free": (None, [_vp]),
     "rmq_batch_level_bytes": (_u64, [_vp]),
     "rmq_fill_uniform": (ctypes.c_int, [_vp, _u64, _u64, _u64, _vp]),
+    "rmq_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(_vp)]),
+    "rmq_server_destroy": (None, [_vp]),
+    "rmq_server_admit": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _u64, _u64, _u32, _u32,
+                                        ctypes.POINTER(ctypes.c_int32)]),
+    "rmq_server_evict": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "rmq_server_submit": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _u64, _vp, _vp,
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+    "rmq_server_poll": (ctypes.c_int, [_vp, _vp, _u32, ctypes.POINTER(ctypes.c_uint32)]),
+    "rmq_server_drain": (ctypes.c_int, [_vp, _vp, _u32, ctypes.POINTER(ctypes.c_uint32)]),
+    "rmq_server_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
     "rmq_gen_array": (None, [_u64, _i32, _vp]),
     "rmq_gen_queries": (ctypes.c_int, [_u64, _i32, _u64, _i32, ctypes.c_int, _vp]),
     "rmq_gen_sample_indices": (_u64, [_u64, _u64, _i32, _vp]),

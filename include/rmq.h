@@ -108,6 +108,15 @@ rmq_status rmq_plan(uint64_t n, uint64_t t, int alg, const rmq_build_options* op
                     uint32_t* num_levels, uint64_t* sizes, uint64_t* offsets,
                     uint64_t* buffer_floats, uint32_t* R);
 
+/* Plain per-array calls for many arrays at once, issued from a C loop: rmq_build_ex for each
+ * (d_arrays[i], n[i], t[i]) and rmq_query for each hierarchy. This is the per-array baseline the
+ * batched form (rmq_batch.h) is measured against; on error everything built so far is freed. */
+rmq_status rmq_build_each(const float* const* d_arrays, const uint64_t* n, const uint64_t* t,
+                          int alg, const rmq_build_options* opt, uint32_t count, void* stream,
+                          rmq_hierarchy** out);
+rmq_status rmq_query_each(rmq_hierarchy* const* hs, const uint32_t* const* d_lr, const uint64_t* q,
+                          float* const* d_val, uint32_t* const* d_idx, uint32_t count, void* stream);
+
 /* GPU BASE (algs 2 / 102): one lane per query scans [l, r] serially with strict `<`.
  * Reference counterpart: kernel_rmq_basic / kernel_rmq_basic_idx / value+index variant,
  * src/cuda_methods.cuh:4-68. Used by the rtxrmq driver's --check, as in src/main.cu:194-214. */
