@@ -22,7 +22,6 @@ using rmq::kBlock;
 
 namespace {
 
-constexpr uint32_t kR = rmq::kMultiR;
 constexpr uint32_t kBuildTile = 1024;    // level-(k+1) entries per build tile
 
 struct BuildTile {
@@ -51,7 +50,7 @@ struct rmq_batch_plan {
   uint32_t ntiles;
   QueryTile* d_tiles;
   uint32_t* d_qstart;   // [kQueues + 1] tile index where each queue starts
-  uint32_t* d_counter;  // [kQueues] next tile of each queue
+  uint32_t* d_counter;  // [kCounterWords] next unit of each queue, one per 256-byte line
 };
 
 struct rmq_graph {
@@ -78,10 +77,12 @@ __global__ __launch_bounds__(kBlock) void k_fill_uniform(float* __restrict__ out
   }
 }
 
-// ---- build: one launch per source level, one workgroup per tile -----------------------------
+// ---- build: one launch per (source level, R), one workgroup per tile -------------------------
+template <int RL>
 __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restrict__ views,
                                                         const BuildTile* __restrict__ tiles) {
-  constexpr uint32_t W = kR / 4;  // 8 lanes per 32-element block
+  constexpr uint32_t kR = 1u << RL;
+  constexpr uint32_t W = kR / 4;  // lanes per block: 8 at R = 32, 2 at R = 8
   const BuildTile T = tiles[blockIdx.x];
   const HierView& H = views[T.array];
   const uint32_t k = T.level;
@@ -153,14 +154,13 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
   for (uint32_t a = 0; a < count; ++a) {
     const rmq_batch_array& d = arrays[a];
     if (d.n == 0 || d.n > 0x7FFFFFFFu || (d.offset & 3u) != 0) return RMQ_EINVAL;
-    if (d.t < 2 * kR) return RMQ_ETHRESHOLD;
-    if (d.variant != RMQ_VARIANT_GROUP4 && d.variant != RMQ_VARIANT_GROUP8 &&
-        d.variant != RMQ_VARIANT_GROUP16 && d.variant != RMQ_VARIANT_LANE)
-      return RMQ_EUNSUPPORTED;
+    if (!rmq::variant_ok(d.variant)) return RMQ_EUNSUPPORTED;
     uint32_t nl = 0;
     uint64_t sizes[RMQ_MAX_LEVELS], offs[RMQ_MAX_LEVELS], floats = 0;
-    rmq_build_options o{3, 2, 0, 0};
-    const rmq_status st = rmq_plan(d.n, d.t, RMQ_ALG_XXX, &o, &nl, sizes, offs, &floats, nullptr);
+    int alg = 0;
+    rmq_build_options o;
+    rmq::variant_build(d.variant, &alg, &o);
+    const rmq_status st = rmq_plan(d.n, d.t, alg, &o, &nl, sizes, offs, &floats, nullptr);
     if (st != RMQ_OK) return st;
     HierView& H = views[a];
     std::memset(&H, 0, sizeof(H));
@@ -196,19 +196,21 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     b->variant[a] = arrays[a].variant;
   }
 
-  // build tiles, grouped by source level
+  // build tiles, grouped by source level and reduction factor (R = 32 first, then R = 8)
   std::vector<BuildTile> tiles;
-  std::vector<uint32_t> level_start(max_levels, 0);
+  std::vector<uint32_t> start;  // [level][2] -> tile index
   for (uint32_t k = 0; k + 1 < max_levels; ++k) {
-    level_start[k] = static_cast<uint32_t>(tiles.size());
-    for (uint32_t a = 0; a < count; ++a) {
-      if (views[a].num_levels <= k + 1) continue;
-      const uint32_t dsize = views[a].size[k + 1];
-      for (uint32_t j = 0; j < dsize; j += kBuildTile)
-        tiles.push_back(BuildTile{a, k, j, std::min(dsize, j + kBuildTile)});
+    for (uint32_t rl : {rmq::kMultiRLog, rmq::kLane8RLog}) {
+      start.push_back(static_cast<uint32_t>(tiles.size()));
+      for (uint32_t a = 0; a < count; ++a) {
+        if (rmq::variant_rlog(arrays[a].variant) != rl || views[a].num_levels <= k + 1) continue;
+        const uint32_t dsize = views[a].size[k + 1];
+        for (uint32_t j = 0; j < dsize; j += kBuildTile)
+          tiles.push_back(BuildTile{a, k, j, std::min(dsize, j + kBuildTile)});
+      }
     }
   }
-  if (max_levels >= 1) level_start[max_levels - 1] = static_cast<uint32_t>(tiles.size());
+  start.push_back(static_cast<uint32_t>(tiles.size()));
   BuildTile* d_tiles = nullptr;
   if (!tiles.empty()) {
     st = dmalloc(&d_tiles, tiles.size());
@@ -221,9 +223,11 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
             hipMemcpyAsync(b->d_variant, b->variant.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice, s) == hipSuccess;
   if (ok && !tiles.empty())
     ok = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(BuildTile), hipMemcpyHostToDevice, s) == hipSuccess;
-  for (uint32_t k = 0; ok && k + 1 < max_levels; ++k) {
-    const uint32_t t0 = level_start[k], t1 = level_start[k + 1];
-    if (t1 > t0) rmq::k_build_batch<<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0);
+  for (size_t g = 0; ok && g + 1 < start.size(); ++g) {
+    const uint32_t t0 = start[g], t1 = start[g + 1];
+    if (t1 == t0) continue;
+    if (g % 2 == 0) rmq::k_build_batch<rmq::kMultiRLog><<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0);
+    else rmq::k_build_batch<rmq::kLane8RLog><<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0);
     ok = hipGetLastError() == hipSuccess;
   }
   // the host vectors die here: wait for the copies (and the build) before returning
@@ -246,13 +250,13 @@ rmq_status rmq_batch_plan_queries(const rmq_batch* b, const uint64_t* qoffsets, 
     if (qoffsets[a + 1] < qoffsets[a]) return RMQ_EINVAL;
   std::vector<QueryTile> tiles;
   std::vector<uint32_t> qstart;
-  rmq::make_query_tiles(b->count, qoffsets, tiles, qstart);
+  if (!rmq::make_query_tiles(b->count, qoffsets, tiles, qstart)) return RMQ_EINVAL;
   rmq_batch_plan* p = new (std::nothrow) rmq_batch_plan();
   if (!p) return RMQ_ENOMEM;
   p->ntiles = static_cast<uint32_t>(tiles.size());
   rmq_status st = dmalloc(&p->d_tiles, tiles.size());
   if (st == RMQ_OK) st = dmalloc(&p->d_qstart, kQueues + 1);
-  if (st == RMQ_OK) st = dmalloc(&p->d_counter, kQueues);
+  if (st == RMQ_OK) st = dmalloc(&p->d_counter, rmq::kCounterWords);
   if (st != RMQ_OK) {
     rmq_batch_plan_free(p);
     return st;
@@ -276,8 +280,8 @@ rmq_status rmq_batch_query(const rmq_batch* b, const rmq_batch_plan* p, const ui
   if (!d_lr || (!d_val && !d_idx) || (reinterpret_cast<uintptr_t>(d_lr) & 7u) != 0) return RMQ_EINVAL;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   const uint2* Q = reinterpret_cast<const uint2*>(d_lr);
-  rmq::launch_multi_query(b->d_views, b->d_variant, p->d_tiles, p->d_qstart, p->d_counter, Q,
-                          d_val, d_idx, s);
+  rmq::launch_multi_query(b->d_views, b->d_variant, p->d_tiles, p->d_qstart, p->d_counter, p->ntiles,
+                          Q, d_val, d_idx, s);
   return err();
 }
 

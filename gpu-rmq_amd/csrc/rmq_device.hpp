@@ -153,6 +153,11 @@ __device__ __forceinline__ uint4 load4u(const uint32_t* __restrict__ base, uint3
   return r;
 }
 
+// Element j (0..3) of a vector.
+__device__ __forceinline__ float pick4(float4 x, uint32_t j) {
+  return j == 0 ? x.x : (j == 1 ? x.y : (j == 2 ? x.z : x.w));
+}
+
 __device__ __forceinline__ bool in_range(uint32_t p, uint32_t lo, uint32_t hi) {
   return p - lo < hi - lo;  // lo <= p < hi, with lo <= hi
 }

@@ -433,23 +433,33 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
   const uint32_t szK = H.size[K];
   const bool left_tail = K > 0 && l < (lk << shK);
   const bool right_tail = K > 0 && (ek << shK) < e;
-  const float gL = left_tail ? vK[lk - 1] : RMQ_INF;
-  const float gR = right_tail ? vK[ek] : RMQ_INF;
+  // Guard entries lk - 1 and ek: unless a range edge is 4-aligned (or the final range is
+  // empty) they lie in the first / last 16-byte vector the scan reads anyway.
+  const bool nonempty = lk < ek;
+  float gL = (left_tail && (!nonempty || (lk & 3u) == 0)) ? vK[lk - 1] : RMQ_INF;
+  float gR = (right_tail && (!nonempty || (ek & 3u) == 0)) ? vK[ek] : RMQ_INF;
   float bv = RMQ_INF;
   uint32_t bpos = dev::kNoIdx;
-  for (uint32_t p = lk & ~3u; p < ek; p += 4) {
-    const float4 x = dev::load4(vK, p, szK);
-    if constexpr (IDX) {
-      const float xs[4] = {x.x, x.y, x.z, x.w};
+  if (nonempty) {
+    float4 x = dev::load4(vK, lk & ~3u, szK);
+    if (left_tail && (lk & 3u)) gL = dev::pick4(x, (lk - 1) & 3u);
+    for (uint32_t p = lk & ~3u;;) {
+      if constexpr (IDX) {
+        const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-      for (uint32_t j = 0; j < 4; ++j) {
-        const bool t = dev::in_range(p + j, lk, ek) && (xs[j] < bv || bpos == dev::kNoIdx);
-        bv = t ? xs[j] : bv;
-        bpos = t ? p + j : bpos;
+        for (uint32_t j = 0; j < 4; ++j) {
+          const bool t = dev::in_range(p + j, lk, ek) && (xs[j] < bv || bpos == dev::kNoIdx);
+          bv = t ? xs[j] : bv;
+          bpos = t ? p + j : bpos;
+        }
+      } else {
+        bv = dev::fold4(bv, x, p, lk, ek);
       }
-    } else {
-      bv = dev::fold4(bv, x, p, lk, ek);
+      p += 4;
+      if (p >= ek) break;
+      x = dev::load4(vK, p, szK);
     }
+    if (right_tail && (ek & 3u)) gR = dev::pick4(x, ek & 3u);  // x is the vector holding ek - 1
   }
   dev::Cand b;
   b.v = bv;

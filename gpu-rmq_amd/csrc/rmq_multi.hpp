@@ -1,12 +1,18 @@
 // rmq_multi.hpp — multi-array query machinery shared by the batched form (rmq_batch.hip) and
 // the serving loop (rmq_serve.hip): a table of per-array HierViews in global memory, 256-query
-// tiles, and one persistent kernel that pulls tiles from per-XCD work queues.
+// tiles, and one persistent kernel whose wavefronts pull work from per-XCD queues.
 //
 // XCD-aware scheduling: workgroups are dispatched round-robin over the 8 XCDs of an MI355X,
-// so workgroup b runs on XCD b % 8. Arrays are assigned to 8 queues and workgroups pull from
-// the queue of their own XCD first, so an array's upper levels are cached in one XCD's L2; an
-// idle workgroup then steals from the other queues. Correctness does not depend on the
-// dispatch order: every tile is taken exactly once through an atomic counter.
+// so workgroup b runs on XCD b % 8. The tiles, ordered by array, are cut into 8 contiguous
+// queues of about equal query count, and the wavefronts of a workgroup work on the queue of
+// their own XCD first, so an array's upper levels are cached in few XCDs' L2. A queue's first
+// half is split statically among its home wavefronts (no atomics); the second half is handed
+// out in 64-query units, kClaimUnits per atomic increment of the queue's counter, so that
+// wavefronts that finish early balance the rest; an idle wavefront then steals from the other
+// queues. Each counter sits on its own 256-byte line. Global atomics are performed past the
+// XCD-private L2s, so every increment is a fabric round trip and increments of one counter
+// serialize: the static half and the multi-unit claims keep them to a few thousand per launch.
+// Correctness does not depend on the dispatch order: every unit is taken exactly once.
 #pragma once
 
 #include <algorithm>
@@ -19,10 +25,43 @@
 
 namespace rmq {
 
-constexpr uint32_t kMultiRLog = 5;       // R = 32 for every array of a batch
+constexpr uint32_t kMultiRLog = 5;       // R = 32: group variants and RMQ_VARIANT_LANE
+constexpr uint32_t kLane8RLog = 3;       // R = 8: RMQ_VARIANT_LANE8
 constexpr uint32_t kMultiR = 1u << kMultiRLog;
+
+// Reduction factor (log2) and plan parameters of a variant.
+__host__ __device__ inline uint32_t variant_rlog(uint32_t variant) {
+  return variant == RMQ_VARIANT_LANE8 ? kLane8RLog : kMultiRLog;
+}
+
+inline bool variant_ok(uint32_t v) {
+  return v == RMQ_VARIANT_GROUP4 || v == RMQ_VARIANT_GROUP8 || v == RMQ_VARIANT_GROUP16 ||
+         v == RMQ_VARIANT_LANE || v == RMQ_VARIANT_LANE8;
+}
+
+// The plain-ABI build parameters that produce the hierarchy a variant reads.
+inline void variant_build(uint32_t variant, int* alg, rmq_build_options* o) {
+  if (variant == RMQ_VARIANT_LANE8) {
+    *alg = RMQ_ALG_VECTOR_LOAD;  // R = 4 * 2^1
+    *o = rmq_build_options{0, 1, 0, RMQ_BUILD_DEFAULT_CG};
+  } else {
+    *alg = RMQ_ALG_XXX;          // R = 2^(3 + 2)
+    *o = rmq_build_options{3, 2, variant == RMQ_VARIANT_LANE ? 0u : variant, 0};
+  }
+}
 constexpr uint32_t kQueues = 8;          // XCDs of an MI355X
 constexpr uint32_t kTileQueries = kBlock;
+constexpr uint32_t kWaveQueries = 64;
+constexpr uint32_t kUnitsPerTile = kTileQueries / kWaveQueries;
+#ifndef RMQ_CLAIM_UNITS
+#define RMQ_CLAIM_UNITS 4
+#endif
+constexpr uint32_t kClaimUnits = RMQ_CLAIM_UNITS;  // units one counter increment hands to a wavefront
+constexpr uint32_t kCounterStride = 64;  // uint32 words between queue counters (256 bytes)
+constexpr uint32_t kCounterWords = kQueues * kCounterStride;
+// Tiles one launch may hold: keeps every counter value (units plus the failed claims of all
+// wavefronts) far below 2^32.
+constexpr uint32_t kMaxTiles = 1u << 28;
 
 struct QueryTile {
   uint32_t array;
@@ -30,19 +69,21 @@ struct QueryTile {
   uint64_t q0;     // first query index (in the query / output buffers of the launch)
 };
 
-template <int N>
+// Zeroes the queue counters (a kernel, not a memset, so that it can sit in a captured graph).
+template <uint32_t N>
 __global__ void k_reset_counters(uint32_t* __restrict__ c) {
-  if (threadIdx.x < N) c[threadIdx.x] = 0;
+  if (threadIdx.x < N) c[threadIdx.x * kCounterStride] = 0;
 }
 
 // ---- query: persistent, tiles pulled from per-XCD queues ----------------------------------------
+// One 64-query unit: queries q0 .. q0 + count - 1 (count <= 64), one per lane of the wavefront.
 template <int G, bool IDX>
-__device__ __forceinline__ void tile_groups(const HierView& H, const uint2* __restrict__ Q,
+__device__ __forceinline__ void unit_groups(const HierView& H, const uint2* __restrict__ Q,
                                             uint64_t q0, uint32_t count, float* __restrict__ ov,
                                             uint32_t* __restrict__ oi) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t gl = lane & (G - 1);
-  const uint32_t local = threadIdx.x;  // one query per lane, 256 per tile
+  const uint32_t local = lane;
   const bool valid = local < count;
   uint32_t l = 0, e = 0;
   if (valid) {
@@ -80,19 +121,57 @@ __device__ __forceinline__ void tile_groups(const HierView& H, const uint2* __re
   }
 }
 
-template <bool IDX>
-__device__ __forceinline__ void tile_lanes(const HierView& H, const uint2* __restrict__ Q,
+template <int RL, bool IDX>
+__device__ __forceinline__ void unit_lanes(const HierView& H, const uint2* __restrict__ Q,
                                            uint64_t q0, uint32_t count, float* __restrict__ ov,
                                            uint32_t* __restrict__ oi) {
-  const uint32_t local = threadIdx.x;
+  const uint32_t local = threadIdx.x & 63;
   if (local >= count) return;
   const uint2 p = dev::load_query(Q, q0 + local);
-  GroupAnswer a = solve_vl_pruned<kMultiRLog, IDX>(p.x, p.y + 1, H);
+  GroupAnswer a = solve_vl_pruned<RL, IDX>(p.x, p.y + 1, H);
   if constexpr (!IDX) {
-    if (a.v == 0.0f) a = solve_vl_pruned<kMultiRLog, true>(p.x, p.y + 1, H);
+    if (a.v == 0.0f) a = solve_vl_pruned<RL, true>(p.x, p.y + 1, H);
   }
   if (ov) ov[q0 + local] = a.v;
   if (IDX && oi) oi[q0 + local] = a.idx;
+}
+
+// Answers unit u (64 queries) of queue q with the variant of the unit's array.
+template <bool IDX>
+__device__ __forceinline__ void run_unit(const HierView* __restrict__ views,
+                                         const uint32_t* __restrict__ variant,
+                                         const QueryTile* __restrict__ tiles, uint32_t first, uint32_t u,
+                                         const uint2* __restrict__ Q, float* __restrict__ ov,
+                                         uint32_t* __restrict__ oi) {
+  const QueryTile T = tiles[first + u / kUnitsPerTile];
+  const uint32_t part = (u % kUnitsPerTile) * kWaveQueries;
+  if (part >= T.count) return;  // the unused quarters of a segment's last tile
+  const uint32_t cnt = min(T.count - part, kWaveQueries);
+  const uint64_t q0 = T.q0 + part;
+  const uint32_t a = __builtin_amdgcn_readfirstlane(T.array);
+  const HierView& H = views[a];
+  switch (variant[a]) {
+    case RMQ_VARIANT_GROUP4: unit_groups<4, IDX>(H, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_GROUP16: unit_groups<16, IDX>(H, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_LANE: unit_lanes<kMultiRLog, IDX>(H, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_LANE8: unit_lanes<kLane8RLog, IDX>(H, Q, q0, cnt, ov, oi); break;
+    default: unit_groups<8, IDX>(H, Q, q0, cnt, ov, oi); break;
+  }
+}
+
+// Wavefronts whose home is queue q (those of workgroups q, q + 8, ...) and the static share of
+// each: the first half of a queue's units is split evenly among its home wavefronts, which take
+// their share without touching the counter; the rest is handed out kClaimUnits at a time.
+__device__ __forceinline__ void queue_shape(const uint32_t* __restrict__ qstart, uint32_t q,
+                                            uint32_t& first, uint32_t& units, uint32_t& dyn0,
+                                            uint32_t& share) {
+  constexpr uint32_t kWaves = kBlock / kWaveQueries;
+  first = qstart[q];
+  units = (qstart[q + 1] - first) * kUnitsPerTile;
+  const uint32_t blocks = gridDim.x > q ? (gridDim.x - q + kQueues - 1) / kQueues : 0;
+  const uint32_t waves = blocks * kWaves;
+  share = waves ? units / (2 * waves) : 0;
+  dyn0 = share * waves;
 }
 
 template <bool IDX>
@@ -101,66 +180,72 @@ __global__ __launch_bounds__(kBlock) void k_query_batch(
     const QueryTile* __restrict__ tiles, const uint32_t* __restrict__ qstart,
     uint32_t* __restrict__ counter, const uint2* __restrict__ Q, float* __restrict__ ov,
     uint32_t* __restrict__ oi) {
-  __shared__ uint32_t s_tile;
+  // Queues a wavefront of this workgroup found drained; the others skip them.
+  __shared__ uint32_t s_drained;
+  if (threadIdx.x == 0) s_drained = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63;
   const uint32_t home = blockIdx.x % kQueues;
+  {
+    uint32_t first, units, dyn0, share;
+    queue_shape(qstart, home, first, units, dyn0, share);
+    const uint32_t w = (blockIdx.x / kQueues) * (kBlock / kWaveQueries) + threadIdx.x / kWaveQueries;
+    for (uint32_t u = w * share; u < (w + 1) * share; ++u)
+      run_unit<IDX>(views, variant, tiles, first, u, Q, ov, oi);
+  }
   uint32_t queue = home;
   uint32_t tried = 0;
   while (tried < kQueues) {
-    if (threadIdx.x == 0) {
-      const uint32_t len = qstart[queue + 1] - qstart[queue];
-      const uint32_t t = atomicAdd(&counter[queue], 1u);
-      s_tile = t < len ? qstart[queue] + t : 0xFFFFFFFFu;
-    }
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    __syncthreads();
-    if (tile == 0xFFFFFFFFu) {  // this queue is drained: move to the next one
+    uint32_t first, units, dyn0, share;
+    queue_shape(qstart, queue, first, units, dyn0, share);
+    uint32_t u0 = units;
+    if (lane == 0 && !(__hip_atomic_load(&s_drained, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> queue & 1u))
+      u0 = dyn0 + atomicAdd(&counter[queue * kCounterStride], kClaimUnits);
+    u0 = __builtin_amdgcn_readfirstlane(u0);
+    if (u0 >= units) {  // this queue is drained: move to the next one
+      if (lane == 0)
+        __hip_atomic_fetch_or(&s_drained, 1u << queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       queue = (queue + 1) % kQueues;
       ++tried;
       continue;
     }
-    const QueryTile T = tiles[tile];
-    const uint32_t a = __builtin_amdgcn_readfirstlane(T.array);
-    const HierView& H = views[a];
-    switch (variant[a]) {
-      case RMQ_VARIANT_GROUP4: tile_groups<4, IDX>(H, Q, T.q0, T.count, ov, oi); break;
-      case RMQ_VARIANT_GROUP16: tile_groups<16, IDX>(H, Q, T.q0, T.count, ov, oi); break;
-      case RMQ_VARIANT_LANE: tile_lanes<IDX>(H, Q, T.q0, T.count, ov, oi); break;
-      default: tile_groups<8, IDX>(H, Q, T.q0, T.count, ov, oi); break;
-    }
+    const uint32_t u1 = min(u0 + kClaimUnits, units);
+    for (uint32_t u = u0; u < u1; ++u) run_unit<IDX>(views, variant, tiles, first, u, Q, ov, oi);
   }
 }
 
-// Tiles of 256 queries per segment, grouped into kQueues queues balanced by query count.
-// qoffsets[a]..qoffsets[a+1] are segment a's queries, on array ids[a] (ids == nullptr: array a);
-// returns the tiles and the queue starts.
-inline void make_query_tiles(uint32_t count, const uint64_t* qoffsets, std::vector<QueryTile>& tiles,
+// Tiles of 256 queries per segment, in segment order sorted by array (segments of one array
+// adjacent), cut into kQueues contiguous queues of about equal query count. qoffsets[a] ..
+// qoffsets[a+1] are segment a's queries, on array ids[a] (ids == nullptr: array a). Returns the
+// tiles and the queue starts, or false when they would exceed kMaxTiles.
+inline bool make_query_tiles(uint32_t count, const uint64_t* qoffsets, std::vector<QueryTile>& tiles,
                              std::vector<uint32_t>& qstart, const uint32_t* ids = nullptr) {
+  uint64_t need = 0;
+  for (uint32_t a = 0; a < count; ++a) need += (qoffsets[a + 1] - qoffsets[a] + kTileQueries - 1) / kTileQueries;
+  if (need > kMaxTiles) return false;
   std::vector<uint32_t> order(count);
   std::iota(order.begin(), order.end(), 0u);
-  std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-    return qoffsets[x + 1] - qoffsets[x] > qoffsets[y + 1] - qoffsets[y];
-  });
-  std::vector<uint64_t> load(kQueues, 0);
-  std::vector<std::vector<uint32_t>> members(kQueues);
-  for (uint32_t a : order) {
-    const uint32_t qn = static_cast<uint32_t>(std::min_element(load.begin(), load.end()) - load.begin());
-    load[qn] += qoffsets[a + 1] - qoffsets[a];
-    members[qn].push_back(a);
-  }
+  if (ids)
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return ids[x] < ids[y]; });
+  uint64_t total = 0;
+  for (uint32_t a = 0; a < count; ++a) total += qoffsets[a + 1] - qoffsets[a];
   tiles.clear();
+  tiles.reserve(need);
   qstart.assign(kQueues + 1, 0);
-  for (uint32_t qn = 0; qn < kQueues; ++qn) {
-    qstart[qn] = static_cast<uint32_t>(tiles.size());
-    std::sort(members[qn].begin(), members[qn].end());
-    for (uint32_t a : members[qn]) {
-      for (uint64_t q = qoffsets[a]; q < qoffsets[a + 1]; q += kTileQueries) {
-        const uint64_t c = std::min<uint64_t>(kTileQueries, qoffsets[a + 1] - q);
-        tiles.push_back(QueryTile{ids ? ids[a] : a, static_cast<uint32_t>(c), q});
-      }
+  uint32_t queue = 0;
+  uint64_t done = 0;
+  for (uint32_t a : order) {
+    for (uint64_t q = qoffsets[a]; q < qoffsets[a + 1]; q += kTileQueries) {
+      // queue k holds the tiles that start before k+1 eighths of all queries
+      while (queue + 1 < kQueues && done * kQueues >= total * (queue + 1)) qstart[++queue] = static_cast<uint32_t>(tiles.size());
+      const uint64_t c = std::min<uint64_t>(kTileQueries, qoffsets[a + 1] - q);
+      tiles.push_back(QueryTile{ids ? ids[a] : a, static_cast<uint32_t>(c), q});
+      done += c;
     }
   }
+  while (queue + 1 < kQueues) qstart[++queue] = static_cast<uint32_t>(tiles.size());
   qstart[kQueues] = static_cast<uint32_t>(tiles.size());
+  return true;
 }
 
 inline uint32_t persistent_grid(const void* kernel) {
@@ -173,17 +258,22 @@ inline uint32_t persistent_grid(const void* kernel) {
   return (g + kQueues - 1) / kQueues * kQueues;
 }
 
-// Counter reset + persistent query launch over the tiles of one plan.
+// Counter reset + persistent query launch over the ntiles tiles of one plan. The grid is the
+// resident maximum, or fewer workgroups when a small launch would leave most wavefronts with
+// nothing to claim (each workgroup's first round claims kClaimUnits units per wavefront).
 inline void launch_multi_query(const HierView* views, const uint32_t* variant, const QueryTile* tiles,
-                               const uint32_t* qstart, uint32_t* counter, const uint2* Q, float* ov,
-                               uint32_t* oi, hipStream_t s) {
+                               const uint32_t* qstart, uint32_t* counter, uint32_t ntiles,
+                               const uint2* Q, float* ov, uint32_t* oi, hipStream_t s) {
+  constexpr uint32_t kTilesPerRound = kBlock / kWaveQueries * kClaimUnits / kUnitsPerTile;
+  const uint32_t want = (ntiles + kTilesPerRound - 1) / kTilesPerRound;
+  const uint32_t small = (want + kQueues - 1) / kQueues * kQueues;
   k_reset_counters<kQueues><<<1, 64, 0, s>>>(counter);
   if (oi) {
     static const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(k_query_batch<true>));
-    k_query_batch<true><<<grid, kBlock, 0, s>>>(views, variant, tiles, qstart, counter, Q, ov, oi);
+    k_query_batch<true><<<std::min(grid, small), kBlock, 0, s>>>(views, variant, tiles, qstart, counter, Q, ov, oi);
   } else {
     static const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(k_query_batch<false>));
-    k_query_batch<false><<<grid, kBlock, 0, s>>>(views, variant, tiles, qstart, counter, Q, ov, oi);
+    k_query_batch<false><<<std::min(grid, small), kBlock, 0, s>>>(views, variant, tiles, qstart, counter, Q, ov, oi);
   }
 }
 

@@ -1,6 +1,6 @@
 // rmq_serve.hip — serving loop over device-resident hierarchies (include/rmq_serve.h).
 //
-// Resident arrays are ordinary hierarchies (rmq_build_ex, R = 32) whose level views sit in a
+// Resident arrays are ordinary hierarchies (rmq_build_ex; R = 8 or 32 by variant) whose level views sit in a
 // device table; a batch is a set of pending requests packed into one slot's staging buffers and
 // answered by the multi-array persistent kernel (rmq_multi.hpp), one request = one segment of
 // tiles on its array. Each slot owns a stream, pinned host staging and device buffers, so up to
@@ -126,7 +126,7 @@ bool init_slot(Slot& sl, uint64_t max_q, uint32_t max_tiles) {
        dalloc(reinterpret_cast<void**>(&sl.d_idx), max_q * 4) &&
        dalloc(reinterpret_cast<void**>(&sl.d_tiles), max_tiles * sizeof(rmq::QueryTile)) &&
        dalloc(reinterpret_cast<void**>(&sl.d_qstart), (rmq::kQueues + 1) * 4) &&
-       dalloc(reinterpret_cast<void**>(&sl.d_counter), rmq::kQueues * 4);
+       dalloc(reinterpret_cast<void**>(&sl.d_counter), rmq::kCounterWords * 4);
   return ok;
 }
 
@@ -170,7 +170,8 @@ rmq_status launch(rmq_server* s, Slot& sl) {
   qoff.push_back(total);
   std::vector<rmq::QueryTile> tiles;
   std::vector<uint32_t> qstart;
-  rmq::make_query_tiles(static_cast<uint32_t>(ids.size()), qoff.data(), tiles, qstart, ids.data());
+  // within kMaxTiles: at most max_batch_queries / 256 + max_batch_requests tiles
+  (void)rmq::make_query_tiles(static_cast<uint32_t>(ids.size()), qoff.data(), tiles, qstart, ids.data());
   std::memcpy(sl.h_tiles, tiles.data(), tiles.size() * sizeof(rmq::QueryTile));
   std::memcpy(sl.h_qstart, qstart.data(), qstart.size() * 4);
   const hipStream_t st = sl.stream;
@@ -180,7 +181,7 @@ rmq_status launch(rmq_server* s, Slot& sl) {
             hipMemcpyAsync(sl.d_qstart, sl.h_qstart, qstart.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
   if (ok) {
     rmq::launch_multi_query(s->d_views, s->d_variant, sl.d_tiles, sl.d_qstart, sl.d_counter,
-                            reinterpret_cast<const uint2*>(sl.d_q), sl.d_val,
+                            static_cast<uint32_t>(tiles.size()), reinterpret_cast<const uint2*>(sl.d_q), sl.d_val,
                             sl.want_idx ? sl.d_idx : nullptr, st);
     ok = hipGetLastError() == hipSuccess;
   }
@@ -194,11 +195,6 @@ rmq_status launch(rmq_server* s, Slot& sl) {
   s->stats.batches++;
   s->stats.queries += total;
   return ok ? RMQ_OK : RMQ_EDEVICE;
-}
-
-bool variant_ok(uint32_t v) {
-  return v == RMQ_VARIANT_GROUP4 || v == RMQ_VARIANT_GROUP8 || v == RMQ_VARIANT_GROUP16 ||
-         v == RMQ_VARIANT_LANE;
 }
 
 }  // namespace
@@ -217,6 +213,10 @@ rmq_status rmq_server_create(const rmq_server_options* opt, rmq_server** out) {
   if (!s->opt.max_arrays) s->opt.max_arrays = 1024;
   if (!s->opt.max_batch_requests) s->opt.max_batch_requests = 4096;
   s->max_tiles = s->opt.max_batch_queries / rmq::kTileQueries + s->opt.max_batch_requests;
+  if (s->opt.max_batch_requests > rmq::kMaxTiles / 2) {
+    delete s;
+    return RMQ_EINVAL;
+  }
   bool ok = hipStreamCreateWithFlags(&s->admin, hipStreamNonBlocking) == hipSuccess &&
             dalloc(reinterpret_cast<void**>(&s->d_views), s->opt.max_arrays * sizeof(rmq::HierView)) &&
             dalloc(reinterpret_cast<void**>(&s->d_variant), s->opt.max_arrays * 4) &&
@@ -249,13 +249,15 @@ void rmq_server_destroy(rmq_server* s) {
 
 rmq_status rmq_server_admit(rmq_server* s, int source, const float* data, uint64_t n, uint64_t seed,
                             uint32_t t, uint32_t variant, int32_t* array_id) {
-  if (!s || !array_id || n == 0 || n > 0x7FFFFFFFull || !variant_ok(variant)) return RMQ_EINVAL;
+  if (!s || !array_id || n == 0 || n > 0x7FFFFFFFull || !rmq::variant_ok(variant)) return RMQ_EINVAL;
   if (source != RMQ_ADMIT_DEVICE && source != RMQ_ADMIT_HOST && source != RMQ_ADMIT_UNIFORM)
     return RMQ_EINVAL;
   if (source != RMQ_ADMIT_UNIFORM && !data) return RMQ_EINVAL;
-  rmq_build_options o{3, 2, variant == RMQ_VARIANT_LANE ? 0u : variant, 0};
+  int alg = 0;
+  rmq_build_options o;
+  rmq::variant_build(variant, &alg, &o);
   uint64_t floats = 0;
-  rmq_status st = rmq_plan(n, t, RMQ_ALG_XXX, &o, nullptr, nullptr, nullptr, &floats, nullptr);
+  rmq_status st = rmq_plan(n, t, alg, &o, nullptr, nullptr, nullptr, &floats, nullptr);
   if (st != RMQ_OK) return st;
   const uint64_t need = floats * 8 + (source == RMQ_ADMIT_DEVICE ? 0 : n * 4) + (1u << 16);
   size_t free_b = 0, total_b = 0;
@@ -283,7 +285,7 @@ rmq_status rmq_server_admit(rmq_server* s, int source, const float* data, uint64
     }
     A = r.owned;
   }
-  if (st == RMQ_OK) st = rmq_build_ex(A, n, t, RMQ_ALG_XXX, &o, s->admin, &r.h);
+  if (st == RMQ_OK) st = rmq_build_ex(A, n, t, alg, &o, s->admin, &r.h);
   if (st == RMQ_OK) {
     s->h_views[id] = *rmq_internal_view(r.h);
     s->h_variant[id] = variant;

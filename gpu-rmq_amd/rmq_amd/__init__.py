@@ -274,7 +274,8 @@ def gpu_base(array, lr, want_val: bool = True, want_idx: bool = False, stream=No
 
 # ---- batched form (include/rmq_batch.h) ------------------------------------------------------
 
-VARIANTS = {"lane": _lib.RMQ_VARIANT_LANE, "group4": _lib.RMQ_VARIANT_GROUP4,
+VARIANTS = {"lane": _lib.RMQ_VARIANT_LANE, "lane8": _lib.RMQ_VARIANT_LANE8,
+            "group4": _lib.RMQ_VARIANT_GROUP4,
             "group8": _lib.RMQ_VARIANT_GROUP8, "group16": _lib.RMQ_VARIANT_GROUP16}
 
 

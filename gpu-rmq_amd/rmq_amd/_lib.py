@@ -27,6 +27,7 @@ RMQ_MAX_LEVELS = 20
 RMQ_BUILD_DEFAULT_CG = 0x1
 RMQ_BUILD_FULL = 0x2
 RMQ_VARIANT_LANE = 1
+RMQ_VARIANT_LANE8 = 2
 RMQ_VARIANT_GROUP4 = 4
 RMQ_VARIANT_GROUP8 = 8
 RMQ_VARIANT_GROUP16 = 16

@@ -23,13 +23,14 @@ import numpy as np
 from . import workloads
 
 TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_gfx950.json")
-VARIANTS = ("lane", "group4", "group8", "group16")
+VARIANTS = ("lane8", "lane", "group4", "group8")
 THRESHOLDS = (64, 256, 1024, 4096, 32768)
+TOTAL_QUERIES = 1 << 23
 SIZE_LOGS = tuple(range(10, 29, 2))
 KINDS = ("small", "medium", "large", "mixed")
 
-# Used when no table has been written yet: one lane per query at the shallowest hierarchy.
-DEFAULT = {"variant": "lane", "t": 64}
+# Used when no table has been written yet: one lane per query on an R = 8 hierarchy.
+DEFAULT = {"variant": "lane8", "t": 1024}
 
 _table_cache = None
 
@@ -107,7 +108,7 @@ def _time_batch(torch, rmq_amd, data, variant: str, t: int) -> float:
         return e0.elapsed_time(e1) / 5
 
 
-def tune(total_queries: int = 1 << 20, verbose: bool = True) -> Dict:
+def tune(total_queries: int = TOTAL_QUERIES, verbose: bool = True) -> Dict:
     import torch
 
     import rmq_amd
@@ -153,7 +154,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--write", action="store_true", help="persist the table")
     ap.add_argument("--out", default=TABLE_PATH, help="where --write puts it")
-    ap.add_argument("--queries", type=int, default=1 << 20)
+    ap.add_argument("--queries", type=int, default=TOTAL_QUERIES)
     a = ap.parse_args()
     table = tune(a.queries)
     if a.write:

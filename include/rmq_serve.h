@@ -51,9 +51,10 @@ rmq_status rmq_server_create(const rmq_server_options* opt, rmq_server** out);
 /* Waits for in-flight batches, frees everything the server owns. NULL is a no-op. */
 void rmq_server_destroy(rmq_server* s);
 
-/* Build and keep resident an array of n floats (R = 32, scan threshold t >= 64, execution
- * variant RMQ_VARIANT_*). Returns RMQ_ENOMEM, with nothing allocated, when the array (if the
- * server owns it) and its levels do not fit in free HBM minus the reserve. */
+/* Build and keep resident an array of n floats with scan threshold t and execution variant
+ * RMQ_VARIANT_* (R = 8 for RMQ_VARIANT_LANE8, t >= 8; R = 32 otherwise, t >= 64). Returns
+ * RMQ_ENOMEM, with nothing allocated, when the array (if the server owns it) and its levels do
+ * not fit in free HBM minus the reserve. */
 rmq_status rmq_server_admit(rmq_server* s, int source, const float* data, uint64_t n, uint64_t seed,
                             uint32_t t, uint32_t variant, int32_t* array_id);
 

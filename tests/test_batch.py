@@ -80,9 +80,9 @@ def test_batch_equals_plain_and_oracle(device, want):
         A_host[o:o + n] = rng.integers(0, 3, size=n).astype(np.float32) * \
             np.where(rng.random(n) < 0.5, np.float32(-1), np.float32(1))
     dA.copy_(torch.from_numpy(A_host))
-    variants = ["group8", "group4", "group16", "lane"]
+    variants = ["group8", "group4", "group16", "lane", "lane8"]
     t = [64 << (a % 5) for a in range(96)]
-    var = [variants[a % 4] for a in range(96)]
+    var = [variants[a % 5] for a in range(96)]
     wv, wi = want in ("value",), want in ("index",)
     with rmq_amd.Batch(dA, w.offsets, w.sizes, t, var) as b:
         plan = b.plan(w.qoffsets)
@@ -100,7 +100,8 @@ def test_batch_equals_plain_and_oracle(device, want):
         if wi:
             assert np.array_equal(bi[q0:q1], ref_i), (a, var[a])
         # per-array plain launch through the C ABI gives the same answers
-        with rmq_amd.Hierarchy(dA[o:o + n], t[a], 16) as h:
+        plain = dict(alg=20, cg_amount_log=1) if var[a] == "lane8" else dict(alg=16)
+        with rmq_amd.Hierarchy(dA[o:o + n], t[a], **plain) as h:
             pv, pi = h.query(dQ[q0:q1], True, True)
             torch.cuda.synchronize()
         assert np.array_equal(pv.cpu().numpy().view(np.uint32), bv[q0:q1].view(np.uint32))
