@@ -29,6 +29,21 @@ for _p in (ROOT, os.path.join(ROOT, "gpu-rmq_amd")):
 METRIC = "Gqueries/s + %HBM roofline at n=2^26,q=2^26; batched vs plain; serve p99"
 SEED = 27722
 LR_NAMES = {-1: "uniform-large", -2: "lognormal-medium", -3: "lognormal-small", -6: "mixed"}
+# The configuration timed when the sweep finds nothing more than 2% faster (lane per query,
+# R = 4, t = 2^13): keeps the timed kernel the one whose counters are committed under profiles/.
+DEFAULT_ALG, DEFAULT_T = "20/c0", 1 << 13
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(key):
+    """HBM bytes per launch (copy-calibrated FETCH_SIZE + WRITE_SIZE) recorded for this exact
+    kernel and workload by a rocprofv3 counter run, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            entry = json.load(f).get(key)
+    except (OSError, ValueError):
+        return None
+    return None if entry is None else entry["bytes_per_launch"]
 
 
 def parse():
@@ -39,11 +54,12 @@ def parse():
     ap.add_argument("--log-n", type=int, default=26)
     ap.add_argument("--log-q", type=int, default=26)
     ap.add_argument("--lr", type=int, default=-2)
-    ap.add_argument("--alg", default="", help="16, 16/g4, 24/g4, 20 or 20/c1 (empty = sweep)")
+    ap.add_argument("--alg", default="", help="16, 16/g4, 24/g4, 20, 20/c0 or 20/c1 (empty = sweep)")
     ap.add_argument("--t", type=int, default=0, help="scan threshold (0 = sweep)")
     ap.add_argument("--mode", choices=["index", "value", "both"], default="index")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-sweep", action="store_true", help="time the default alg 16, t=2^15")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help=f"time the default alg {DEFAULT_ALG}, t={DEFAULT_T} without a sweep")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--batch-arrays", type=int, default=4096)
@@ -189,10 +205,14 @@ def serve_leg(requests: int, rate: float):
 
     rng = np.random.default_rng(SEED)
     sizes = np.exp2(rng.uniform(20, 28, size=64)).astype(np.int64)
+    mrng = np.random.default_rng(SEED + 7)  # length samples for tuning only
     with rmq_amd.Server(slots=3, max_batch_queries=1 << 21) as srv:
         ids = []
         for a, n in enumerate(sizes):
-            var, t = autotune.choose(int(n), np.array([max(1, int(n ** 0.6))]))
+            # a resident array answers every length class: tune for the mix it will see
+            mix = np.concatenate([workloads.query_lengths(mrng, int(n), 256, k)
+                                  for k in ("small", "medium", "large")])
+            var, t = autotune.choose(int(n), mix)
             ids.append(srv.admit_uniform(int(n), seed=SEED + a, t=t, variant=var))
         ranks = np.arange(1, 65, dtype=np.float64)
         p = ranks ** -1.1
@@ -310,14 +330,14 @@ def main():
 
     # --- choose (alg, t): explicit, default, or a short sweep ------------------------------
     opts_of = {"16": {}, "16/g4": dict(group_lanes=4), "24/g4": dict(group_lanes=4), "20": {},
-               "20/c1": dict(cg_amount_log=1)}
+               "20/c0": dict(cg_amount_log=0), "20/c1": dict(cg_amount_log=1)}
     if args.alg and args.t:
         candidates = [(args.alg, args.t)]
     elif args.no_sweep:
-        candidates = [("20/c1", 1 << 15)]
+        candidates = [(DEFAULT_ALG, DEFAULT_T)]
     else:
         candidates = [(v, 1 << tl) for v in ("16", "16/g4", "24/g4") for tl in (6, 10, 15, 18)]
-        candidates += [(v, 1 << tl) for v in ("20", "20/c1") for tl in (6, 10, 13, 15)]
+        candidates += [(v, 1 << tl) for v in ("20", "20/c1", "20/c0") for tl in (10, 13, 15, 17)]
     sweep = []
     best = None
     for var, t in candidates:
@@ -330,6 +350,9 @@ def main():
         log(f"sweep alg={var} t={t}: {ev:.3f} ms/launch, {q / ev / 1e6:.2f} Gq/s")
         if best is None or ev < best[2]:
             best = (var, t, ev)
+    pref = [e for e in sweep if (e["alg"], e["t"]) == (DEFAULT_ALG, DEFAULT_T)]
+    if pref and pref[0]["ms"] <= best[2] * 1.02:
+        best = (DEFAULT_ALG, DEFAULT_T, pref[0]["ms"])
     var, t, _ = best
     alg = int(var.split("/")[0])
 
@@ -355,6 +378,7 @@ def main():
             assert torch.equal(out_v[sel].view(torch.int32), bv.view(torch.int32))
     gqs = q / (wall_ms / 1e3) / 1e9
     achieved = bytes_total / (ev_ms / 1e3) / 1e9
+    traffic_key = f"alg={var} t={t} n=2^{args.log_n} q=2^{args.log_q} lr={lr_mode} {args.mode}"
 
     result = {
         "metric": METRIC,
@@ -380,7 +404,8 @@ def main():
             "peak": roofline.HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / roofline.HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(traffic_key),
+            "traffic_key": traffic_key,
             "bytes_per_query": round(bytes_total / q, 3),
             "kernel_ms": round(ev_ms, 4),
         },

@@ -412,45 +412,50 @@ __device__ __forceinline__ GroupAnswer solve_vl(uint32_t l, uint32_t e, const Hi
 
 // Pruned evaluation order for alg 20 (same rules as solve_cl_pruned, one lane per query; the
 // climb stops when r - l <= R or the aligned bounds cross, src/xxx.h:903-913).
+// Lane-per-query pruned evaluation. The final range is taken at the highest level K whose
+// aligned blocks include one inside [l, e), found in O(1) from the length (any level whose
+// range is nonempty yields the same answer; this one leaves at most 2R - 2 entries to scan
+// below the top level). Sides need no position keys: every left piece lies left of all that was
+// read before it, and every right piece right of it, so a left element wins ties (folded right
+// to left with <=) and a right element must be strictly smaller (folded left to right with <).
+// In the final range, folded left to right with <, the first entry seeds the answer, so a range
+// of +inf answers with its leftmost entry.
 template <int RL, bool IDX>
 __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H) {
   constexpr uint32_t R = 1u << RL;
   constexpr uint32_t V = R / 4;
-  uint32_t K = 0, lk = l, ek = e;
-  for (uint32_t lev = 0; lev + 1 < H.num_levels; ++lev) {
-    if (K == lev && ek - lk > R) {
-      const uint32_t nl = (lk + R - 1) & ~(R - 1);
-      const uint32_t pr = ek & ~(R - 1);
-      if (pr >= nl) {
-        lk = nl >> RL;
-        ek = pr >> RL;
-        K = lev + 1;
-      }
-    }
+  const uint32_t top = H.num_levels - 1;
+  uint32_t K = min((31u - static_cast<uint32_t>(__clz(e - l))) / RL, top);
+  uint32_t shK = RL * K;
+  uint32_t lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
+  uint32_t ek = e >> shK;
+  if (lk >= ek) {  // K >= 1 here, and the level below holds a block inside [l, e)
+    --K;
+    shK -= RL;
+    lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
+    ek = e >> shK;
   }
-  const uint32_t shK = RL * K;
   const float* vK = level_values(H, K);
   const uint32_t szK = H.size[K];
-  const bool left_tail = K > 0 && l < (lk << shK);
-  const bool right_tail = K > 0 && (ek << shK) < e;
-  // Guard entries lk - 1 and ek: unless a range edge is 4-aligned (or the final range is
-  // empty) they lie in the first / last 16-byte vector the scan reads anyway.
-  const bool nonempty = lk < ek;
-  float gL = (left_tail && (!nonempty || (lk & 3u) == 0)) ? vK[lk - 1] : RMQ_INF;
-  float gR = (right_tail && (!nonempty || (ek & 3u) == 0)) ? vK[ek] : RMQ_INF;
+  const bool left_tail = l < (lk << shK);
+  const bool right_tail = (ek << shK) < e;
   float bv = RMQ_INF;
-  uint32_t bpos = dev::kNoIdx;
-  if (nonempty) {
-    float4 x = dev::load4(vK, lk & ~3u, szK);
-    if (left_tail && (lk & 3u)) gL = dev::pick4(x, (lk - 1) & 3u);
-    for (uint32_t p = lk & ~3u;;) {
+  uint32_t bpay = (K == 0 ? dev::kDirect : H.off[K]) + lk;
+  float gL = RMQ_INF, gR = RMQ_INF;
+  {
+    // guards lk - 1 and ek: in the first / last vector the scan reads unless 4-aligned
+    uint32_t p = lk & ~3u;
+    float4 x = dev::load4(vK, p, szK);
+    if (left_tail) gL = (lk & 3u) ? dev::pick4(x, (lk - 1u) & 3u) : vK[lk - 1u];
+    const uint32_t pay0 = bpay - lk;
+    for (;;) {
       if constexpr (IDX) {
         const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
-          const bool t = dev::in_range(p + j, lk, ek) && (xs[j] < bv || bpos == dev::kNoIdx);
+          const bool t = dev::in_range(p + j, lk, ek) && xs[j] < bv;
           bv = t ? xs[j] : bv;
-          bpos = t ? p + j : bpos;
+          bpay = t ? pay0 + p + j : bpay;
         }
       } else {
         bv = dev::fold4(bv, x, p, lk, ek);
@@ -459,56 +464,66 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
       if (p >= ek) break;
       x = dev::load4(vK, p, szK);
     }
-    if (right_tail && (ek & 3u)) gR = dev::pick4(x, ek & 3u);  // x is the vector holding ek - 1
+    if (right_tail) gR = (ek & 3u) ? dev::pick4(x, ek & 3u) : vK[ek];
   }
-  dev::Cand b;
-  b.v = bv;
-  b.key = bpos == dev::kNoIdx ? dev::kNoIdx : bpos << shK;
-  b.pay = bpos == dev::kNoIdx ? dev::kNoIdx : (K == 0 ? dev::kDirect : H.off[K]) + bpos;
-  if (left_tail && !(gL > bv)) {
+  if (left_tail && gL <= bv) {
     for (int k = static_cast<int>(K) - 1; k >= 0; --k) {
       const uint32_t sk = RL * static_cast<uint32_t>(k);
-      const uint32_t lo = static_cast<uint32_t>((static_cast<uint64_t>(l) + (1ull << sk) - 1) >> sk);
-      const uint32_t hi = (lo + R - 1) & ~(R - 1);
+      const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);
+      const uint32_t base = lo & ~(R - 1u);
       const float* vk = level_values(H, k);
-      const uint32_t pay = k == 0 ? dev::kDirect : H.off[k];
+      const uint32_t pay0 = k == 0 ? dev::kDirect : H.off[k];
+      float g = RMQ_INF;
 #pragma unroll
-      for (uint32_t s = 0; s < V; ++s) {
-        const uint32_t p = (lo & ~(R - 1)) + 4 * s;
-        if (p + 4 > lo && p < hi) {
+      for (int s = static_cast<int>(V) - 1; s >= 0; --s) {
+        const uint32_t p = base + 4u * static_cast<uint32_t>(s);
+        if ((lo & (R - 1u)) != 0u && p + 4u > lo) {  // the vectors of [lo, roundup_R(lo))
           const float4 x = dev::load4(vk, p, H.size[k]);
-          if constexpr (IDX) dev::cand_fold4(b, x, p, lo, hi, sk, pay);
-          else b.v = dev::fold4(b.v, x, p, lo, hi);
+          if (p <= lo - 1u) g = dev::pick4(x, (lo - 1u) & 3u);  // next guard, lo - 1
+          const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+          for (int j = 3; j >= 0; --j) {
+            const bool t = p + static_cast<uint32_t>(j) >= lo && xs[j] <= bv;
+            bv = t ? xs[j] : bv;
+            if constexpr (IDX) bpay = t ? pay0 + p + static_cast<uint32_t>(j) : bpay;
+          }
         }
       }
-      bv = b.v;
       if (k == 0 || !(l < (lo << sk))) break;
-      if (vk[lo - 1] > bv) break;
+      if ((lo & 3u) == 0) g = vk[lo - 1u];
+      if (g > bv) break;
     }
   }
   if (right_tail && gR < bv) {
     for (int k = static_cast<int>(K) - 1; k >= 0; --k) {
       const uint32_t sk = RL * static_cast<uint32_t>(k);
       const uint32_t hi = e >> sk;
-      const uint32_t lo = hi & ~(R - 1);
+      const uint32_t lo = hi & ~(R - 1u);
       const float* vk = level_values(H, k);
-      const uint32_t pay = k == 0 ? dev::kDirect : H.off[k];
+      const uint32_t pay0 = k == 0 ? dev::kDirect : H.off[k];
+      float g = RMQ_INF;
 #pragma unroll
       for (uint32_t s = 0; s < V; ++s) {
-        const uint32_t p = lo + 4 * s;
+        const uint32_t p = lo + 4u * s;
         if (p < hi) {
           const float4 x = dev::load4(vk, p, H.size[k]);
-          if constexpr (IDX) dev::cand_fold4(b, x, p, lo, hi, sk, pay);
-          else b.v = dev::fold4(b.v, x, p, lo, hi);
+          if (hi - p < 4u) g = dev::pick4(x, hi & 3u);  // next guard, hi
+          const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+          for (uint32_t j = 0; j < 4; ++j) {
+            const bool t = p + j < hi && xs[j] < bv;
+            bv = t ? xs[j] : bv;
+            if constexpr (IDX) bpay = t ? pay0 + p + j : bpay;
+          }
         }
       }
-      bv = b.v;
       if (k == 0 || !((hi << sk) < e)) break;
-      if (!(vk[hi] < bv)) break;
+      if ((hi & 3u) == 0) g = vk[hi];
+      if (!(g < bv)) break;
     }
   }
-  if constexpr (IDX) return GroupAnswer{b.v, dev::cand_index(b, H.li)};
-  else return GroupAnswer{b.v, dev::kNoIdx};
+  if constexpr (IDX) return GroupAnswer{bv, (bpay & dev::kDirect) ? (bpay & ~dev::kDirect) : H.li[bpay]};
+  else return GroupAnswer{bv, dev::kNoIdx};
 }
 
 template <int RL, bool P, bool IDX>

@@ -4,7 +4,11 @@
         --stats DIR/kt_kernel_stats.csv --fetch DIR/fetch_counter_collection.csv \
         --write DIR/write_counter_collection.csv --copy-fetch DIR/copyf_counter_collection.csv \
         --copy-write DIR/copyw_counter_collection.csv [--l2 DIR/l2_counter_collection.csv] \
+        [--sq DIR/sq_counter_collection.csv] [--traffic-json profiles/traffic.json --traffic-key K] \
         --command "..." > profiles/<kernel>_<workload>.md
+
+--traffic-json records the measured bytes per dispatch under key K (the bench's traffic_key for
+the same kernel and workload), so that bench.py reports it as roofline.traffic.
 
 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what the same counters read for a
 plain device copy of known size (bench.py --copy-probe: 1 GiB read + 1 GiB written per launch).
@@ -15,6 +19,8 @@ from __future__ import annotations
 import argparse
 import collections
 import csv
+import json
+import os
 
 
 def _per_kernel(path, counter):
@@ -55,6 +61,9 @@ def main():
     ap.add_argument("--copy-fetch", required=True)
     ap.add_argument("--copy-write", required=True)
     ap.add_argument("--l2")
+    ap.add_argument("--sq", help="counter CSV with VALUBusy, VALUUtilization, OccupancyPercent")
+    ap.add_argument("--traffic-json")
+    ap.add_argument("--traffic-key")
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("--title", default="")
     a = ap.parse_args()
@@ -97,7 +106,24 @@ def main():
         _, miss, _ = _match(_per_kernel(a.l2, "TCC_MISS_sum"), a.kernel)
         out.append(f"| L2 hits / misses per dispatch | {hit:.4g} / {miss:.4g} "
                    f"(hit rate {hit / (hit + miss):.3f}, {miss / q:.2f} misses per query) |")
+    if a.sq:
+        for counter, label in (("VALUBusy", "VALU busy (% of GPU time)"),
+                               ("VALUUtilization", "VALU utilization (% active lanes)"),
+                               ("OccupancyPercent", "occupancy (% of maximum waves)")):
+            vals = _per_kernel(a.sq, counter)
+            if any(a.kernel in k for k in vals):
+                out.append(f"| {label} | {_match(vals, a.kernel)[1]:.1f} |")
     print("\n".join(out))
+    if a.traffic_json and a.traffic_key:
+        table = {}
+        if os.path.exists(a.traffic_json):
+            with open(a.traffic_json) as f:
+                table = json.load(f)
+        table[a.traffic_key] = {"bytes_per_launch": round(fetch_b + write_b),
+                                "fetch_bytes": round(fetch_b), "write_bytes": round(write_b),
+                                "kernel": name, "avg_ns": round(avg_ns)}
+        with open(a.traffic_json, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
