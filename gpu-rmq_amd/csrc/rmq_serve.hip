@@ -3,8 +3,10 @@
 // Resident arrays are ordinary hierarchies (rmq_build_ex; R = 8 or 32 by variant) whose level views sit in a
 // device table; a batch is a set of pending requests packed into one slot's staging buffers and
 // answered by the multi-array persistent kernel (rmq_multi.hpp), one request = one segment of
-// tiles on its array. Each slot owns a stream, pinned host staging and device buffers, so up to
-// `slots` batches overlap on the device while the host packs the next one.
+// tiles on its array. Each slot owns a stream and pinned, device-coherent host buffers that the
+// kernel reads (queries, tiles) and writes (answers) in place: a batch costs two kernel
+// launches and an event, with no copy commands, so the host never waits on a copy engine.
+// Up to `slots` batches overlap on the device while the host packs the next one.
 
 #include <hip/hip_runtime.h>
 
@@ -43,6 +45,7 @@ struct Slot {
   bool busy = false;
   bool want_idx = false;
   uint64_t total = 0;
+  // pinned host memory, coherent with the device; d_* are the device's addresses of the same
   uint32_t* h_q = nullptr;
   float* h_val = nullptr;
   uint32_t* h_idx = nullptr;
@@ -53,7 +56,7 @@ struct Slot {
   uint32_t* d_idx = nullptr;
   rmq::QueryTile* d_tiles = nullptr;
   uint32_t* d_qstart = nullptr;
-  uint32_t* d_counter = nullptr;
+  uint32_t* d_counter = nullptr;  // device memory
   std::vector<Request> reqs;
   std::vector<uint64_t> offs;
 };
@@ -71,6 +74,23 @@ bool halloc(void** p, size_t bytes) {
   if (hipHostMalloc(p, bytes ? bytes : 4, hipHostMallocDefault) != hipSuccess) {
     (void)hipGetLastError();
     *p = nullptr;
+    return false;
+  }
+  return true;
+}
+
+// Pinned host memory that kernels read and write in place (coherent: no GPU caching, so the
+// host's writes before a launch and the kernel's writes before its completion are seen).
+template <typename T>
+bool zalloc(T** h, T** d, size_t bytes) {
+  if (hipHostMalloc(reinterpret_cast<void**>(h), bytes ? bytes : 4, hipHostMallocMapped | hipHostMallocCoherent) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    *h = nullptr;
+    return false;
+  }
+  if (hipHostGetDevicePointer(reinterpret_cast<void**>(d), *h, 0) != hipSuccess) {
+    (void)hipGetLastError();
     return false;
   }
   return true;
@@ -102,11 +122,6 @@ void free_slot(Slot& sl) {
   (void)hipHostFree(sl.h_idx);
   (void)hipHostFree(sl.h_tiles);
   (void)hipHostFree(sl.h_qstart);
-  (void)hipFree(sl.d_q);
-  (void)hipFree(sl.d_val);
-  (void)hipFree(sl.d_idx);
-  (void)hipFree(sl.d_tiles);
-  (void)hipFree(sl.d_qstart);
   (void)hipFree(sl.d_counter);
   if (sl.done) (void)hipEventDestroy(sl.done);
   if (sl.stream) (void)hipStreamDestroy(sl.stream);
@@ -114,20 +129,13 @@ void free_slot(Slot& sl) {
 }
 
 bool init_slot(Slot& sl, uint64_t max_q, uint32_t max_tiles) {
-  bool ok = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
-            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess;
-  ok = ok && halloc(reinterpret_cast<void**>(&sl.h_q), max_q * 8) &&
-       halloc(reinterpret_cast<void**>(&sl.h_val), max_q * 4) &&
-       halloc(reinterpret_cast<void**>(&sl.h_idx), max_q * 4) &&
-       halloc(reinterpret_cast<void**>(&sl.h_tiles), max_tiles * sizeof(rmq::QueryTile)) &&
-       halloc(reinterpret_cast<void**>(&sl.h_qstart), (rmq::kQueues + 1) * 4);
-  ok = ok && dalloc(reinterpret_cast<void**>(&sl.d_q), max_q * 8) &&
-       dalloc(reinterpret_cast<void**>(&sl.d_val), max_q * 4) &&
-       dalloc(reinterpret_cast<void**>(&sl.d_idx), max_q * 4) &&
-       dalloc(reinterpret_cast<void**>(&sl.d_tiles), max_tiles * sizeof(rmq::QueryTile)) &&
-       dalloc(reinterpret_cast<void**>(&sl.d_qstart), (rmq::kQueues + 1) * 4) &&
-       dalloc(reinterpret_cast<void**>(&sl.d_counter), rmq::kCounterWords * 4);
-  return ok;
+  return hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
+         zalloc(&sl.h_q, &sl.d_q, max_q * 8) && zalloc(&sl.h_val, &sl.d_val, max_q * 4) &&
+         zalloc(&sl.h_idx, &sl.d_idx, max_q * 4) &&
+         zalloc(&sl.h_tiles, &sl.d_tiles, max_tiles * sizeof(rmq::QueryTile)) &&
+         zalloc(&sl.h_qstart, &sl.d_qstart, (rmq::kQueues + 1) * 4) &&
+         dalloc(reinterpret_cast<void**>(&sl.d_counter), rmq::kCounterWords * 4);
 }
 
 // Copy a finished slot's answers to its requests' buffers and free the slot.
@@ -175,19 +183,10 @@ rmq_status launch(rmq_server* s, Slot& sl) {
   std::memcpy(sl.h_tiles, tiles.data(), tiles.size() * sizeof(rmq::QueryTile));
   std::memcpy(sl.h_qstart, qstart.data(), qstart.size() * 4);
   const hipStream_t st = sl.stream;
-  bool ok = hipMemcpyAsync(sl.d_q, sl.h_q, total * 8, hipMemcpyHostToDevice, st) == hipSuccess &&
-            hipMemcpyAsync(sl.d_tiles, sl.h_tiles, tiles.size() * sizeof(rmq::QueryTile),
-                           hipMemcpyHostToDevice, st) == hipSuccess &&
-            hipMemcpyAsync(sl.d_qstart, sl.h_qstart, qstart.size() * 4, hipMemcpyHostToDevice, st) == hipSuccess;
-  if (ok) {
-    rmq::launch_multi_query(s->d_views, s->d_variant, sl.d_tiles, sl.d_qstart, sl.d_counter,
-                            static_cast<uint32_t>(tiles.size()), reinterpret_cast<const uint2*>(sl.d_q), sl.d_val,
-                            sl.want_idx ? sl.d_idx : nullptr, st);
-    ok = hipGetLastError() == hipSuccess;
-  }
-  ok = ok && hipMemcpyAsync(sl.h_val, sl.d_val, total * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
-  if (ok && sl.want_idx)
-    ok = hipMemcpyAsync(sl.h_idx, sl.d_idx, total * 4, hipMemcpyDeviceToHost, st) == hipSuccess;
+  rmq::launch_multi_query(s->d_views, s->d_variant, sl.d_tiles, sl.d_qstart, sl.d_counter,
+                          static_cast<uint32_t>(tiles.size()), reinterpret_cast<const uint2*>(sl.d_q), sl.d_val,
+                          sl.want_idx ? sl.d_idx : nullptr, st);
+  bool ok = hipGetLastError() == hipSuccess;
   ok = ok && hipEventRecord(sl.done, st) == hipSuccess;
   sl.total = total;
   sl.busy = true;

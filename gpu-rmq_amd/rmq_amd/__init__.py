@@ -445,11 +445,22 @@ class Server:
     def evict(self, array_id: int) -> None:
         _lib.check(_lib.lib().rmq_server_evict(self._s, array_id), "rmq_server_evict")
 
-    def submit(self, array_id: int, lr: np.ndarray, want_val=True, want_idx=False):
+    def submit(self, array_id: int, lr: np.ndarray, want_val=True, want_idx=False,
+               out_val: np.ndarray = None, out_idx: np.ndarray = None):
+        """Queue queries on a resident array; returns (ticket, val, idx). The answers land in
+        out_val / out_idx when given (contiguous float32 / uint32 of q entries, e.g. views of a
+        reused pool), else in new arrays."""
         lr = np.ascontiguousarray(lr, dtype=np.uint32)
         q = lr.size // 2
-        val = np.empty(q, dtype=np.float32) if want_val else None
-        idx = np.empty(q, dtype=np.uint32) if want_idx else None
+        val = idx = None
+        if want_val:
+            val = np.empty(q, dtype=np.float32) if out_val is None else out_val
+            if val.dtype != np.float32 or val.shape != (q,) or not val.flags.c_contiguous:
+                raise ValueError("out_val must be a contiguous float32 array of q entries")
+        if want_idx:
+            idx = np.empty(q, dtype=np.uint32) if out_idx is None else out_idx
+            if idx.dtype != np.uint32 or idx.shape != (q,) or not idx.flags.c_contiguous:
+                raise ValueError("out_idx must be a contiguous uint32 array of q entries")
         t = ctypes.c_uint64()
         _lib.check(_lib.lib().rmq_server_submit(
             self._s, array_id, lr.ctypes.data, q, val.ctypes.data if want_val else None,
