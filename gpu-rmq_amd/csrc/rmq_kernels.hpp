@@ -412,6 +412,19 @@ __device__ __forceinline__ GroupAnswer solve_vl(uint32_t l, uint32_t e, const Hi
 
 // Pruned evaluation order for alg 20 (same rules as solve_cl_pruned, one lane per query; the
 // climb stops when r - l <= R or the aligned bounds cross, src/xxx.h:903-913).
+// Level offsets and sizes as the lane solver looks them up (one lookup per level it visits):
+// from the HierView itself, or from a copy the kernel placed in LDS.
+struct LevelsOf {
+  const HierView& H;
+  __device__ __forceinline__ uint32_t off(uint32_t k) const { return H.off[k]; }
+  __device__ __forceinline__ uint32_t size(uint32_t k) const { return H.size[k]; }
+};
+struct LevelsLds {
+  const uint32_t* meta;  // LDS: off[kMaxLevels], then size[kMaxLevels]
+  __device__ __forceinline__ uint32_t off(uint32_t k) const { return meta[k]; }
+  __device__ __forceinline__ uint32_t size(uint32_t k) const { return meta[kMaxLevels + k]; }
+};
+
 // Lane-per-query pruned evaluation. The final range is taken at the highest level K whose
 // aligned blocks include one inside [l, e), found in O(1) from the length (any level whose
 // range is nonempty yields the same answer; this one leaves at most 2R - 2 entries to scan
@@ -420,8 +433,8 @@ __device__ __forceinline__ GroupAnswer solve_vl(uint32_t l, uint32_t e, const Hi
 // to left with <=) and a right element must be strictly smaller (folded left to right with <).
 // In the final range, folded left to right with <, the first entry seeds the answer, so a range
 // of +inf answers with its leftmost entry.
-template <int RL, bool IDX>
-__device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H) {
+template <int RL, bool IDX, class LT>
+__device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
   constexpr uint32_t R = 1u << RL;
   constexpr uint32_t V = R / 4;
   const uint32_t top = H.num_levels - 1;
@@ -435,12 +448,12 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
     lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
     ek = e >> shK;
   }
-  const float* vK = level_values(H, K);
-  const uint32_t szK = H.size[K];
+  const float* vK = K == 0 ? H.A : H.lv + T.off(K);
+  const uint32_t szK = T.size(K);
   const bool left_tail = l < (lk << shK);
   const bool right_tail = (ek << shK) < e;
   float bv = RMQ_INF;
-  uint32_t bpay = (K == 0 ? dev::kDirect : H.off[K]) + lk;
+  uint32_t bpay = (K == 0 ? dev::kDirect : T.off(K)) + lk;
   float gL = RMQ_INF, gR = RMQ_INF;
   {
     // guards lk - 1 and ek: in the first / last vector the scan reads unless 4-aligned
@@ -471,14 +484,15 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
       const uint32_t sk = RL * static_cast<uint32_t>(k);
       const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);
       const uint32_t base = lo & ~(R - 1u);
-      const float* vk = level_values(H, k);
-      const uint32_t pay0 = k == 0 ? dev::kDirect : H.off[k];
+      const uint32_t ok = k == 0 ? 0u : T.off(k);
+      const float* vk = k == 0 ? H.A : H.lv + ok;
+      const uint32_t pay0 = k == 0 ? dev::kDirect : ok;
       float g = RMQ_INF;
 #pragma unroll
       for (int s = static_cast<int>(V) - 1; s >= 0; --s) {
         const uint32_t p = base + 4u * static_cast<uint32_t>(s);
         if ((lo & (R - 1u)) != 0u && p + 4u > lo) {  // the vectors of [lo, roundup_R(lo))
-          const float4 x = dev::load4(vk, p, H.size[k]);
+          const float4 x = dev::load4(vk, p, T.size(k));
           if (p <= lo - 1u) g = dev::pick4(x, (lo - 1u) & 3u);  // next guard, lo - 1
           const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
@@ -499,14 +513,15 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
       const uint32_t sk = RL * static_cast<uint32_t>(k);
       const uint32_t hi = e >> sk;
       const uint32_t lo = hi & ~(R - 1u);
-      const float* vk = level_values(H, k);
-      const uint32_t pay0 = k == 0 ? dev::kDirect : H.off[k];
+      const uint32_t ok = k == 0 ? 0u : T.off(k);
+      const float* vk = k == 0 ? H.A : H.lv + ok;
+      const uint32_t pay0 = k == 0 ? dev::kDirect : ok;
       float g = RMQ_INF;
 #pragma unroll
       for (uint32_t s = 0; s < V; ++s) {
         const uint32_t p = lo + 4u * s;
         if (p < hi) {
-          const float4 x = dev::load4(vk, p, H.size[k]);
+          const float4 x = dev::load4(vk, p, T.size(k));
           if (hi - p < 4u) g = dev::pick4(x, hi & 3u);  // next guard, hi
           const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
@@ -526,9 +541,9 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
   else return GroupAnswer{bv, dev::kNoIdx};
 }
 
-template <int RL, bool P, bool IDX>
-__device__ __forceinline__ GroupAnswer solve_lane(uint32_t l, uint32_t e, const HierView& H) {
-  if constexpr (P) return solve_vl_pruned<RL, IDX>(l, e, H);
+template <int RL, bool P, bool IDX, class LT>
+__device__ __forceinline__ GroupAnswer solve_lane(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
+  if constexpr (P) return solve_vl_pruned<RL, IDX>(l, e, H, T);
   else return solve_vl<RL, IDX>(l, e, H);
 }
 
@@ -536,15 +551,22 @@ template <int RL, bool P, bool IDX>
 __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q, uint32_t q,
                                                      float* __restrict__ out_v,
                                                      uint32_t* __restrict__ out_i, HierView H) {
+  // level offsets and sizes in LDS: the solver looks one up per level it visits
+  __shared__ uint32_t s_meta[2 * kMaxLevels];
+  if (threadIdx.x < 2 * kMaxLevels)
+    s_meta[threadIdx.x] = threadIdx.x < kMaxLevels ? H.off[threadIdx.x] : H.size[threadIdx.x - kMaxLevels];
+  __syncthreads();
+  const LevelsLds T{s_meta};
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (qi >= q) return;
   const uint2 p = dev::load_query(Q, qi);
-  GroupAnswer a = solve_lane<RL, P, IDX>(p.x, p.y + 1, H);
+  GroupAnswer a = solve_lane<RL, P, IDX>(p.x, p.y + 1, H, T);
   if constexpr (!IDX) {
-    if (a.v == 0.0f) a = solve_lane<RL, P, true>(p.x, p.y + 1, H);
+    if (a.v == 0.0f) a = solve_lane<RL, P, true>(p.x, p.y + 1, H, T);
   }
   if (out_v) out_v[qi] = a.v;
   if (IDX && out_i) out_i[qi] = a.idx;
 }
+
 
 }  // namespace rmq

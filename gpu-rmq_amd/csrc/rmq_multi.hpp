@@ -128,9 +128,10 @@ __device__ __forceinline__ void unit_lanes(const HierView& H, const uint2* __res
   const uint32_t local = threadIdx.x & 63;
   if (local >= count) return;
   const uint2 p = dev::load_query(Q, q0 + local);
-  GroupAnswer a = solve_vl_pruned<RL, IDX>(p.x, p.y + 1, H);
+  const LevelsOf T{H};
+  GroupAnswer a = solve_vl_pruned<RL, IDX>(p.x, p.y + 1, H, T);
   if constexpr (!IDX) {
-    if (a.v == 0.0f) a = solve_vl_pruned<RL, true>(p.x, p.y + 1, H);
+    if (a.v == 0.0f) a = solve_vl_pruned<RL, true>(p.x, p.y + 1, H, T);
   }
   if (ov) ov[q0 + local] = a.v;
   if (IDX && oi) oi[q0 + local] = a.idx;
