@@ -274,7 +274,10 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   h->t = t;
   h->buffer_floats = P.buffer_floats;
   if (P.num_levels > 1) {
-    const size_t bytes = static_cast<size_t>(P.buffer_floats) * 4;
+    // padded to the level alignment past the last level too, so that every 16-byte vector that
+    // starts inside a level lies inside the allocation
+    const size_t bytes = static_cast<size_t>((P.buffer_floats + RMQ_LEVEL_ALIGNMENT - 1) &
+                                             ~static_cast<uint64_t>(RMQ_LEVEL_ALIGNMENT - 1)) * 4;
     if (hipMalloc(reinterpret_cast<void**>(&h->lv), bytes) != hipSuccess) {
       (void)hipGetLastError();
       delete h;

@@ -433,6 +433,14 @@ struct LevelsLds {
 // to left with <=) and a right element must be strictly smaller (folded left to right with <).
 // In the final range, folded left to right with <, the first entry seeds the answer, so a range
 // of +inf answers with its leftmost entry.
+// A 16-byte vector of level k at p (p % 4 == 0, p < size). Upper levels sit in the level buffer,
+// padded to 32 entries past each level's end, so their vectors need no bounds test; level 0 is
+// the caller's array of exactly n floats.
+__device__ __forceinline__ float4 load_level(const float* __restrict__ v, uint32_t p, uint32_t size, uint32_t k) {
+  if (k != 0) return *reinterpret_cast<const float4*>(v + p);
+  return dev::load4(v, p, size);
+}
+
 template <int RL, bool IDX, class LT>
 __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
   constexpr uint32_t R = 1u << RL;
@@ -458,7 +466,7 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
   {
     // guards lk - 1 and ek: in the first / last vector the scan reads unless 4-aligned
     uint32_t p = lk & ~3u;
-    float4 x = dev::load4(vK, p, szK);
+    float4 x = load_level(vK, p, szK, K);
     if (left_tail) gL = (lk & 3u) ? dev::pick4(x, (lk - 1u) & 3u) : vK[lk - 1u];
     const uint32_t pay0 = bpay - lk;
     for (;;) {
@@ -475,7 +483,7 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
       }
       p += 4;
       if (p >= ek) break;
-      x = dev::load4(vK, p, szK);
+      x = load_level(vK, p, szK, K);
     }
     if (right_tail) gR = (ek & 3u) ? dev::pick4(x, ek & 3u) : vK[ek];
   }
@@ -492,7 +500,7 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
       for (int s = static_cast<int>(V) - 1; s >= 0; --s) {
         const uint32_t p = base + 4u * static_cast<uint32_t>(s);
         if ((lo & (R - 1u)) != 0u && p + 4u > lo) {  // the vectors of [lo, roundup_R(lo))
-          const float4 x = dev::load4(vk, p, T.size(k));
+          const float4 x = load_level(vk, p, T.size(k), static_cast<uint32_t>(k));
           if (p <= lo - 1u) g = dev::pick4(x, (lo - 1u) & 3u);  // next guard, lo - 1
           const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
@@ -521,7 +529,7 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
       for (uint32_t s = 0; s < V; ++s) {
         const uint32_t p = lo + 4u * s;
         if (p < hi) {
-          const float4 x = dev::load4(vk, p, T.size(k));
+          const float4 x = load_level(vk, p, T.size(k), static_cast<uint32_t>(k));
           if (hi - p < 4u) g = dev::pick4(x, hi & 3u);  // next guard, hi
           const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
