@@ -41,6 +41,9 @@ __global__ __launch_bounds__(kBlock) void k_gpu_base(const float* __restrict__ A
   if (out_i) out_i[qi] = mi;
 }
 
+// One word set from the stream (a kernel, not a memset, so that a captured build stays valid).
+__global__ void k_set_word(uint32_t* __restrict__ w, uint32_t v) { *w = v; }
+
 }  // namespace rmq
 
 struct rmq_hierarchy {
@@ -54,6 +57,7 @@ struct rmq_hierarchy {
   uint64_t buffer_floats;
   float* lv;
   uint32_t* li;
+  uint32_t* zeros;  // device word: kZeroPos | kZeroNeg of level 0
   HierView view;
 };
 
@@ -135,26 +139,26 @@ rmq_status last_error() { return hipGetLastError() == hipSuccess ? RMQ_OK : RMQ_
 // ---- build dispatch -----------------------------------------------------------------------
 template <int RL>
 void launch_build_level(const float* sv, const uint32_t* si, uint32_t ssize, float* dv,
-                        uint32_t* di, uint32_t dsize, hipStream_t s) {
+                        uint32_t* di, uint32_t dsize, uint32_t* zeros, hipStream_t s) {
   constexpr uint32_t V = (1u << RL) / 4;
   constexpr uint32_t W = V < 64 ? V : 64;
   const uint32_t grid = grid_for(static_cast<uint64_t>(dsize) * W);
   if (si) {
-    rmq::k_build_level<RL, true><<<grid, kBlock, 0, s>>>(sv, si, ssize, dv, di, dsize);
+    rmq::k_build_level<RL, true><<<grid, kBlock, 0, s>>>(sv, si, ssize, dv, di, dsize, zeros);
   } else {
-    rmq::k_build_level<RL, false><<<grid, kBlock, 0, s>>>(sv, si, ssize, dv, di, dsize);
+    rmq::k_build_level<RL, false><<<grid, kBlock, 0, s>>>(sv, si, ssize, dv, di, dsize, zeros);
   }
 }
 
 rmq_status build_level(uint32_t R_log, const float* sv, const uint32_t* si, uint32_t ssize,
-                       float* dv, uint32_t* di, uint32_t dsize, hipStream_t s) {
+                       float* dv, uint32_t* di, uint32_t dsize, uint32_t* zeros, hipStream_t s) {
   switch (R_log) {
-    case 2: launch_build_level<2>(sv, si, ssize, dv, di, dsize, s); break;
-    case 3: launch_build_level<3>(sv, si, ssize, dv, di, dsize, s); break;
-    case 4: launch_build_level<4>(sv, si, ssize, dv, di, dsize, s); break;
-    case 5: launch_build_level<5>(sv, si, ssize, dv, di, dsize, s); break;
-    case 6: launch_build_level<6>(sv, si, ssize, dv, di, dsize, s); break;
-    case 7: launch_build_level<7>(sv, si, ssize, dv, di, dsize, s); break;
+    case 2: launch_build_level<2>(sv, si, ssize, dv, di, dsize, zeros, s); break;
+    case 3: launch_build_level<3>(sv, si, ssize, dv, di, dsize, zeros, s); break;
+    case 4: launch_build_level<4>(sv, si, ssize, dv, di, dsize, zeros, s); break;
+    case 5: launch_build_level<5>(sv, si, ssize, dv, di, dsize, zeros, s); break;
+    case 6: launch_build_level<6>(sv, si, ssize, dv, di, dsize, zeros, s); break;
+    case 7: launch_build_level<7>(sv, si, ssize, dv, di, dsize, zeros, s); break;
     default: return RMQ_EUNSUPPORTED;
   }
   return last_error();
@@ -273,6 +277,11 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   h->n = n;
   h->t = t;
   h->buffer_floats = P.buffer_floats;
+  if (hipMalloc(reinterpret_cast<void**>(&h->zeros), sizeof(uint32_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    delete h;
+    return RMQ_ENOMEM;
+  }
   if (P.num_levels > 1) {
     // padded to the level alignment past the last level too, so that every 16-byte vector that
     // starts inside a level lies inside the allocation
@@ -280,12 +289,14 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
                                              ~static_cast<uint64_t>(RMQ_LEVEL_ALIGNMENT - 1)) * 4;
     if (hipMalloc(reinterpret_cast<void**>(&h->lv), bytes) != hipSuccess) {
       (void)hipGetLastError();
+      (void)hipFree(h->zeros);
       delete h;
       return RMQ_ENOMEM;
     }
     if (hipMalloc(reinterpret_cast<void**>(&h->li), bytes) != hipSuccess) {
       (void)hipGetLastError();
       (void)hipFree(h->lv);
+      (void)hipFree(h->zeros);
       delete h;
       return RMQ_ENOMEM;
     }
@@ -294,18 +305,25 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   H.A = d_array;
   H.lv = h->lv;
   H.li = h->li;
+  H.zeros = h->zeros;
   H.num_levels = P.num_levels;
   for (uint32_t k = 0; k < rmq::kMaxLevels; ++k) {
     H.size[k] = k < P.num_levels ? static_cast<uint32_t>(P.size[k]) : 0;
     H.off[k] = k < P.num_levels ? static_cast<uint32_t>(P.off[k]) : 0;
   }
   const hipStream_t s = static_cast<hipStream_t>(stream);
+  // the level-0 pass records the zero signs; with no such pass both are assumed
+  rmq::k_set_word<<<1, 1, 0, s>>>(h->zeros, P.num_levels > 1 ? 0u : (rmq::kZeroPos | rmq::kZeroNeg));
+  if (last_error() != RMQ_OK) {
+    rmq_free(h);
+    return RMQ_EDEVICE;
+  }
   for (uint32_t k = 0; k + 1 < P.num_levels; ++k) {
     const float* sv = k == 0 ? d_array : h->lv + P.off[k];
     const uint32_t* si = k == 0 ? nullptr : h->li + P.off[k];
     const rmq_status bs =
         build_level(P.R_log, sv, si, static_cast<uint32_t>(P.size[k]), h->lv + P.off[k + 1],
-                    h->li + P.off[k + 1], static_cast<uint32_t>(P.size[k + 1]), s);
+                    h->li + P.off[k + 1], static_cast<uint32_t>(P.size[k + 1]), h->zeros, s);
     if (bs != RMQ_OK) {
       rmq_free(h);
       return bs;
@@ -335,6 +353,7 @@ void rmq_free(rmq_hierarchy* h) {
   if (!h) return;
   if (h->lv) (void)hipFree(h->lv);
   if (h->li) (void)hipFree(h->li);
+  if (h->zeros) (void)hipFree(h->zeros);
   delete h;
 }
 

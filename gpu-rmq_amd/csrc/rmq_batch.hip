@@ -42,6 +42,7 @@ struct rmq_batch {
   uint32_t* d_variant;
   float* lv;
   uint32_t* li;
+  uint32_t* zeros;  // per array: kZeroPos | kZeroNeg of its level 0
   uint64_t level_floats;
   std::vector<uint32_t> variant;  // host copy
 };
@@ -92,6 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
   float* dv = const_cast<float*>(H.lv) + H.off[k + 1];
   uint32_t* di = const_cast<uint32_t*>(H.li) + H.off[k + 1];
   const uint32_t gl = threadIdx.x & (W - 1);
+  uint32_t zb = 0;
   for (uint32_t j = T.j0 + threadIdx.x / W; j < T.j1; j += kBlock / W) {
     float bv = RMQ_INF;
     uint32_t bi = dev::kNoIdx;
@@ -99,6 +101,9 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
     if (pos < ssize) {
       const float4 x = dev::load4(sv, pos, ssize);
       dev::fold4_idx(bv, bi, x, si, pos, ssize, pos, ssize);
+      if (k == 0)
+        zb |= dev::zero_bits(x.x, true) | dev::zero_bits(x.y, pos + 1 < ssize) |
+              dev::zero_bits(x.z, pos + 2 < ssize) | dev::zero_bits(x.w, pos + 3 < ssize);
     }
     dev::group_argmin<W>(bv, bi);
     if (gl == 0) {
@@ -106,6 +111,14 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
       di[j] = bi;
     }
   }
+  if (k == 0) dev::publish_zeros(const_cast<uint32_t*>(H.zeros), zb);
+}
+
+// Every array's zero-sign word: cleared for the level-0 pass, or both signs when there is none.
+__global__ void k_init_zeros(const HierView* __restrict__ views, uint32_t count) {
+  const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < count)
+    *const_cast<uint32_t*>(views[a].zeros) = views[a].num_levels > 1 ? 0u : (kZeroPos | kZeroNeg);
 }
 
 }  // namespace rmq
@@ -181,6 +194,7 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
   (void)hipGetDevice(&b->device);
   rmq_status st = dmalloc(&b->lv, arena);
   if (st == RMQ_OK) st = dmalloc(&b->li, arena);
+  if (st == RMQ_OK) st = dmalloc(&b->zeros, count);
   if (st == RMQ_OK) st = dmalloc(&b->d_views, count);
   if (st == RMQ_OK) st = dmalloc(&b->d_variant, count);
   if (st != RMQ_OK) {
@@ -193,6 +207,7 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     H.A = d_arrays + arrays[a].offset;
     H.lv = b->lv + lv_off[a];
     H.li = b->li + lv_off[a];
+    H.zeros = b->zeros + a;
     b->variant[a] = arrays[a].variant;
   }
 
@@ -223,6 +238,10 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
             hipMemcpyAsync(b->d_variant, b->variant.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice, s) == hipSuccess;
   if (ok && !tiles.empty())
     ok = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(BuildTile), hipMemcpyHostToDevice, s) == hipSuccess;
+  if (ok) {
+    rmq::k_init_zeros<<<(count + 255) / 256, 256, 0, s>>>(b->d_views, count);
+    ok = hipGetLastError() == hipSuccess;
+  }
   for (size_t g = 0; ok && g + 1 < start.size(); ++g) {
     const uint32_t t0 = start[g], t1 = start[g + 1];
     if (t1 == t0) continue;
@@ -339,6 +358,7 @@ void rmq_batch_plan_free(rmq_batch_plan* p) {
 void rmq_batch_free(rmq_batch* b) {
   if (!b) return;
   if (b->lv) (void)hipFree(b->lv);
+  if (b->zeros) (void)hipFree(b->zeros);
   if (b->li) (void)hipFree(b->li);
   if (b->d_views) (void)hipFree(b->d_views);
   if (b->d_variant) (void)hipFree(b->d_variant);

@@ -19,10 +19,16 @@ struct HierView {
   const float* A;        // level 0: the caller's array (borrowed)
   const float* lv;       // level buffer, values  (levels 1..num_levels-1)
   const uint32_t* li;    // level buffer, leftmost level-0 index of each value
+  const uint32_t* zeros; // which signs of zero level 0 holds (kZeroPos | kZeroNeg), see below
   uint32_t num_levels;
   uint32_t size[kMaxLevels];
   uint32_t off[kMaxLevels];
 };
+
+// A value-only answer must carry the bit pattern of A[leftmost argmin]; a plain minimum can only
+// miss it when the minimum is zero and the array holds both +0 and -0. The level-0 build pass
+// records which signs occur; a hierarchy with no such pass (one level) records both.
+constexpr uint32_t kZeroPos = 1u, kZeroNeg = 2u;
 
 namespace dev {
 
@@ -156,6 +162,29 @@ __device__ __forceinline__ uint4 load4u(const uint32_t* __restrict__ base, uint3
 // Element j (0..3) of a vector.
 __device__ __forceinline__ float pick4(float4 x, uint32_t j) {
   return j == 0 ? x.x : (j == 1 ? x.y : (j == 2 ? x.z : x.w));
+}
+
+// The sign bits of the zeros among x (kZeroPos / kZeroNeg), if `ok`.
+__device__ __forceinline__ uint32_t zero_bits(float x, bool ok) {
+  const uint32_t b = __float_as_uint(x);
+  return ok && (b & 0x7FFFFFFFu) == 0 ? (1u << (b >> 31)) : 0u;
+}
+
+// A zero minimum v of a value-only query, given the array's zero signs: true (and v set to the
+// right zero) unless both signs occur, when the query must be answered with index tracking.
+__device__ __forceinline__ bool zero_known(uint32_t zeros, float& v) {
+  if (zeros == (kZeroPos | kZeroNeg)) return false;
+  v = (zeros & kZeroNeg) ? -0.0f : 0.0f;
+  return true;
+}
+
+// OR of the wave's `bits` (kZeroPos | kZeroNeg) into *zeros, one atomic per wave that saw any.
+// All lanes of the wave must call it.
+__device__ __forceinline__ void publish_zeros(uint32_t* zeros, uint32_t bits) {
+  const uint64_t pos = __ballot((bits & kZeroPos) != 0u);
+  const uint64_t neg = __ballot((bits & kZeroNeg) != 0u);
+  const uint32_t w = (pos ? kZeroPos : 0u) | (neg ? kZeroNeg : 0u);
+  if (w && (threadIdx.x & 63u) == 0) atomicOr(zeros, w);
 }
 
 __device__ __forceinline__ bool in_range(uint32_t p, uint32_t lo, uint32_t hi) {

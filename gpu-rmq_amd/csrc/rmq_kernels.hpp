@@ -19,12 +19,13 @@ constexpr int kBlock = 256;  // 4 waves per workgroup
 // the value only). W = R/4 lanes share one block, each lane owns 16-byte vectors, so one
 // wave instruction streams 1 KiB of contiguous level-k data; the W lanes meet in a DPP
 // all-reduce. Grid-stride over output entries.
+// The level-0 pass (SRC_IDX false) also records which signs of zero the array holds in *zeros.
 template <int RL, bool SRC_IDX>
 __global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict__ sv,
                                                         const uint32_t* __restrict__ si,
                                                         uint32_t ssize, float* __restrict__ dv,
                                                         uint32_t* __restrict__ di,
-                                                        uint32_t dsize) {
+                                                        uint32_t dsize, uint32_t* __restrict__ zeros) {
   constexpr uint32_t R = 1u << RL;
   constexpr uint32_t V = R / 4;                 // 16-byte vectors per block
   constexpr uint32_t W = V < 64 ? V : 64;       // lanes per block
@@ -32,6 +33,7 @@ __global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict_
   const uint32_t gl = threadIdx.x & (W - 1);
   const uint64_t grp = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / W;
   const uint64_t ngrp = static_cast<uint64_t>(gridDim.x) * kBlock / W;
+  uint32_t zb = 0;
   for (uint64_t j = grp; j < dsize; j += ngrp) {
     float bv = RMQ_INF;
     uint32_t bi = dev::kNoIdx;
@@ -42,6 +44,10 @@ __global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict_
       if (pos < ssize) {
         const float4 x = dev::load4(sv, pos, ssize);
         dev::fold4_idx(bv, bi, x, SRC_IDX ? si : nullptr, pos, ssize, pos, ssize);
+        if constexpr (!SRC_IDX) {
+          zb |= dev::zero_bits(x.x, true) | dev::zero_bits(x.y, pos + 1 < ssize) |
+                dev::zero_bits(x.z, pos + 2 < ssize) | dev::zero_bits(x.w, pos + 3 < ssize);
+        }
       }
     }
     dev::group_argmin<W>(bv, bi);
@@ -50,6 +56,7 @@ __global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict_
       di[j] = bi;
     }
   }
+  if constexpr (!SRC_IDX) dev::publish_zeros(zeros, zb);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -259,13 +266,14 @@ __device__ __forceinline__ GroupAnswer solve_group(uint32_t l, uint32_t e, uint3
 }
 
 // Value-only answers must carry the bit pattern of A[leftmost argmin]. A plain minimum can
-// only differ from that when the minimum is zero (+0 and -0 compare equal); those queries
-// are answered again with index tracking.
+// only differ from that when the minimum is zero (+0 and -0 compare equal): the zero's sign is
+// taken from the array's zero signs, and only when the array holds both is the query answered
+// again with index tracking.
 template <int G, int RL, bool P>
 __device__ __forceinline__ float value_exact(uint32_t l, uint32_t e, uint32_t gl,
-                                             const HierView& H) {
+                                             const HierView& H, uint32_t zeros) {
   GroupAnswer a = solve_group<G, RL, P, false>(l, e, gl, H);
-  if (a.v == 0.0f) a = solve_group<G, RL, P, true>(l, e, gl, H);
+  if (a.v == 0.0f && !dev::zero_known(zeros, a.v)) a = solve_group<G, RL, P, true>(l, e, gl, H);
   return a.v;
 }
 
@@ -281,6 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ 
   const uint32_t gl = lane & (G - 1);
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   const bool valid = qi < q;
+  const uint32_t zeros = IDX ? 0u : *H.zeros;
   uint32_t l = 0, e = 0;
   if (valid) {
     const uint2 p = dev::load_query(Q, qi);
@@ -303,7 +312,7 @@ __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ 
         v = a.v;
         i = a.idx;
       } else {
-        v = value_exact<G, RL, P>(lk, ek, gl, H);
+        v = value_exact<G, RL, P>(lk, ek, gl, H, zeros);
       }
       if (gl == static_cast<uint32_t>(k)) {
         my_v = v;
@@ -328,6 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
   const uint32_t gl = threadIdx.x & (G - 1);
   const uint64_t grp = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / G;
   const uint64_t ngrp = static_cast<uint64_t>(gridDim.x) * kBlock / G;
+  const uint32_t zeros = IDX ? 0u : *H.zeros;
   for (uint64_t qi = grp; qi < q; qi += ngrp) {
     const uint2 p = dev::load_query(Q, qi);
     float v;
@@ -337,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
       v = a.v;
       i = a.idx;
     } else {
-      v = value_exact<G, RL, P>(p.x, p.y + 1, gl, H);
+      v = value_exact<G, RL, P>(p.x, p.y + 1, gl, H, zeros);
     }
     if (gl == 0) {
       if (out_v) out_v[qi] = v;
@@ -565,12 +575,13 @@ __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q
     s_meta[threadIdx.x] = threadIdx.x < kMaxLevels ? H.off[threadIdx.x] : H.size[threadIdx.x - kMaxLevels];
   __syncthreads();
   const LevelsLds T{s_meta};
+  const uint32_t zeros = IDX ? 0u : *H.zeros;
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (qi >= q) return;
   const uint2 p = dev::load_query(Q, qi);
   GroupAnswer a = solve_lane<RL, P, IDX>(p.x, p.y + 1, H, T);
   if constexpr (!IDX) {
-    if (a.v == 0.0f) a = solve_lane<RL, P, true>(p.x, p.y + 1, H, T);
+    if (a.v == 0.0f && !dev::zero_known(zeros, a.v)) a = solve_lane<RL, P, true>(p.x, p.y + 1, H, T);
   }
   if (out_v) out_v[qi] = a.v;
   if (IDX && out_i) out_i[qi] = a.idx;

@@ -107,7 +107,7 @@ __device__ __forceinline__ void unit_groups(const HierView& H, const uint2* __re
         v = a.v;
         i = a.idx;
       } else {
-        v = value_exact<G, kMultiRLog, true>(lk, ek, gl, H);
+        v = value_exact<G, kMultiRLog, true>(lk, ek, gl, H, *H.zeros);
       }
       if (gl == static_cast<uint32_t>(k)) {
         my_v = v;
@@ -131,7 +131,7 @@ __device__ __forceinline__ void unit_lanes(const HierView& H, const uint2* __res
   const LevelsOf T{H};
   GroupAnswer a = solve_vl_pruned<RL, IDX>(p.x, p.y + 1, H, T);
   if constexpr (!IDX) {
-    if (a.v == 0.0f) a = solve_vl_pruned<RL, true>(p.x, p.y + 1, H, T);
+    if (a.v == 0.0f && !dev::zero_known(*H.zeros, a.v)) a = solve_vl_pruned<RL, true>(p.x, p.y + 1, H, T);
   }
   if (ov) ov[q0 + local] = a.v;
   if (IDX && oi) oi[q0 + local] = a.idx;

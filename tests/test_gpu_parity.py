@@ -156,6 +156,28 @@ def test_tie_vectors(device):
             assert v.view(np.uint32).tolist() == case["value_bits"], (case["name"], alg, opts)
 
 
+@pytest.mark.parametrize("signs", ["pos", "neg", "both", "none"])
+def test_zero_signs_value_mode(device, signs):
+    """Value-only queries whose minimum is zero: the zero's sign comes from the array's recorded
+    zero signs, and only an array holding both +0 and -0 needs index tracking. Covers every
+    variant, hierarchies of one level (t >= n: no level-0 pass records the signs) and many."""
+    rng = np.random.default_rng({"pos": 1, "neg": 2, "both": 3, "none": 4}[signs])
+    n = 20000
+    A = rng.random(n, dtype=np.float32) + np.float32(0.5)
+    zpos = rng.choice(n, 40, replace=False)
+    if signs in ("pos", "both"):
+        A[zpos[:20]] = np.float32(0.0)
+    if signs in ("neg", "both"):
+        A[zpos[20:]] = np.float32(-0.0)
+    lr = np.sort(rng.integers(0, n, size=(4000, 2)), axis=1).astype(np.uint32)
+    ref_i = oracle.cpu_base_index(A, lr)
+    for alg, opts in VARIANTS:
+        for t in (_min_t(alg, opts), 1 << 15):
+            _say(f"zero_signs {signs} alg={alg} opts={opts} t={t}")
+            v, _ = run_gpu(A, lr, alg, t, True, False, **opts)
+            assert_exact(A, lr, v, None, ref_i, f"zero signs {signs} alg={alg} {opts} t={t}")
+
+
 def test_gpu_base_matches_cpu_base(device):
     rng = np.random.default_rng(4)
     import torch
