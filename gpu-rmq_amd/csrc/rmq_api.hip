@@ -104,7 +104,7 @@ rmq_status make_plan(uint64_t n, uint64_t t, int alg, const rmq_build_options* o
       const uint32_t v = (1u << R_log) / 4;  // one 16-byte vector per lane per block
       G = v < 4 ? 4 : (v > 16 ? 16 : v);
     }
-    if (G != 4 && G != 8 && G != 16) return RMQ_EUNSUPPORTED;
+    if (G != 4 && G != 8 && G != 16 && G != 32 && G != 64) return RMQ_EUNSUPPORTED;
   }
   P->R_log = R_log;
   P->G = G;
@@ -193,6 +193,10 @@ rmq_status launch_cl_g(uint32_t G, bool full, int alg, const uint2* Q, uint32_t 
                         : launch_cl<8, RL, true>(alg, Q, q, ov, oi, H, s);
     case 16: return full ? launch_cl<16, RL, false>(alg, Q, q, ov, oi, H, s)
                          : launch_cl<16, RL, true>(alg, Q, q, ov, oi, H, s);
+    case 32: return full ? launch_cl<32, RL, false>(alg, Q, q, ov, oi, H, s)
+                         : launch_cl<32, RL, true>(alg, Q, q, ov, oi, H, s);
+    case 64: return full ? launch_cl<64, RL, false>(alg, Q, q, ov, oi, H, s)   // a wave per query
+                         : launch_cl<64, RL, true>(alg, Q, q, ov, oi, H, s);
     default: return RMQ_EUNSUPPORTED;
   }
 }

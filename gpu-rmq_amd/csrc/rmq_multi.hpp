@@ -36,7 +36,8 @@ __host__ __device__ inline uint32_t variant_rlog(uint32_t variant) {
 
 inline bool variant_ok(uint32_t v) {
   return v == RMQ_VARIANT_GROUP4 || v == RMQ_VARIANT_GROUP8 || v == RMQ_VARIANT_GROUP16 ||
-         v == RMQ_VARIANT_LANE || v == RMQ_VARIANT_LANE8;
+         v == RMQ_VARIANT_GROUP32 || v == RMQ_VARIANT_WAVE || v == RMQ_VARIANT_LANE ||
+         v == RMQ_VARIANT_LANE8;
 }
 
 // The plain-ABI build parameters that produce the hierarchy a variant reads.
@@ -154,6 +155,8 @@ __device__ __forceinline__ void run_unit(const HierView* __restrict__ views,
   switch (variant[a]) {
     case RMQ_VARIANT_GROUP4: unit_groups<4, IDX>(H, Q, q0, cnt, ov, oi); break;
     case RMQ_VARIANT_GROUP16: unit_groups<16, IDX>(H, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_GROUP32: unit_groups<32, IDX>(H, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_WAVE: unit_groups<64, IDX>(H, Q, q0, cnt, ov, oi); break;
     case RMQ_VARIANT_LANE: unit_lanes<kMultiRLog, IDX>(H, Q, q0, cnt, ov, oi); break;
     case RMQ_VARIANT_LANE8: unit_lanes<kLane8RLog, IDX>(H, Q, q0, cnt, ov, oi); break;
     default: unit_groups<8, IDX>(H, Q, q0, cnt, ov, oi); break;

@@ -275,8 +275,9 @@ def gpu_base(array, lr, want_val: bool = True, want_idx: bool = False, stream=No
 # ---- batched form (include/rmq_batch.h) ------------------------------------------------------
 
 VARIANTS = {"lane": _lib.RMQ_VARIANT_LANE, "lane8": _lib.RMQ_VARIANT_LANE8,
-            "group4": _lib.RMQ_VARIANT_GROUP4,
-            "group8": _lib.RMQ_VARIANT_GROUP8, "group16": _lib.RMQ_VARIANT_GROUP16}
+            "group4": _lib.RMQ_VARIANT_GROUP4, "group8": _lib.RMQ_VARIANT_GROUP8,
+            "group16": _lib.RMQ_VARIANT_GROUP16, "group32": _lib.RMQ_VARIANT_GROUP32,
+            "wave": _lib.RMQ_VARIANT_WAVE}
 
 
 def fill_uniform(out, seed: int, start: int = 0, stream=None) -> None:

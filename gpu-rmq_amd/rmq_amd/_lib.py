@@ -31,6 +31,8 @@ RMQ_VARIANT_LANE8 = 2
 RMQ_VARIANT_GROUP4 = 4
 RMQ_VARIANT_GROUP8 = 8
 RMQ_VARIANT_GROUP16 = 16
+RMQ_VARIANT_GROUP32 = 32
+RMQ_VARIANT_WAVE = 64
 
 
 RMQ_ADMIT_DEVICE = 0

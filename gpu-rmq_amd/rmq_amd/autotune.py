@@ -23,7 +23,7 @@ import numpy as np
 from . import workloads
 
 TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_gfx950.json")
-VARIANTS = ("lane8", "lane", "group4", "group8")
+VARIANTS = ("lane8", "lane", "group4", "group8", "wave")
 THRESHOLDS = (64, 256, 1024, 4096, 32768)
 TOTAL_QUERIES = 1 << 23
 SIZE_LOGS = tuple(range(10, 29, 2))

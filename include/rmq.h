@@ -63,11 +63,12 @@ typedef struct rmq_hierarchy rmq_hierarchy;
  *   algs 16/24: R = 2^(cg_size_log + cg_amount_log)      (src/xxx.h:208-209)
  *   alg 20:     R = 2^(cg_amount_log + 2)                 (src/xxx.h:934-935)
  * group_lanes is this library's own execution choice (lanes that cooperate on one
- * query in algs 16/24; 0 = auto). It never changes results. */
+ * query in algs 16/24; 0 = auto; 64 = a whole wavefront per query, the long-scan form for
+ * large t). It never changes results. */
 typedef struct {
   uint32_t cg_size_log;     /* 0 -> 3 */
   uint32_t cg_amount_log;   /* 0 -> 2 */
-  uint32_t group_lanes;     /* 0 -> auto; else 4, 8 or 16 */
+  uint32_t group_lanes;     /* 0 -> auto; else 4, 8, 16, 32 or 64 */
   uint32_t flags;           /* RMQ_BUILD_* below */
 } rmq_build_options;
 

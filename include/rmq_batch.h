@@ -31,6 +31,8 @@ enum {
   RMQ_VARIANT_GROUP4 = 4,    /* 4 lanes cooperate on one query */
   RMQ_VARIANT_GROUP8 = 8,    /* 8 lanes: one 16-byte vector per lane per 32-float block */
   RMQ_VARIANT_GROUP16 = 16,  /* 16 lanes */
+  RMQ_VARIANT_GROUP32 = 32,  /* 32 lanes */
+  RMQ_VARIANT_WAVE = 64,     /* a whole wavefront per query (long final ranges, large t) */
   RMQ_VARIANT_LANE = 1,      /* one lane per query, 16-byte loads, R = 32 hierarchy */
   RMQ_VARIANT_LANE8 = 2      /* one lane per query, 16-byte loads, R = 8 hierarchy */
 };

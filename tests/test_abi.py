@@ -109,6 +109,11 @@ def test_invalid_arguments():
     with pytest.raises(rmq_amd.RMQError) as e:
         rmq_amd.plan(100, 64, 16, group_lanes=3)
     assert e.value.status == rmq_amd.RMQ_EUNSUPPORTED
+    for g in (4, 8, 16, 32, 64):  # up to a whole wavefront per query
+        rmq_amd.plan(100, 64, 16, group_lanes=g)
+    with pytest.raises(rmq_amd.RMQError) as e:
+        rmq_amd.plan(100, 64, 16, group_lanes=128)
+    assert e.value.status == rmq_amd.RMQ_EUNSUPPORTED
 
 
 def test_null_and_misaligned_pointers_rejected_before_device_use():

@@ -80,9 +80,9 @@ def test_batch_equals_plain_and_oracle(device, want):
         A_host[o:o + n] = rng.integers(0, 3, size=n).astype(np.float32) * \
             np.where(rng.random(n) < 0.5, np.float32(-1), np.float32(1))
     dA.copy_(torch.from_numpy(A_host))
-    variants = ["group8", "group4", "group16", "lane", "lane8"]
+    variants = ["group8", "group4", "group16", "lane", "lane8", "group32", "wave"]
     t = [64 << (a % 5) for a in range(96)]
-    var = [variants[a % 5] for a in range(96)]
+    var = [variants[a % 7] for a in range(96)]
     wv, wi = want in ("value",), want in ("index",)
     with rmq_amd.Batch(dA, w.offsets, w.sizes, t, var) as b:
         plan = b.plan(w.qoffsets)

@@ -29,6 +29,9 @@ VARIANTS = [
     (16, dict(group_lanes=4)),
     (16, dict(group_lanes=16)),
     (24, dict(group_lanes=16)),
+    (16, dict(group_lanes=64)),                              # a wave per query
+    (24, dict(group_lanes=32)),
+    (24, dict(group_lanes=64, cg_size_log=4, cg_amount_log=3)),  # a wave per query, R=128
     (16, dict(cg_size_log=3, cg_amount_log=0)),              # R=8
     (16, dict(cg_size_log=3, cg_amount_log=3)),              # R=64
     (24, dict(cg_size_log=4, cg_amount_log=3)),              # R=128
@@ -128,7 +131,7 @@ def test_every_depth(device, t):
     lr = rmq_amd.gen_queries(1 << 16, -6, n, 77)
     ref_v, ref_i = oracle.CpuHierarchy(A, 64, 5, 16).query(lr)
     for alg, opts in ((16, {}), (24, {}), (16, dict(full=True)), (16, dict(group_lanes=4)),
-                      (24, dict(group_lanes=16)), (20, {})):
+                      (24, dict(group_lanes=16)), (16, dict(group_lanes=64)), (20, {})):
         _say(f"depth t={t} alg={alg} opts={opts}")
         v, i = run_gpu(A, lr, alg, t, True, True, **opts)
         assert_exact(A, lr, v, i, ref_i, f"depth t={t} alg={alg} {opts}")

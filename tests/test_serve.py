@@ -40,9 +40,9 @@ def _host_arrays(rng, count, lo, hi):
 def test_serve_matches_oracle_and_is_order_independent(device):
     rng = np.random.default_rng(17)
     arrays = _host_arrays(rng, 12, 1000, 300000)
-    variants = ["lane", "group8", "group4", "group16", "lane8"]
+    variants = ["lane", "group8", "group4", "group16", "lane8", "wave"]
     with rmq_amd.Server(slots=3, max_batch_queries=1 << 16) as srv:
-        ids = [srv.admit_host(A, t=64 << (i % 4), variant=variants[i % 5])
+        ids = [srv.admit_host(A, t=64 << (i % 4), variant=variants[i % 6])
                for i, A in enumerate(arrays)]
         reqs = []
         for k in range(150):
