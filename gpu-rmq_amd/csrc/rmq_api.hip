@@ -11,6 +11,7 @@
 #include <new>
 
 #include "../../include/rmq.h"
+#include "rmq_long.hpp"
 #include "rmq_internal.hpp"
 #include "rmq_kernels.hpp"
 
@@ -59,11 +60,24 @@ struct rmq_hierarchy {
   uint32_t* li;
   uint32_t* zeros;  // device word: kZeroPos | kZeroNeg of level 0
   HierView view;
+  // 64-bit form: chunks of 2^chunk_log elements, each its own hierarchy (chunk_log = 0: one
+  // hierarchy over the whole array). views / cmin / cidx are device tables: the chunks' views
+  // (or this hierarchy's own view, made by the first rmq_query64) and every chunk's minimum.
+  uint32_t chunk_log = 0;
+  uint32_t chunks = 0;
+  uint64_t msize[RMQ_MAX_LEVELS] = {};  // the reference's metadata for the whole array
+  uint64_t moff[RMQ_MAX_LEVELS] = {};
+  rmq_hierarchy** part = nullptr;
+  HierView* views = nullptr;
+  float* cmin = nullptr;
+  uint64_t* cidx = nullptr;
 };
 
 namespace {
 
 constexpr uint64_t kMaxN = 0x7FFFFFFFull;  // int32 n, as the reference's CmdArgs (src/tools.h:30)
+constexpr uint64_t kMaxN64 = 1ull << 38;    // the 64-bit form (nType = uint64_t, src/tools.h:27-29)
+constexpr uint32_t kChunkLog = 30;          // chunk size of the 64-bit form
 constexpr uint64_t kMaxGrid = 1ull << 20;
 
 struct Plan {
@@ -82,7 +96,7 @@ bool alg_supported(int alg) {
 // Metadata recurrence of src/xxx.h:225-243 and the t checks of src/xxx.h:211-217 / 938-944.
 rmq_status make_plan(uint64_t n, uint64_t t, int alg, const rmq_build_options* opt, Plan* P) {
   if (!alg_supported(alg)) return RMQ_EINVAL;
-  if (n == 0 || n > kMaxN || t == 0) return RMQ_EINVAL;
+  if (n == 0 || n > kMaxN64 || t == 0) return RMQ_EINVAL;
   uint32_t g = 3, c = 2;  // XXX_CG_SIZE_LOG, XXX_CG_AMOUNT_LOG defaults (CMakeLists.txt:61-67)
   uint32_t G = 0;
   if (opt) {
@@ -239,9 +253,103 @@ rmq_status query_dispatch(const rmq_hierarchy* h, const uint2* Q, uint32_t q, fl
   }
 }
 
+
+// ---- 64-bit form ----------------------------------------------------------------------------
+template <int RL>
+void launch_chunk_min(const HierView* views, uint32_t chunks, uint32_t clog, float* cv, uint64_t* ci,
+                      hipStream_t s) {
+  rmq::k_chunk_min<RL><<<(chunks + kBlock - 1) / kBlock, kBlock, 0, s>>>(views, chunks, clog, cv, ci);
+}
+
+template <int RL>
+void launch_query64(const ulonglong2* Q, uint64_t q, float* ov, uint64_t* oi, const HierView* views,
+                    uint32_t clog, const float* cv, const uint64_t* ci, hipStream_t s) {
+  const uint64_t grid = (q + kBlock - 1) / kBlock;
+  rmq::k_query64<RL><<<static_cast<uint32_t>(grid), kBlock, 0, s>>>(Q, q, ov, oi, views, clog, cv, ci);
+}
+
+}  // namespace
+
+void rmq_free(rmq_hierarchy* h);
+rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
+                        const rmq_build_options* opt, void* stream, rmq_hierarchy** out);
+
+namespace {
+
+rmq_status build_chunked(const float* d_array, uint64_t n, uint64_t t, int alg, const rmq_build_options* opt,
+                         uint32_t clog, const Plan& P, void* stream, rmq_hierarchy** out) {
+  if (clog < 8 || clog > kChunkLog) return RMQ_EINVAL;
+  rmq_build_options o = opt ? *opt : rmq_build_options{0, 0, 0, 0};
+  o.flags &= ~(0xFFu << 8);
+  rmq_hierarchy* h = new (std::nothrow) rmq_hierarchy();
+  if (!h) return RMQ_ENOMEM;
+  h->alg = alg;
+  (void)hipGetDevice(&h->device);
+  h->R_log = P.R_log;
+  h->group_lanes = P.G;
+  h->flags = o.flags;
+  h->n = n;
+  h->t = t;
+  h->buffer_floats = P.buffer_floats;
+  h->lv = nullptr;
+  h->li = nullptr;
+  h->zeros = nullptr;
+  h->view = HierView{};
+  h->view.num_levels = P.num_levels;
+  for (uint32_t k = 0; k < P.num_levels; ++k) {
+    h->msize[k] = P.size[k];
+    h->moff[k] = P.off[k];
+  }
+  h->chunk_log = clog;
+  h->chunks = static_cast<uint32_t>((n + (1ull << clog) - 1) >> clog);
+  h->part = new (std::nothrow) rmq_hierarchy*[h->chunks]();
+  if (!h->part) {
+    delete h;
+    return RMQ_ENOMEM;
+  }
+  std::vector<HierView> views(h->chunks);
+  rmq_status st = RMQ_OK;
+  for (uint32_t c = 0; c < h->chunks && st == RMQ_OK; ++c) {
+    const uint64_t b = static_cast<uint64_t>(c) << clog;
+    st = rmq_build_ex(d_array + b, std::min<uint64_t>(1ull << clog, n - b), t, alg, &o, stream, &h->part[c]);
+    if (st == RMQ_OK) views[c] = h->part[c]->view;
+  }
+  if (st == RMQ_OK &&
+      (hipMalloc(reinterpret_cast<void**>(&h->views), h->chunks * sizeof(HierView)) != hipSuccess ||
+       hipMalloc(reinterpret_cast<void**>(&h->cmin), h->chunks * sizeof(float)) != hipSuccess ||
+       hipMalloc(reinterpret_cast<void**>(&h->cidx), h->chunks * sizeof(uint64_t)) != hipSuccess)) {
+    (void)hipGetLastError();
+    st = RMQ_ENOMEM;
+  }
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (st == RMQ_OK &&
+      (hipMemcpyAsync(h->views, views.data(), h->chunks * sizeof(HierView), hipMemcpyHostToDevice, s) != hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess))
+    st = RMQ_EDEVICE;
+  if (st == RMQ_OK) {
+    switch (P.R_log) {
+      case 2: launch_chunk_min<2>(h->views, h->chunks, clog, h->cmin, h->cidx, s); break;
+      case 3: launch_chunk_min<3>(h->views, h->chunks, clog, h->cmin, h->cidx, s); break;
+      case 4: launch_chunk_min<4>(h->views, h->chunks, clog, h->cmin, h->cidx, s); break;
+      case 5: launch_chunk_min<5>(h->views, h->chunks, clog, h->cmin, h->cidx, s); break;
+      case 6: launch_chunk_min<6>(h->views, h->chunks, clog, h->cmin, h->cidx, s); break;
+      case 7: launch_chunk_min<7>(h->views, h->chunks, clog, h->cmin, h->cidx, s); break;
+      default: st = RMQ_EUNSUPPORTED;
+    }
+    if (st == RMQ_OK) st = last_error();
+  }
+  if (st != RMQ_OK) {
+    rmq_free(h);
+    return st;
+  }
+  *out = h;
+  return RMQ_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
 
 rmq_status rmq_plan(uint64_t n, uint64_t t, int alg, const rmq_build_options* opt,
                     uint32_t* num_levels, uint64_t* sizes, uint64_t* offsets,
@@ -267,6 +375,8 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   Plan P;
   const rmq_status st = make_plan(n, t, alg, opt, &P);
   if (st != RMQ_OK) return st;
+  const uint32_t clog = opt ? (opt->flags >> 8) & 0xFFu : 0u;
+  if (n > kMaxN || clog != 0) return build_chunked(d_array, n, t, alg, opt, clog ? clog : kChunkLog, P, stream, out);
 
   rmq_hierarchy* h = new (std::nothrow) rmq_hierarchy();
   if (!h) return RMQ_ENOMEM;
@@ -344,7 +454,7 @@ rmq_status rmq_build(const float* d_array, uint64_t n, uint64_t t, int alg, void
 
 rmq_status rmq_query(const rmq_hierarchy* h, const uint32_t* d_lr, uint64_t q, float* d_val,
                      uint32_t* d_idx, void* stream) {
-  if (!h) return RMQ_EINVAL;
+  if (!h || h->chunk_log) return RMQ_EINVAL;  // a chunked (64-bit) hierarchy: rmq_query64
   if (q == 0) return RMQ_OK;
   if (!d_lr || (!d_val && !d_idx)) return RMQ_EINVAL;
   if ((reinterpret_cast<uintptr_t>(d_lr) & 7u) != 0) return RMQ_EINVAL;
@@ -355,6 +465,11 @@ rmq_status rmq_query(const rmq_hierarchy* h, const uint32_t* d_lr, uint64_t q, f
 
 void rmq_free(rmq_hierarchy* h) {
   if (!h) return;
+  for (uint32_t c = 0; h->part && c < h->chunks; ++c) rmq_free(h->part[c]);
+  delete[] h->part;
+  if (h->views) (void)hipFree(h->views);
+  if (h->cmin) (void)hipFree(h->cmin);
+  if (h->cidx) (void)hipFree(h->cidx);
   if (h->lv) (void)hipFree(h->lv);
   if (h->li) (void)hipFree(h->li);
   if (h->zeros) (void)hipFree(h->zeros);
@@ -367,8 +482,8 @@ rmq_status rmq_get_metadata(const rmq_hierarchy* h, uint32_t* num_levels, uint64
   const HierView& H = h->view;
   if (num_levels) *num_levels = H.num_levels;
   for (uint32_t k = 0; k < H.num_levels; ++k) {
-    if (sizes) sizes[k] = H.size[k];
-    if (offsets) offsets[k] = k == 0 ? 0xFFFFFFFFull : H.off[k];
+    if (sizes) sizes[k] = h->chunk_log ? h->msize[k] : H.size[k];
+    if (offsets) offsets[k] = k == 0 ? 0xFFFFFFFFull : (h->chunk_log ? h->moff[k] : H.off[k]);
   }
   if (buffer_floats) *buffer_floats = h->buffer_floats;
   if (R) *R = 1u << h->R_log;
@@ -413,6 +528,54 @@ rmq_status rmq_gpu_base(const float* d_array, uint64_t n, const uint32_t* d_lr, 
   if (static_cast<uint64_t>(grid) * kBlock < q) return RMQ_EINVAL;
   rmq::k_gpu_base<<<grid, kBlock, 0, static_cast<hipStream_t>(stream)>>>(
       d_array, reinterpret_cast<const uint2*>(d_lr), static_cast<uint32_t>(q), d_val, d_idx);
+  return last_error();
+}
+
+rmq_status rmq_query64(const rmq_hierarchy* hc, const uint64_t* d_lr, uint64_t q, float* d_val,
+                       uint64_t* d_idx, void* stream) {
+  if (!hc) return RMQ_EINVAL;
+  if (q == 0) return RMQ_OK;
+  if (!d_lr || (!d_val && !d_idx) || (reinterpret_cast<uintptr_t>(d_lr) & 15u) != 0) return RMQ_EINVAL;
+  if ((q + kBlock - 1) / kBlock > 0xFFFFFFFFull) return RMQ_EINVAL;
+  rmq_hierarchy* h = const_cast<rmq_hierarchy*>(hc);
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t clog = h->chunk_log;
+  const float* cv = h->cmin;
+  const uint64_t* ci = h->cidx;
+  if (!clog) {  // one hierarchy over the whole array: a one-entry view table, made once
+    if (!h->views) {
+      if (hipMalloc(reinterpret_cast<void**>(&h->views), sizeof(HierView)) != hipSuccess) {
+        (void)hipGetLastError();
+        h->views = nullptr;
+        return RMQ_ENOMEM;
+      }
+      if (hipMemcpyAsync(h->views, &h->view, sizeof(HierView), hipMemcpyHostToDevice, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return RMQ_EDEVICE;
+    }
+    clog = 63;  // every position is in chunk 0
+  }
+  const ulonglong2* Q = reinterpret_cast<const ulonglong2*>(d_lr);
+  switch (h->R_log) {
+    case 2: launch_query64<2>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
+    case 3: launch_query64<3>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
+    case 4: launch_query64<4>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
+    case 5: launch_query64<5>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
+    case 6: launch_query64<6>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
+    case 7: launch_query64<7>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
+    default: return RMQ_EUNSUPPORTED;
+  }
+  return last_error();
+}
+
+rmq_status rmq_gpu_base64(const float* d_array, uint64_t n, const uint64_t* d_lr, uint64_t q, float* d_val,
+                          uint64_t* d_idx, void* stream) {
+  if (!d_array || n == 0 || n > kMaxN64) return RMQ_EINVAL;
+  if (q == 0) return RMQ_OK;
+  if (!d_lr || (!d_val && !d_idx) || (reinterpret_cast<uintptr_t>(d_lr) & 15u) != 0) return RMQ_EINVAL;
+  if ((q + kBlock - 1) / kBlock > 0xFFFFFFFFull) return RMQ_EINVAL;
+  rmq::k_gpu_base64<<<static_cast<uint32_t>((q + kBlock - 1) / kBlock), kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      d_array, reinterpret_cast<const ulonglong2*>(d_lr), q, d_val, d_idx);
   return last_error();
 }
 

@@ -53,14 +53,15 @@ class Plan:
     R: int
 
 
-def _options(cg_size_log, cg_amount_log, group_lanes, full=False) -> Optional[BuildOptions]:
-    if cg_size_log is None and cg_amount_log is None and not group_lanes and not full:
+def _options(cg_size_log, cg_amount_log, group_lanes, full=False, chunk_log=0) -> Optional[BuildOptions]:
+    if cg_size_log is None and cg_amount_log is None and not group_lanes and not full and not chunk_log:
         return None
     o = BuildOptions()
     o.cg_size_log = 3 if cg_size_log is None else int(cg_size_log)
     o.cg_amount_log = 2 if cg_amount_log is None else int(cg_amount_log)
     o.group_lanes = int(group_lanes or 0)
-    o.flags = _lib.RMQ_BUILD_DEFAULT_CG | (_lib.RMQ_BUILD_FULL if full else 0)
+    o.flags = (_lib.RMQ_BUILD_DEFAULT_CG | (_lib.RMQ_BUILD_FULL if full else 0) |
+               _lib.RMQ_BUILD_CHUNK_LOG(chunk_log))
     return o
 
 
@@ -97,6 +98,16 @@ def gen_queries(q: int, lr: int, n: int, seed: int, trivial_check: bool = False)
     """uint32[q, 2] inclusive (l, r) pairs exactly as the reference driver makes them."""
     out = np.empty((q, 2), dtype=np.uint32)
     st = _lib.lib().rmq_gen_queries(q, lr, n, seed, int(bool(trivial_check)), out.ctypes.data)
+    if st != 0:
+        raise ValueError(f"invalid query generator arguments: lr={lr}, n={n}")
+    return out
+
+
+def gen_queries64(q: int, lr: int, n: int, seed: int, trivial_check: bool = False) -> np.ndarray:
+    """uint64[q, 2] pairs as the reference's IS_LONG build makes them (equal to gen_queries
+    while n <= 2^32)."""
+    out = np.empty((q, 2), dtype=np.uint64)
+    st = _lib.lib().rmq_gen_queries64(q, lr, n, seed, int(bool(trivial_check)), out.ctypes.data)
     if st != 0:
         raise ValueError(f"invalid query generator arguments: lr={lr}, n={n}")
     return out
@@ -140,7 +151,8 @@ class Hierarchy:
     """
 
     def __init__(self, array, t: int, alg: int = RMQ_ALG_XXX, cg_size_log=None,
-                 cg_amount_log=None, group_lanes: int = 0, full: bool = False, stream=None):
+                 cg_amount_log=None, group_lanes: int = 0, full: bool = False, chunk_log: int = 0,
+                 stream=None):
         torch = _torch()
         _check_device_tensor(array, "array", (torch.float32,))
         if array.dim() != 1:
@@ -150,7 +162,7 @@ class Hierarchy:
         self.t = int(t)
         self.alg = int(alg)
         self._h = ctypes.c_void_p()
-        o = _options(cg_size_log, cg_amount_log, group_lanes, full)
+        o = _options(cg_size_log, cg_amount_log, group_lanes, full, chunk_log)
         st = _lib.lib().rmq_build_ex(array.data_ptr(), self.n, self.t, self.alg, _opt_ptr(o),
                                      _stream_handle(stream), ctypes.byref(self._h))
         _lib.check(st, "rmq_build")
@@ -178,6 +190,21 @@ class Hierarchy:
         st = _lib.lib().rmq_query(self._h, lr.data_ptr(), q, vp, ip, _stream_handle(stream))
         _lib.check(st, "rmq_query")
         return (out_val if want_val else None, out_idx if want_idx else None)
+
+    def query64(self, lr, want_val: bool = True, want_idx: bool = False, stream=None):
+        """64-bit queries (int64/uint64 tensor [q, 2]) through rmq_query64: (values float32[q],
+        leftmost indices int64[q]) — the only query of a hierarchy built in chunks."""
+        torch = _torch()
+        if self._h.value is None:
+            raise RuntimeError("hierarchy already freed")
+        _check_device_tensor(lr, "lr", (torch.int64, torch.uint64))
+        q = lr.numel() // 2
+        val = torch.empty(q, dtype=torch.float32, device=lr.device) if want_val else None
+        idx = torch.empty(q, dtype=torch.int64, device=lr.device) if want_idx else None
+        st = _lib.lib().rmq_query64(self._h, lr.data_ptr(), q, val.data_ptr() if want_val else None,
+                                    idx.data_ptr() if want_idx else None, _stream_handle(stream))
+        _lib.check(st, "rmq_query64")
+        return val, idx
 
     def metadata(self) -> Plan:
         nl = ctypes.c_uint32()
@@ -269,6 +296,21 @@ def gpu_base(array, lr, want_val: bool = True, want_idx: bool = False, stream=No
                                  val.data_ptr() if want_val else None,
                                  idx.data_ptr() if want_idx else None, _stream_handle(stream))
     _lib.check(st, "rmq_gpu_base")
+    return val, idx
+
+
+def gpu_base64(array, lr, want_val: bool = True, want_idx: bool = False, stream=None):
+    """GPU BASE with 64-bit queries and indices (rmq_gpu_base64)."""
+    torch = _torch()
+    _check_device_tensor(array, "array", (torch.float32,))
+    _check_device_tensor(lr, "lr", (torch.int64, torch.uint64))
+    q = lr.numel() // 2
+    val = torch.empty(q, dtype=torch.float32, device=lr.device) if want_val else None
+    idx = torch.empty(q, dtype=torch.int64, device=lr.device) if want_idx else None
+    st = _lib.lib().rmq_gpu_base64(array.data_ptr(), array.numel(), lr.data_ptr(), q,
+                                   val.data_ptr() if want_val else None,
+                                   idx.data_ptr() if want_idx else None, _stream_handle(stream))
+    _lib.check(st, "rmq_gpu_base64")
     return val, idx
 
 

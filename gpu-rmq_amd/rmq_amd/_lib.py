@@ -26,6 +26,11 @@ RMQ_ALG_XXX_MULTI_LOAD = 24
 RMQ_MAX_LEVELS = 20
 RMQ_BUILD_DEFAULT_CG = 0x1
 RMQ_BUILD_FULL = 0x2
+
+
+def RMQ_BUILD_CHUNK_LOG(k: int) -> int:
+    """Chunk size (log2) of the 64-bit form for one build."""
+    return (k & 0xFF) << 8
 RMQ_VARIANT_LANE = 1
 RMQ_VARIANT_LANE8 = 2
 RMQ_VARIANT_GROUP4 = 4
@@ -138,6 +143,9 @@ SIGNATURES = {
     "rmq_server_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
     "rmq_gen_array": (None, [_u64, _i32, _vp]),
     "rmq_gen_queries": (ctypes.c_int, [_u64, _i32, _u64, _i32, ctypes.c_int, _vp]),
+    "rmq_gen_queries64": (ctypes.c_int, [_u64, _i32, _u64, _i32, ctypes.c_int, _vp]),
+    "rmq_query64": (ctypes.c_int, [_vp, _vp, _u64, _vp, _vp, _vp]),
+    "rmq_gpu_base64": (ctypes.c_int, [_vp, _u64, _vp, _u64, _vp, _vp, _vp]),
     "rmq_gen_sample_indices": (_u64, [_u64, _u64, _i32, _vp]),
 }
 

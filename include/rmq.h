@@ -75,11 +75,18 @@ typedef struct {
 #define RMQ_BUILD_DEFAULT_CG 0x1u   /* use cg_size_log = cg_amount_log = 0 literally */
 #define RMQ_BUILD_FULL 0x2u         /* read every piece of the traversal in the reference's
                                        order instead of the pruned order (same answers) */
+/* Chunk size (log2, 8..30) of the 64-bit form for this build; arrays of more than 2^31 - 1
+ * elements use 2^30 without it. Setting it holds even a small array in chunks, which is how the
+ * 64-bit form is tested on small inputs. */
+#define RMQ_BUILD_CHUNK_LOG(k) ((uint32_t)(k) << 8)
 
 /* Build the hierarchy of block minima over d_array[0..n) on `stream`.
  * Reference counterpart: metadata loop + per-level xxx_build_step launches,
  * src/xxx.h:225-271 (alg 16), 552-599 (alg 24), 959-1010 (alg 20).
- * Requires 1 <= n <= 2^31-1 (the reference's int32 n), d_array 16-byte aligned, t >= 1. */
+ * Requires 1 <= n <= 2^38, d_array 16-byte aligned, t >= 1. Up to 2^31 - 1 (the reference's
+ * int32 n) the hierarchy is one piece; above, it is the 64-bit form (the reference's IS_LONG
+ * build, src/main.cu:21-27): chunks of 2^30 elements, each its own hierarchy, plus every
+ * chunk's minimum, queried with rmq_query64. */
 rmq_status rmq_build(const float* d_array, uint64_t n, uint64_t t, int alg,
                      void* stream, rmq_hierarchy** out);
 
@@ -93,6 +100,15 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
  * d_lr must be 8-byte aligned. q = 0 is a no-op. */
 rmq_status rmq_query(const rmq_hierarchy* h, const uint32_t* d_lr, uint64_t q,
                      float* d_val, uint32_t* d_idx, void* stream);
+
+/* 64-bit queries: q pairs {l, r} of uint64_t (the reference's IS_LONG ulong2, src/main.cu:21-27,
+ * src/tools.h:27-31), 16-byte aligned; indices out as uint64_t. Works on every hierarchy; it is
+ * the only query of a 64-bit-form (chunked) one, on which rmq_query returns RMQ_EINVAL. A query
+ * across chunks is answered from its first and last chunk's hierarchies and the minima of the
+ * chunks in between, with the same leftmost rule. The first call on a one-piece hierarchy
+ * allocates a small device table for it. */
+rmq_status rmq_query64(const rmq_hierarchy* h, const uint64_t* d_lr, uint64_t q,
+                       float* d_val, uint64_t* d_idx, void* stream);
 
 /* Release the level buffers and the handle. Synchronises the device. NULL is a no-op. */
 void rmq_free(rmq_hierarchy* h);
@@ -123,6 +139,10 @@ rmq_status rmq_query_each(rmq_hierarchy* const* hs, const uint32_t* const* d_lr,
  * src/cuda_methods.cuh:4-68. Used by the rtxrmq driver's --check, as in src/main.cu:194-214. */
 rmq_status rmq_gpu_base(const float* d_array, uint64_t n, const uint32_t* d_lr, uint64_t q,
                         float* d_val, uint32_t* d_idx, void* stream);
+
+/* GPU BASE with 64-bit queries and indices (IS_LONG), n <= 2^38. */
+rmq_status rmq_gpu_base64(const float* d_array, uint64_t n, const uint64_t* d_lr, uint64_t q,
+                          float* d_val, uint64_t* d_idx, void* stream);
 
 /* Human-readable name of a status code. */
 const char* rmq_status_str(rmq_status s);

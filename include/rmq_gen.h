@@ -36,6 +36,12 @@ void rmq_gen_array(uint64_t n, int32_t seed, float* out);
 int rmq_gen_queries(uint64_t q, int32_t lr, uint64_t n, int32_t seed, int trivial_check,
                     uint32_t* out);
 
+/* The same with 64-bit positions, as the reference's IS_LONG build draws them (nType =
+ * uint64_t, src/main.cu:21-27): out[0..2q) as uint64_t. Equal to rmq_gen_queries while
+ * n <= 2^32; n up to 2^40. */
+int rmq_gen_queries64(uint64_t q, int32_t lr, uint64_t n, int32_t seed, int trivial_check,
+                      uint64_t* out);
+
 /* Draw sample_size indices in [0, original_size) (or the identity if sample_size >=
  * original_size); returns the number written. */
 uint64_t rmq_gen_sample_indices(uint64_t original_size, uint64_t sample_size, int32_t seed,

@@ -63,6 +63,15 @@ def uniform_int(g: MT, a: int, b: int) -> int:
     urange = b - a
     if urange == MASK32:
         return g() + a
+    if urange > MASK32:
+        # ranges wider than the generator's 32 bits (64-bit positions above 2^32): a draw of
+        # the high part over [0, urange / 2^32], times 2^32, plus one raw output; redrawn while
+        # the sum leaves the range or wraps 64 bits
+        while True:
+            tmp = ((1 << 32) * uniform_int(g, 0, urange >> 32)) & ((1 << 64) - 1)
+            ret = (tmp + g()) & ((1 << 64) - 1)
+            if not (ret > urange or ret < tmp):
+                return ret + a
     erange = urange + 1
     product = g() * erange
     low = product & MASK32
@@ -120,9 +129,12 @@ def _lognormal_len(g: MT, d: LogNormal, n: int) -> int:
             return v
 
 
-def gen_queries(q: int, lr: int, n: int, seed: int, trivial_check: bool = False) -> np.ndarray:
+def gen_queries(q: int, lr: int, n: int, seed: int, trivial_check: bool = False,
+                dtype=np.uint32) -> np.ndarray:
+    """dtype uint64: the reference's IS_LONG build (64-bit positions; the same draws while
+    n <= 2^32)."""
     qgen = 1024 if (trivial_check and q > 1024) else q
-    out = np.empty((qgen, 2), dtype=np.uint32)
+    out = np.empty((qgen, 2), dtype=dtype)
     if lr == -6:
         g = MT(seed)
         med = LogNormal(math.log(int(math.pow(float(n), 0.6))), 0.3)

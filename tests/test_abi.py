@@ -101,8 +101,19 @@ def test_threshold_rejection(alg, R, ok_t):
     rmq_amd.plan(1 << 20, ok_t, alg)
 
 
+def test_plan_of_the_64bit_form():
+    """n >= 2^31 (the reference's IS_LONG build) plans with the same recurrence in 64 bits."""
+    n = (1 << 34) + 5
+    p = rmq_amd.plan(n, 1 << 15, 20, cg_amount_log=1)
+    sizes = [n]
+    while sizes[-1] > (1 << 15):
+        sizes.append((sizes[-1] + 7) // 8)
+    assert p.sizes == sizes and p.R == 8
+    assert all(o % 32 == 0 for o in p.offsets[1:])
+
+
 def test_invalid_arguments():
-    for n, t, alg in ((0, 64, 16), (1 << 31, 64, 16), (100, 64, 17), (100, 0, 16), (100, 64, 2)):
+    for n, t, alg in ((0, 64, 16), ((1 << 38) + 1, 64, 16), (100, 64, 17), (100, 0, 16), (100, 64, 2)):
         with pytest.raises(rmq_amd.RMQError) as e:
             rmq_amd.plan(n, t, alg)
         assert e.value.status == rmq_amd.RMQ_EINVAL

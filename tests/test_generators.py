@@ -76,6 +76,27 @@ def test_random_tuples_against_restatement(lr):
         assert np.array_equal(a, b), (lr, n, seed)
 
 
+@pytest.mark.parametrize("lr", [-1, -2, -6, 7])
+def test_queries64_equal_32bit_below_2e32(lr):
+    """The IS_LONG generator draws through the same 64-bit unsigned type: identical pairs."""
+    for n, seed in ((1000, 3), (1 << 24, 27722), ((1 << 32) - 5, 11)):
+        a = rmq_amd.gen_queries(500, lr, n, seed).astype(np.uint64)
+        b = rmq_amd.gen_queries64(500, lr, n, seed)
+        assert np.array_equal(a, b), (lr, n)
+
+
+@pytest.mark.parametrize("lr", [-1, -2, -3, -6, 7])
+def test_queries64_above_2e32_against_restatement(lr):
+    """n > 2^32: 64-bit draws combine two 32-bit outputs (the restatement's wide branch)."""
+    for n, seed in (((1 << 32) + 17, 5), ((1 << 33) + 12345, 27722), ((1 << 37) + 3, 9)):
+        a = rmq_amd.gen_queries64(300, lr, n, seed)
+        b = pygen.gen_queries(300, lr, n, seed, dtype=np.uint64)
+        assert np.array_equal(a, b), (lr, n)
+        assert (a[:, 0] <= a[:, 1]).all() and (a[:, 1] < n).all()
+    if lr == -1:  # uniform lengths reach far above 2^32 only through the wide branch
+        assert int(rmq_amd.gen_queries64(300, -1, (1 << 37) + 3, 9)[:, 1].max()) > (1 << 33)
+
+
 def test_query_invariants():
     for lr in (-1, -2, -3, -6, 5):
         n = 100003
