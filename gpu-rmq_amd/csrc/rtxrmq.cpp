@@ -4,7 +4,7 @@
 //   ./rtxrmq <n> <q> <lr> <alg> [--log_bs L] [--reps R] [--dev D] [--nt T] [--seed S]
 //            [--check | --trivialCheck | --randTrivialCheck] [--save-time[=file]]
 //            [--save-input-data] [--nb B] [--save-power[=file]] [--log_build_threshold[=x]]
-//            [--cg_size_log G] [--cg_amount_log C] [--group_lanes K]
+//            [--cg_size_log G] [--cg_amount_log C] [--group_lanes K] [--long]
 //
 // alg: 16 (XXX / GPU-RMQ CL), 20 (hierarchical vector load / GPU-RMQ VL), 24 (XXX multi load),
 //      2 (GPU BASE); +100 returns leftmost indices instead of values (102, 116, 120, 124).
@@ -14,6 +14,8 @@
 // The runtime flags --cg_size_log / --cg_amount_log replace the reference's compile-time
 // XXX_CG_SIZE_LOG / XXX_CG_AMOUNT_LOG (CMakeLists.txt:61-67); --group_lanes picks this
 // library's lanes-per-query for algs 16/24 (0 = auto; never changes results).
+// --long is the reference's IS_LONG build (CMakeLists.txt:57-59, src/main.cu:21-27): 64-bit
+// positions, {l, r} pairs and indices; it is implied for n > 2^31 - 1 (up to 2^38).
 
 #include <getopt.h>
 #include <hip/hip_runtime_api.h>
@@ -44,7 +46,7 @@ constexpr int kTrivLog = 10;      // TRIVCHECK_LOG_NUM_DIFFQ, src/main.cu:30
 enum {
   ARG_BS = 1, ARG_NB, ARG_REPS, ARG_DEV, ARG_NT, ARG_SEED, ARG_CHECK, ARG_TIME, ARG_POWER,
   ARG_TRIVIALCHECK, ARG_THRESHOLD, ARG_SAVE_INPUTDATA, ARG_RAND_TRIVIALCHECK,
-  ARG_CG_SIZE, ARG_CG_AMOUNT, ARG_GROUP_LANES
+  ARG_CG_SIZE, ARG_CG_AMOUNT, ARG_GROUP_LANES, ARG_LONG
 };
 
 const char* alg_name(int alg) {
@@ -91,7 +93,8 @@ void print_help() {
           "   --trivialCheck            use 2^10 distinct queries, repeated\n"
           "   --save-time=<file>        append a CSV line\n"
           "   --save-input-data         cache arrays, queries and results in $RMQ_AUX_DIR\n"
-          "   --cg_size_log <g> --cg_amount_log <c> --group_lanes <k>\n");
+          "   --cg_size_log <g> --cg_amount_log <c> --group_lanes <k>\n"
+          "   --long                    64-bit positions and queries (IS_LONG; implied for n > 2^31-1)\n");
 }
 
 uint32_t log2_floor(uint64_t x) {
@@ -115,7 +118,7 @@ CmdArgs get_args(int argc, char* argv[]) {
     print_help();
     exit(EXIT_FAILURE);
   }
-  if (a.n < 1 || a.n > 0x7FFFFFFFll) {
+  if (a.n < 1 || a.n > (1ll << 38)) {
     fprintf(stderr, "Error: n=%ld is a non positive and therefore invalid array size!\n", (long)a.n);
     exit(EXIT_FAILURE);
   }
@@ -143,6 +146,7 @@ CmdArgs get_args(int argc, char* argv[]) {
       {"cg_size_log", required_argument, 0, ARG_CG_SIZE},
       {"cg_amount_log", required_argument, 0, ARG_CG_AMOUNT},
       {"group_lanes", required_argument, 0, ARG_GROUP_LANES},
+      {"long", no_argument, 0, ARG_LONG},
       {0, 0, 0, 0}};
   int opt, idx;
   while ((opt = getopt_long(argc, argv, "123456", opts, &idx)) != -1) {
@@ -172,9 +176,11 @@ CmdArgs get_args(int argc, char* argv[]) {
       case ARG_CG_SIZE: a.cg_size_log = static_cast<uint32_t>(atoi(optarg)); break;
       case ARG_CG_AMOUNT: a.cg_amount_log = static_cast<uint32_t>(atoi(optarg)); break;
       case ARG_GROUP_LANES: a.group_lanes = static_cast<uint32_t>(atoi(optarg)); break;
+      case ARG_LONG: a.is_long = 1; break;
       default: break;
     }
   }
+  if (a.n > 0x7FFFFFFFll) a.is_long = 1;
   if (!is_hier(a.alg)) {  // src/tools.h:221-235
     a.log_bs = 0;
     a.nb = 0;
@@ -236,15 +242,24 @@ std::vector<float> get_or_generate_array(const CmdArgs& a) {
   return A;
 }
 
-std::vector<uint32_t> get_or_generate_queries(const CmdArgs& a) {
-  std::vector<uint32_t> Q(2ull * a.q);
+int gen_queries_p(const CmdArgs& a, uint32_t* out) {
+  return rmq_gen_queries(a.q, a.lr, static_cast<uint64_t>(a.n), a.seed, a.trivialCheck, out);
+}
+int gen_queries_p(const CmdArgs& a, uint64_t* out) {
+  return rmq_gen_queries64(a.q, a.lr, static_cast<uint64_t>(a.n), a.seed, a.trivialCheck, out);
+}
+
+// {l, r} pairs of the position type P (the reference's n2Type: uint2 or ulong2).
+template <typename P>
+std::vector<P> get_or_generate_queries(const CmdArgs& a) {
+  std::vector<P> Q(2ull * a.q);
   const std::string f = make_filename(aux_dir() + "queries", a.n, a.q, a.lr, a.seed, a);
-  if (load_file(f, Q.data(), Q.size() * 4)) return Q;
-  if (rmq_gen_queries(a.q, a.lr, static_cast<uint64_t>(a.n), a.seed, a.trivialCheck, Q.data()) != 0) {
+  if (load_file(f, Q.data(), Q.size() * sizeof(P))) return Q;
+  if (gen_queries_p(a, Q.data()) != 0) {
     print_help();
     exit(EXIT_FAILURE);
   }
-  if (a.save_input_data) save_file(f, Q.data(), Q.size() * 4);
+  if (a.save_input_data) save_file(f, Q.data(), Q.size() * sizeof(P));
   return Q;
 }
 
@@ -302,41 +317,43 @@ void write_check_result(int check_result, const CmdArgs& a) {
 }
 
 // Expected answers from GPU BASE over `qs` queries (value bits and leftmost indices).
-void expected(const float* dA, int64_t n, const std::vector<uint32_t>& qs, std::vector<float>& val,
-              std::vector<uint32_t>& idx) {
+template <typename P>
+void expected(const float* dA, int64_t n, const std::vector<P>& qs, std::vector<float>& val,
+              std::vector<P>& idx) {
   const uint64_t q = qs.size() / 2;
   val.assign(q, 0.f);
   idx.assign(q, 0u);
   void *dq = nullptr, *dv = nullptr, *di = nullptr;
-  if (hipMalloc(&dq, qs.size() * 4 + 8) != hipSuccess || hipMalloc(&dv, q * 4 + 4) != hipSuccess ||
-      hipMalloc(&di, q * 4 + 4) != hipSuccess) {
+  if (hipMalloc(&dq, qs.size() * sizeof(P) + 16) != hipSuccess || hipMalloc(&dv, q * 4 + 4) != hipSuccess ||
+      hipMalloc(&di, q * sizeof(P) + 8) != hipSuccess) {
     fprintf(stderr, "Error: out of device memory for --check\n");
     exit(EXIT_FAILURE);
   }
-  (void)hipMemcpy(dq, qs.data(), qs.size() * 4, hipMemcpyHostToDevice);
-  const rmq_status st = rmq_gpu_base(dA, n, static_cast<uint32_t*>(dq), q, static_cast<float*>(dv),
-                                     static_cast<uint32_t*>(di), nullptr);
+  (void)hipMemcpy(dq, qs.data(), qs.size() * sizeof(P), hipMemcpyHostToDevice);
+  const rmq_status st = rtxrmq::base_p(dA, n, static_cast<const P*>(dq), static_cast<uint32_t>(q),
+                                       static_cast<float*>(dv), static_cast<P*>(di));
   if (st != RMQ_OK || hipDeviceSynchronize() != hipSuccess) {
     fprintf(stderr, "Error: GPU BASE failed: %s\n", rmq_status_str(st));
     exit(EXIT_FAILURE);
   }
   (void)hipMemcpy(val.data(), dv, q * 4, hipMemcpyDeviceToHost);
-  (void)hipMemcpy(idx.data(), di, q * 4, hipMemcpyDeviceToHost);
+  (void)hipMemcpy(idx.data(), di, q * sizeof(P), hipMemcpyDeviceToHost);
   (void)hipFree(dq);
   (void)hipFree(dv);
   (void)hipFree(di);
 }
 
 // Bit-exact comparison of result[pos[i]] with expected[i] (values by bit pattern).
-bool compare(const std::vector<float>& A, const std::vector<uint32_t>& Q, const void* result,
-             bool idx_mode, const std::vector<float>& ev, const std::vector<uint32_t>& ei,
+template <typename P>
+bool compare(const std::vector<float>& A, const std::vector<P>& Q, const void* result,
+             bool idx_mode, const std::vector<float>& ev, const std::vector<P>& ei,
              const std::vector<uint32_t>& pos) {
   uint64_t wrong = 0;
   for (size_t i = 0; i < pos.size(); ++i) {
     const uint32_t k = pos[i];
     bool ok;
     if (idx_mode) {
-      ok = static_cast<const uint32_t*>(result)[k] == ei[i];
+      ok = static_cast<const P*>(result)[k] == ei[i];
     } else {
       uint32_t a, b;
       memcpy(&a, static_cast<const float*>(result) + k, 4);
@@ -345,8 +362,9 @@ bool compare(const std::vector<float>& A, const std::vector<uint32_t>& Q, const 
     }
     if (!ok) {
       if (wrong < 10) {
-        printf("Error on %u-th query [%u,%u]: expected idx %u (value %.10f)\n", k, Q[2 * k],
-               Q[2 * k + 1], ei[i], A[ei[i]]);
+        printf("Error on %u-th query [%llu,%llu]: expected idx %llu (value %.10f)\n", k,
+               (unsigned long long)Q[2ull * k], (unsigned long long)Q[2ull * k + 1],
+               (unsigned long long)ei[i], A[ei[i]]);
       }
       ++wrong;
     }
@@ -355,13 +373,8 @@ bool compare(const std::vector<float>& A, const std::vector<uint32_t>& Q, const 
   return wrong == 0;
 }
 
-}  // namespace
-
-int main(int argc, char* argv[]) {
-  printf("----------------------------------\n");
-  printf("   rtxrmq driver (MI355X, gfx950)  \n");
-  printf("----------------------------------\n");
-  CmdArgs a = get_args(argc, argv);
+template <typename P>
+int run(CmdArgs a) {
   create_header(a, is_hier(a.alg));
   if (hipSetDevice(a.dev) != hipSuccess) {
     fprintf(stderr, "Error: cannot select device %d\n", a.dev);
@@ -382,13 +395,13 @@ int main(int argc, char* argv[]) {
   timer.restart();
   printf("Generating q=%-10u queries...........", a.q);
   fflush(stdout);
-  std::vector<uint32_t> Q = get_or_generate_queries(a);
-  uint32_t* dQ = nullptr;
-  if (hipMalloc(reinterpret_cast<void**>(&dQ), Q.size() * 4) != hipSuccess) {
+  std::vector<P> Q = get_or_generate_queries<P>(a);
+  P* dQ = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&dQ), Q.size() * sizeof(P)) != hipSuccess) {
     fprintf(stderr, "Error: out of device memory for the queries\n");
     exit(EXIT_FAILURE);
   }
-  (void)hipMemcpy(dQ, Q.data(), Q.size() * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(dQ, Q.data(), Q.size() * sizeof(P), hipMemcpyHostToDevice);
   printf("done: %f secs\n", timer.ms() / 1000.0);
   write_params(a);
 
@@ -420,7 +433,7 @@ int main(int argc, char* argv[]) {
                    : (a.trivialCheck ? "\nTRIVIALLY CHECKING RESULT:\n"
                                      : "\nRANDOMLY TRIVIALLY CHECKING RESULT:\n"));
     std::vector<uint32_t> pos;
-    std::vector<uint32_t> qs;
+    std::vector<P> qs;
     if (a.check) {
       pos.resize(a.q);
       for (uint32_t i = 0; i < a.q; ++i) pos[i] = i;
@@ -440,13 +453,13 @@ int main(int argc, char* argv[]) {
     }
     (void)kTrivLog;
     std::vector<float> ev;
-    std::vector<uint32_t> ei;
+    std::vector<P> ei;
     printf("Checking result...........................");
     fflush(stdout);
     expected(dA, a.n, qs, ev, ei);
     if (a.save_input_data && a.check) {
       save_file(make_filename(aux_dir() + "results_value", a.n, a.q, a.lr, a.seed, a), ev.data(), ev.size() * 4);
-      save_file(make_filename(aux_dir() + "results_index", a.n, a.q, a.lr, a.seed, a), ei.data(), ei.size() * 4);
+      save_file(make_filename(aux_dir() + "results_index", a.n, a.q, a.lr, a.seed, a), ei.data(), ei.size() * sizeof(P));
     }
     const bool pass = compare(A, Q, out, idx_mode, ev, ei, pos);
     a.save_time = save_time;
@@ -464,4 +477,14 @@ int main(int argc, char* argv[]) {
   (void)hipFree(dQ);
   printf("Benchmark Finished\n");
   return 0;
+}
+
+}  // namespace
+
+int main(int argc, char* argv[]) {
+  printf("----------------------------------\n");
+  printf("   rtxrmq driver (MI355X, gfx950)  \n");
+  printf("----------------------------------\n");
+  const CmdArgs a = get_args(argc, argv);
+  return a.is_long ? run<uint64_t>(a) : run<uint32_t>(a);
 }

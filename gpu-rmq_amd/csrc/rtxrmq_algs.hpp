@@ -8,7 +8,9 @@
 // appends the CSV fields, and returns a malloc'd host array the caller frees. An invalid t
 // prints the reference's message, completes the CSV line and exits(EXIT_FAILURE).
 // The *_idx variants return the leftmost argmin indices (no reference counterpart; they follow
-// CPU BASE IDX, src/cpu_methods.h:42-53).
+// CPU BASE IDX, src/cpu_methods.h:42-53). Every entry point is a template on the position type
+// P, the reference's nType (src/main.cu:21-27): uint32_t, or uint64_t for its IS_LONG build
+// (64-bit {l, r} pairs and indices, rmq_query64 / rmq_gpu_base64).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -32,6 +34,7 @@ struct CmdArgs {  // src/tools.h:26-35
   int save_time = 0, save_power = 0;
   uint32_t cg_size_log = 3, cg_amount_log = 2, group_lanes = 0;  // runtime forms of the
                                                                  // compile-time constants
+  int is_long = 0;  // IS_LONG (CMakeLists.txt:57-59): --long, or n > 2^31 - 1
   std::string time_file, power_file;
 };
 
@@ -98,9 +101,26 @@ inline void fail(rmq_status st, const char* what, uint64_t t, uint32_t R, int al
   exit(EXIT_FAILURE);
 }
 
+// Queries and GPU BASE by position type.
+inline rmq_status query_p(const rmq_hierarchy* h, const uint32_t* d_lr, uint32_t q, float* v, uint32_t* i) {
+  return rmq_query(h, d_lr, q, v, i, nullptr);
+}
+inline rmq_status query_p(const rmq_hierarchy* h, const uint64_t* d_lr, uint32_t q, float* v, uint64_t* i) {
+  return rmq_query64(h, d_lr, q, v, i, nullptr);
+}
+inline rmq_status base_p(const float* d_array, int64_t n, const uint32_t* d_lr, uint32_t q, float* v,
+                         uint32_t* i) {
+  return rmq_gpu_base(d_array, static_cast<uint64_t>(n), d_lr, q, v, i, nullptr);
+}
+inline rmq_status base_p(const float* d_array, int64_t n, const uint64_t* d_lr, uint32_t q, float* v,
+                         uint64_t* i) {
+  return rmq_gpu_base64(d_array, static_cast<uint64_t>(n), d_lr, q, v, i, nullptr);
+}
+
 // Shared body of the three algorithm entry points: build, reps x query, report, D2H.
+template <typename P>
 inline void* run_hierarchical(int alg, int64_t n, uint64_t t, uint32_t q, const float* d_array,
-                              const uint32_t* d_query, const CmdArgs& a, bool want_idx) {
+                              const P* d_query, const CmdArgs& a, bool want_idx) {
   rmq_build_options opt{};
   opt.cg_size_log = a.cg_size_log;
   opt.cg_amount_log = a.cg_amount_log;
@@ -112,7 +132,7 @@ inline void* run_hierarchical(int alg, int64_t n, uint64_t t, uint32_t q, const 
   if (st != RMQ_OK) fail(st, "rmq_plan", t, alg == RMQ_ALG_VECTOR_LOAD ? 4u << a.cg_amount_log
                                                   : 1u << (a.cg_size_log + a.cg_amount_log), alg, a);
   void* d_out = nullptr;
-  const size_t out_bytes = static_cast<size_t>(q) * 4;
+  const size_t out_bytes = static_cast<size_t>(q) * (want_idx ? sizeof(P) : 4);
   if (hipMalloc(&d_out, out_bytes ? out_bytes : 4) != hipSuccess) fail(RMQ_ENOMEM, "hipMalloc", t, R, alg, a);
 
   Timer timer;
@@ -134,8 +154,8 @@ inline void* run_hierarchical(int alg, int64_t n, uint64_t t, uint32_t q, const 
   fflush(stdout);
   timer.restart();
   for (int i = 0; i < a.reps; ++i) {
-    st = rmq_query(h, d_query, q, want_idx ? nullptr : static_cast<float*>(d_out),
-                   want_idx ? static_cast<uint32_t*>(d_out) : nullptr, nullptr);
+    st = query_p(h, d_query, q, want_idx ? nullptr : static_cast<float*>(d_out),
+                 want_idx ? static_cast<P*>(d_out) : nullptr);
     if (st != RMQ_OK) fail(st, "rmq_query", t, R, alg, a);
     if (hipDeviceSynchronize() != hipSuccess) fail(RMQ_EDEVICE, "query", t, R, alg, a);
   }
@@ -157,42 +177,45 @@ inline void* run_hierarchical(int alg, int64_t n, uint64_t t, uint32_t q, const 
   return out;
 }
 
-inline float* xxx_rmq(int64_t n, uint64_t t, uint32_t q, const float* d_array,
-                      const uint32_t* d_query, const CmdArgs& a) {
+template <typename P>
+inline float* xxx_rmq(int64_t n, uint64_t t, uint32_t q, const float* d_array, const P* d_query,
+                      const CmdArgs& a) {
   return static_cast<float*>(run_hierarchical(RMQ_ALG_XXX, n, t, q, d_array, d_query, a, false));
 }
 
+template <typename P>
 inline float* xxx_rmq_multi_load(int64_t n, uint64_t t, uint32_t q, const float* d_array,
-                                 const uint32_t* d_query, const CmdArgs& a) {
+                                 const P* d_query, const CmdArgs& a) {
   return static_cast<float*>(
       run_hierarchical(RMQ_ALG_XXX_MULTI_LOAD, n, t, q, d_array, d_query, a, false));
 }
 
+template <typename P>
 inline float* hierarchical_vector_load(int64_t n, uint64_t t, uint32_t q, const float* d_array,
-                                       const uint32_t* d_query, const CmdArgs& a) {
+                                       const P* d_query, const CmdArgs& a) {
   return static_cast<float*>(
       run_hierarchical(RMQ_ALG_VECTOR_LOAD, n, t, q, d_array, d_query, a, false));
 }
 
-inline uint32_t* hierarchical_rmq_idx(int alg, int64_t n, uint64_t t, uint32_t q,
-                                      const float* d_array, const uint32_t* d_query,
-                                      const CmdArgs& a) {
-  return static_cast<uint32_t*>(run_hierarchical(alg, n, t, q, d_array, d_query, a, true));
+template <typename P>
+inline P* hierarchical_rmq_idx(int alg, int64_t n, uint64_t t, uint32_t q, const float* d_array,
+                               const P* d_query, const CmdArgs& a) {
+  return static_cast<P*>(run_hierarchical(alg, n, t, q, d_array, d_query, a, true));
 }
 
 // GPU BASE (algs 2 / 102), gpu_rmq_basic / gpu_rmq_basic_idx, src/cuda_methods.cuh:71-137.
-inline void* gpu_rmq_basic(int64_t n, uint32_t q, const float* d_array, const uint32_t* d_query,
+template <typename P>
+inline void* gpu_rmq_basic(int64_t n, uint32_t q, const float* d_array, const P* d_query,
                            const CmdArgs& a, bool want_idx) {
   void* d_out = nullptr;
-  const size_t out_bytes = static_cast<size_t>(q) * 4;
+  const size_t out_bytes = static_cast<size_t>(q) * (want_idx ? sizeof(P) : 4);
   if (hipMalloc(&d_out, out_bytes ? out_bytes : 4) != hipSuccess) fail(RMQ_ENOMEM, "hipMalloc", 0, 0, 2, a);
   printf("Computing RMQs (%-16s,r=%-3i)...", want_idx ? "[GPU] BASE IDX" : "[GPU] BASE", a.reps);
   fflush(stdout);
   Timer timer;
   for (int i = 0; i < a.reps; ++i) {
-    const rmq_status st =
-        rmq_gpu_base(d_array, n, d_query, q, want_idx ? nullptr : static_cast<float*>(d_out),
-                     want_idx ? static_cast<uint32_t*>(d_out) : nullptr, nullptr);
+    const rmq_status st = base_p(d_array, n, d_query, q, want_idx ? nullptr : static_cast<float*>(d_out),
+                                 want_idx ? static_cast<P*>(d_out) : nullptr);
     if (st != RMQ_OK) fail(st, "rmq_gpu_base", 0, 0, 2, a);
     if (hipDeviceSynchronize() != hipSuccess) fail(RMQ_EDEVICE, "gpu base", 0, 0, 2, a);
   }

@@ -65,3 +65,22 @@ def test_rand_trivial_check_and_paper_constants(tmp_path, device):
     p = run(1 << 22, 1 << 20, -3, 20, "--seed", 913, "--reps", 1, "--trivialCheck",
             "--cg_amount_log", 1, "--log_bs", 5)
     assert p.returncode == 0 and "pass" in p.stdout, p.stdout + p.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("alg", [20, 120, 116, 124, 102])
+def test_long_positions_check(alg, tmp_path, device):
+    """--long: the reference's IS_LONG build (64-bit {l, r} pairs and indices) on an array below
+    2^31, checked bit-exactly against 64-bit GPU BASE."""
+    p = run(300007, 40000, -6, alg, "--seed", 5, "--reps", 1, "--check", "--long", "--log_bs", 10)
+    assert p.returncode == 0 and "pass" in p.stdout, p.stdout + p.stderr
+
+
+@pytest.mark.gpu
+def test_long_array_above_2e31(tmp_path, device):
+    """n = 2^31 + 7 turns the 64-bit form on by itself; a random sample is checked. Short ranges
+    (lr = -3): the check is GPU BASE, one lane scanning each range, so long ranges would take
+    minutes at this n (the long ranges of this form are checked in test_long.py)."""
+    p = run((1 << 31) + 7, 1 << 20, -3, 120, "--seed", 3, "--reps", 1, "--randTrivialCheck",
+            "--log_bs", 13, "--cg_amount_log", 1, timeout=300)
+    assert p.returncode == 0 and "pass" in p.stdout, p.stdout + p.stderr
