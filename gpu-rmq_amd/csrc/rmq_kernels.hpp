@@ -497,22 +497,38 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
     }
     if (right_tail) gR = (ek & 3u) ? dev::pick4(x, ek & 3u) : vK[ek];
   }
-  if (left_tail && gL <= bv) {
-    for (int k = static_cast<int>(K) - 1; k >= 0; --k) {
-      const uint32_t sk = RL * static_cast<uint32_t>(k);
-      const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);
-      const uint32_t base = lo & ~(R - 1u);
-      const uint32_t ok = k == 0 ? 0u : T.off(k);
-      const float* vk = k == 0 ? H.A : H.lv + ok;
-      const uint32_t pay0 = k == 0 ? dev::kDirect : ok;
+  // Both tails walk down together, one level per step, so that a step's left and right loads
+  // are in flight at once. Any interleaving is exact: a left element wins ties whenever it is
+  // folded, a right one never does, and each guard test compares with the best so far.
+  bool la = left_tail && gL <= bv;
+  bool ra = right_tail && gR < bv;
+  for (int k = static_cast<int>(K) - 1; k >= 0 && (la || ra); --k) {
+    const uint32_t uk = static_cast<uint32_t>(k);
+    const uint32_t sk = RL * uk;
+    const uint32_t ok = k == 0 ? 0u : T.off(uk);
+    const float* vk = k == 0 ? H.A : H.lv + ok;
+    const uint32_t szk = T.size(uk);
+    const uint32_t pay0 = k == 0 ? dev::kDirect : ok;
+    const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);  // left piece [lo, roundup_R(lo))
+    const uint32_t lbase = lo & ~(R - 1u);
+    const uint32_t hi = e >> sk;                                       // right piece [rlo, hi)
+    const uint32_t rlo = hi & ~(R - 1u);
+    float4 xl[V], xr[V];
+#pragma unroll
+    for (uint32_t s = 0; s < V; ++s) {
+      const uint32_t pl = lbase + 4u * s, pr = rlo + 4u * s;
+      xl[s] = la && (lo & (R - 1u)) != 0u && pl + 4u > lo ? load_level(vk, pl, szk, uk)
+                                                        : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+      xr[s] = ra && pr < hi ? load_level(vk, pr, szk, uk) : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+    }
+    if (la) {
       float g = RMQ_INF;
 #pragma unroll
       for (int s = static_cast<int>(V) - 1; s >= 0; --s) {
-        const uint32_t p = base + 4u * static_cast<uint32_t>(s);
-        if ((lo & (R - 1u)) != 0u && p + 4u > lo) {  // the vectors of [lo, roundup_R(lo))
-          const float4 x = load_level(vk, p, T.size(k), static_cast<uint32_t>(k));
-          if (p <= lo - 1u) g = dev::pick4(x, (lo - 1u) & 3u);  // next guard, lo - 1
-          const float xs[4] = {x.x, x.y, x.z, x.w};
+        const uint32_t p = lbase + 4u * static_cast<uint32_t>(s);
+        if ((lo & (R - 1u)) != 0u && p + 4u > lo) {
+          if (p <= lo - 1u) g = dev::pick4(xl[s], (lo - 1u) & 3u);  // next guard, lo - 1
+          const float xs[4] = {xl[s].x, xl[s].y, xl[s].z, xl[s].w};
 #pragma unroll
           for (int j = 3; j >= 0; --j) {
             const bool t = p + static_cast<uint32_t>(j) >= lo && xs[j] <= bv;
@@ -521,27 +537,21 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
           }
         }
       }
-      if (k == 0 || !(l < (lo << sk))) break;
-      if ((lo & 3u) == 0) g = vk[lo - 1u];
-      if (g > bv) break;
+      if (k == 0 || !(l < (lo << sk))) {
+        la = false;
+      } else {
+        if ((lo & 3u) == 0) g = vk[lo - 1u];
+        la = !(g > bv);
+      }
     }
-  }
-  if (right_tail && gR < bv) {
-    for (int k = static_cast<int>(K) - 1; k >= 0; --k) {
-      const uint32_t sk = RL * static_cast<uint32_t>(k);
-      const uint32_t hi = e >> sk;
-      const uint32_t lo = hi & ~(R - 1u);
-      const uint32_t ok = k == 0 ? 0u : T.off(k);
-      const float* vk = k == 0 ? H.A : H.lv + ok;
-      const uint32_t pay0 = k == 0 ? dev::kDirect : ok;
+    if (ra) {
       float g = RMQ_INF;
 #pragma unroll
       for (uint32_t s = 0; s < V; ++s) {
-        const uint32_t p = lo + 4u * s;
+        const uint32_t p = rlo + 4u * s;
         if (p < hi) {
-          const float4 x = load_level(vk, p, T.size(k), static_cast<uint32_t>(k));
-          if (hi - p < 4u) g = dev::pick4(x, hi & 3u);  // next guard, hi
-          const float xs[4] = {x.x, x.y, x.z, x.w};
+          if (hi - p < 4u) g = dev::pick4(xr[s], hi & 3u);  // next guard, hi
+          const float xs[4] = {xr[s].x, xr[s].y, xr[s].z, xr[s].w};
 #pragma unroll
           for (uint32_t j = 0; j < 4; ++j) {
             const bool t = p + j < hi && xs[j] < bv;
@@ -550,9 +560,12 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
           }
         }
       }
-      if (k == 0 || !((hi << sk) < e)) break;
-      if ((hi & 3u) == 0) g = vk[hi];
-      if (!(g < bv)) break;
+      if (k == 0 || !((hi << sk) < e)) {
+        ra = false;
+      } else {
+        if ((hi & 3u) == 0) g = vk[hi];
+        ra = g < bv;
+      }
     }
   }
   if constexpr (IDX) return GroupAnswer{bv, (bpay & dev::kDirect) ? (bpay & ~dev::kDirect) : H.li[bpay]};
