@@ -89,6 +89,19 @@ __device__ __forceinline__ const float* level_values(const HierView& H, uint32_t
   return k == 0 ? H.A : H.lv + H.off[k];
 }
 
+// Where a solver finds a level's offset and size (one lookup per level it visits): in the
+// HierView itself, or in a copy the kernel placed in LDS.
+struct LevelsOf {
+  const HierView& H;
+  __device__ __forceinline__ uint32_t off(uint32_t k) const { return H.off[k]; }
+  __device__ __forceinline__ uint32_t size(uint32_t k) const { return H.size[k]; }
+};
+struct LevelsLds {
+  const uint32_t* meta;  // LDS: off[kMaxLevels], then size[kMaxLevels]
+  __device__ __forceinline__ uint32_t off(uint32_t k) const { return meta[k]; }
+  __device__ __forceinline__ uint32_t size(uint32_t k) const { return meta[kMaxLevels + k]; }
+};
+
 template <int G, int RL, bool IDX>
 __device__ __forceinline__ GroupAnswer solve_cl(uint32_t l, uint32_t e, uint32_t gl,
                                                 const HierView& H) {
@@ -163,26 +176,30 @@ __device__ __forceinline__ void fold_piece(dev::Cand& b, float& bv, const float*
   }
 }
 
-template <int G, int RL, bool IDX>
+template <int G, int RL, bool IDX, class LT>
 __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, uint32_t gl,
-                                                       const HierView& H) {
+                                                       const HierView& H, const LT& T) {
   constexpr uint32_t R = 1u << RL;
   constexpr uint32_t STEP = 4 * G;
-  // climb with the reference's stopping rule; K = level of the final range
-  uint32_t K = 0, lk = l, ek = e;
-  for (uint32_t lev = 0; lev + 1 < H.num_levels; ++lev) {
-    if (K == lev && ek - lk > 2 * R) {
-      lk = (lk + R - 1) >> RL;
-      ek >>= RL;
-      K = lev + 1;
-    }
+  // K = the highest level whose aligned blocks include one inside [l, e), in O(1) (the
+  // reference climbs while the range is longer than 2R; both decompose [l, e) exactly)
+  const uint32_t top = H.num_levels - 1;
+  uint32_t K = min((31u - static_cast<uint32_t>(__clz(e - l))) / RL, top);
+  uint32_t shK = RL * K;
+  uint32_t lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
+  uint32_t ek = e >> shK;
+  if (lk >= ek) {
+    --K;
+    shK -= RL;
+    lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
+    ek = e >> shK;
   }
-  const uint32_t shK = RL * K;
-  const float* vK = level_values(H, K);
-  const uint32_t szK = H.size[K];
-  const uint32_t payK = K == 0 ? dev::kDirect : H.off[K];
-  const bool left_tail = K > 0 && l < (lk << shK);
-  const bool right_tail = K > 0 && (ek << shK) < e;
+  const uint32_t offK = K == 0 ? 0u : T.off(K);
+  const float* vK = K == 0 ? H.A : H.lv + offK;
+  const uint32_t szK = T.size(K);
+  const uint32_t payK = K == 0 ? dev::kDirect : offK;
+  const bool left_tail = l < (lk << shK);
+  const bool right_tail = (ek << shK) < e;
   // guards are issued together with the final range
   const float gL = left_tail ? vK[lk - 1] : RMQ_INF;
   const float gR = right_tail ? vK[ek] : RMQ_INF;
@@ -217,51 +234,41 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
     bv = dev::group_min<G>(bv);
   }
 
-  // left tail, walking down (rare for random data)
-  if (left_tail && !(gL > bv)) {
-    for (int k = static_cast<int>(K) - 1; k >= 0; --k) {
-      const uint32_t sk = RL * static_cast<uint32_t>(k);
-      const uint32_t lo = static_cast<uint32_t>((static_cast<uint64_t>(l) + (1ull << sk) - 1) >> sk);
-      const uint32_t hi = (lo + R - 1) & ~(R - 1);
-      const float* vk = level_values(H, k);
-      fold_piece<G, RL, IDX>(b, bv, vk, H.size[k], lo, hi, gl, sk, k == 0 ? dev::kDirect : H.off[k]);
-      if constexpr (IDX) {
-        dev::group_cand<G>(b);
-        bv = b.v;
-      } else {
-        bv = dev::group_min<G>(bv);
-      }
-      if (k == 0 || !(l < (lo << sk))) break;  // nothing deeper
-      if (vk[lo - 1] > bv) break;              // deeper guard cannot win
+  // Both tails walk down together (rare for random data): per level the group folds the left
+  // piece and the right piece and reduces once. Candidates compare by (value, span start), so
+  // the order of the pieces does not matter; bv is group-reduced, so every decision is uniform
+  // across the group.
+  bool la = left_tail && !(gL > bv);
+  bool ra = right_tail && gR < bv;
+  for (int k = static_cast<int>(K) - 1; k >= 0 && (la || ra); --k) {
+    const uint32_t uk = static_cast<uint32_t>(k);
+    const uint32_t sk = RL * uk;
+    const uint32_t ok = k == 0 ? 0u : T.off(uk);
+    const float* vk = k == 0 ? H.A : H.lv + ok;
+    const uint32_t szk = T.size(uk);
+    const uint32_t pay = k == 0 ? dev::kDirect : ok;
+    const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);
+    const uint32_t hi = e >> sk;
+    if (la) fold_piece<G, RL, IDX>(b, bv, vk, szk, lo, (lo + R - 1) & ~(R - 1), gl, sk, pay);
+    if (ra) fold_piece<G, RL, IDX>(b, bv, vk, szk, hi & ~(R - 1), hi, gl, sk, pay);
+    if constexpr (IDX) {
+      dev::group_cand<G>(b);
+      bv = b.v;
+    } else {
+      bv = dev::group_min<G>(bv);
     }
-  }
-  // right tail
-  if (right_tail && gR < bv) {
-    for (int k = static_cast<int>(K) - 1; k >= 0; --k) {
-      const uint32_t sk = RL * static_cast<uint32_t>(k);
-      const uint32_t hi = e >> sk;
-      const uint32_t lo = hi & ~(R - 1);
-      const float* vk = level_values(H, k);
-      fold_piece<G, RL, IDX>(b, bv, vk, H.size[k], lo, hi, gl, sk, k == 0 ? dev::kDirect : H.off[k]);
-      if constexpr (IDX) {
-        dev::group_cand<G>(b);
-        bv = b.v;
-      } else {
-        bv = dev::group_min<G>(bv);
-      }
-      if (k == 0 || !((hi << sk) < e)) break;
-      if (!(vk[hi] < bv)) break;
-    }
+    if (la) la = k != 0 && l < (lo << sk) && !(vk[lo - 1] > bv);  // deeper left pieces can still win
+    if (ra) ra = k != 0 && (hi << sk) < e && vk[hi] < bv;
   }
   if constexpr (IDX) return GroupAnswer{b.v, dev::cand_index(b, H.li)};
   else return GroupAnswer{bv, dev::kNoIdx};
 }
 
 // P selects the pruned evaluation order (default) or the full one.
-template <int G, int RL, bool P, bool IDX>
+template <int G, int RL, bool P, bool IDX, class LT>
 __device__ __forceinline__ GroupAnswer solve_group(uint32_t l, uint32_t e, uint32_t gl,
-                                                   const HierView& H) {
-  if constexpr (P) return solve_cl_pruned<G, RL, IDX>(l, e, gl, H);
+                                                   const HierView& H, const LT& T) {
+  if constexpr (P) return solve_cl_pruned<G, RL, IDX>(l, e, gl, H, T);
   else return solve_cl<G, RL, IDX>(l, e, gl, H);
 }
 
@@ -269,11 +276,11 @@ __device__ __forceinline__ GroupAnswer solve_group(uint32_t l, uint32_t e, uint3
 // only differ from that when the minimum is zero (+0 and -0 compare equal): the zero's sign is
 // taken from the array's zero signs, and only when the array holds both is the query answered
 // again with index tracking.
-template <int G, int RL, bool P>
+template <int G, int RL, bool P, class LT>
 __device__ __forceinline__ float value_exact(uint32_t l, uint32_t e, uint32_t gl,
-                                             const HierView& H, uint32_t zeros) {
-  GroupAnswer a = solve_group<G, RL, P, false>(l, e, gl, H);
-  if (a.v == 0.0f && !dev::zero_known(zeros, a.v)) a = solve_group<G, RL, P, true>(l, e, gl, H);
+                                             const HierView& H, const LT& T, uint32_t zeros) {
+  GroupAnswer a = solve_group<G, RL, P, false>(l, e, gl, H, T);
+  if (a.v == 0.0f && !dev::zero_known(zeros, a.v)) a = solve_group<G, RL, P, true>(l, e, gl, H, T);
   return a.v;
 }
 
@@ -285,6 +292,11 @@ template <int G, int RL, bool P, bool IDX>
 __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ Q, uint32_t q,
                                                       float* __restrict__ out_v,
                                                       uint32_t* __restrict__ out_i, HierView H) {
+  __shared__ uint32_t s_meta[2 * kMaxLevels];
+  if (threadIdx.x < 2 * kMaxLevels)
+    s_meta[threadIdx.x] = threadIdx.x < kMaxLevels ? H.off[threadIdx.x] : H.size[threadIdx.x - kMaxLevels];
+  __syncthreads();
+  const LevelsLds T{s_meta};
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t gl = lane & (G - 1);
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -308,11 +320,11 @@ __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ 
       float v;
       uint32_t i = dev::kNoIdx;
       if constexpr (IDX) {
-        const GroupAnswer a = solve_group<G, RL, P, true>(lk, ek, gl, H);
+        const GroupAnswer a = solve_group<G, RL, P, true>(lk, ek, gl, H, T);
         v = a.v;
         i = a.idx;
       } else {
-        v = value_exact<G, RL, P>(lk, ek, gl, H, zeros);
+        v = value_exact<G, RL, P>(lk, ek, gl, H, T, zeros);
       }
       if (gl == static_cast<uint32_t>(k)) {
         my_v = v;
@@ -334,6 +346,11 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
                                                         float* __restrict__ out_v,
                                                         uint32_t* __restrict__ out_i,
                                                         HierView H) {
+  __shared__ uint32_t s_meta[2 * kMaxLevels];
+  if (threadIdx.x < 2 * kMaxLevels)
+    s_meta[threadIdx.x] = threadIdx.x < kMaxLevels ? H.off[threadIdx.x] : H.size[threadIdx.x - kMaxLevels];
+  __syncthreads();
+  const LevelsLds T{s_meta};
   const uint32_t gl = threadIdx.x & (G - 1);
   const uint64_t grp = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / G;
   const uint64_t ngrp = static_cast<uint64_t>(gridDim.x) * kBlock / G;
@@ -343,11 +360,11 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
     float v;
     uint32_t i = dev::kNoIdx;
     if constexpr (IDX) {
-      const GroupAnswer a = solve_group<G, RL, P, true>(p.x, p.y + 1, gl, H);
+      const GroupAnswer a = solve_group<G, RL, P, true>(p.x, p.y + 1, gl, H, T);
       v = a.v;
       i = a.idx;
     } else {
-      v = value_exact<G, RL, P>(p.x, p.y + 1, gl, H, zeros);
+      v = value_exact<G, RL, P>(p.x, p.y + 1, gl, H, T, zeros);
     }
     if (gl == 0) {
       if (out_v) out_v[qi] = v;
@@ -420,29 +437,6 @@ __device__ __forceinline__ GroupAnswer solve_vl(uint32_t l, uint32_t e, const Hi
   else return GroupAnswer{bv, dev::kNoIdx};
 }
 
-// Pruned evaluation order for alg 20 (same rules as solve_cl_pruned, one lane per query; the
-// climb stops when r - l <= R or the aligned bounds cross, src/xxx.h:903-913).
-// Level offsets and sizes as the lane solver looks them up (one lookup per level it visits):
-// from the HierView itself, or from a copy the kernel placed in LDS.
-struct LevelsOf {
-  const HierView& H;
-  __device__ __forceinline__ uint32_t off(uint32_t k) const { return H.off[k]; }
-  __device__ __forceinline__ uint32_t size(uint32_t k) const { return H.size[k]; }
-};
-struct LevelsLds {
-  const uint32_t* meta;  // LDS: off[kMaxLevels], then size[kMaxLevels]
-  __device__ __forceinline__ uint32_t off(uint32_t k) const { return meta[k]; }
-  __device__ __forceinline__ uint32_t size(uint32_t k) const { return meta[kMaxLevels + k]; }
-};
-
-// Lane-per-query pruned evaluation. The final range is taken at the highest level K whose
-// aligned blocks include one inside [l, e), found in O(1) from the length (any level whose
-// range is nonempty yields the same answer; this one leaves at most 2R - 2 entries to scan
-// below the top level). Sides need no position keys: every left piece lies left of all that was
-// read before it, and every right piece right of it, so a left element wins ties (folded right
-// to left with <=) and a right element must be strictly smaller (folded left to right with <).
-// In the final range, folded left to right with <, the first entry seeds the answer, so a range
-// of +inf answers with its leftmost entry.
 // A 16-byte vector of level k at p (p % 4 == 0, p < size). Upper levels sit in the level buffer,
 // padded to 32 entries past each level's end, so their vectors need no bounds test; level 0 is
 // the caller's array of exactly n floats.
@@ -451,6 +445,15 @@ __device__ __forceinline__ float4 load_level(const float* __restrict__ v, uint32
   return dev::load4(v, p, size);
 }
 
+// Lane-per-query pruned evaluation (the pieces of alg 20, read in the order of
+// solve_cl_pruned). The final range is taken at the highest level K whose
+// aligned blocks include one inside [l, e), found in O(1) from the length (any level whose
+// range is nonempty yields the same answer; this one leaves at most 2R - 2 entries to scan
+// below the top level). Sides need no position keys: every left piece lies left of all that was
+// read before it, and every right piece right of it, so a left element wins ties (folded right
+// to left with <=) and a right element must be strictly smaller (folded left to right with <).
+// In the final range, folded left to right with <, the first entry seeds the answer, so a range
+// of +inf answers with its leftmost entry.
 template <int RL, bool IDX, class LT>
 __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
   constexpr uint32_t R = 1u << RL;

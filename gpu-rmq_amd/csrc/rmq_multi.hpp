@@ -104,11 +104,11 @@ __device__ __forceinline__ void unit_groups(const HierView& H, const uint2* __re
       float v;
       uint32_t i = dev::kNoIdx;
       if constexpr (IDX) {
-        const GroupAnswer a = solve_cl_pruned<G, kMultiRLog, true>(lk, ek, gl, H);
+        const GroupAnswer a = solve_cl_pruned<G, kMultiRLog, true>(lk, ek, gl, H, LevelsOf{H});
         v = a.v;
         i = a.idx;
       } else {
-        v = value_exact<G, kMultiRLog, true>(lk, ek, gl, H, *H.zeros);
+        v = value_exact<G, kMultiRLog, true>(lk, ek, gl, H, LevelsOf{H}, *H.zeros);
       }
       if (gl == static_cast<uint32_t>(k)) {
         my_v = v;
