@@ -516,13 +516,19 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
     const uint32_t lbase = lo & ~(R - 1u);
     const uint32_t hi = e >> sk;                                       // right piece [rlo, hi)
     const uint32_t rlo = hi & ~(R - 1u);
-    float4 xl[V], xr[V];
+    // R <= 8: both pieces' vectors are loaded before either is folded; wider blocks load per
+    // side (holding 2 x R/4 vectors would cost the occupancy the overlap is meant to buy)
+    constexpr bool kEarly = V <= 2;
+    constexpr uint32_t VE = kEarly ? V : 1;
+    float4 xl[VE], xr[VE];
+    if constexpr (kEarly) {
 #pragma unroll
-    for (uint32_t s = 0; s < V; ++s) {
-      const uint32_t pl = lbase + 4u * s, pr = rlo + 4u * s;
-      xl[s] = la && (lo & (R - 1u)) != 0u && pl + 4u > lo ? load_level(vk, pl, szk, uk)
-                                                        : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
-      xr[s] = ra && pr < hi ? load_level(vk, pr, szk, uk) : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+      for (uint32_t s = 0; s < V; ++s) {
+        const uint32_t pl = lbase + 4u * s, pr = rlo + 4u * s;
+        xl[s] = la && (lo & (R - 1u)) != 0u && pl + 4u > lo ? load_level(vk, pl, szk, uk)
+                                                          : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+        xr[s] = ra && pr < hi ? load_level(vk, pr, szk, uk) : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+      }
     }
     if (la) {
       float g = RMQ_INF;
@@ -530,8 +536,9 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
       for (int s = static_cast<int>(V) - 1; s >= 0; --s) {
         const uint32_t p = lbase + 4u * static_cast<uint32_t>(s);
         if ((lo & (R - 1u)) != 0u && p + 4u > lo) {
-          if (p <= lo - 1u) g = dev::pick4(xl[s], (lo - 1u) & 3u);  // next guard, lo - 1
-          const float xs[4] = {xl[s].x, xl[s].y, xl[s].z, xl[s].w};
+          const float4 x = kEarly ? xl[kEarly ? s : 0] : load_level(vk, p, szk, uk);
+          if (p <= lo - 1u) g = dev::pick4(x, (lo - 1u) & 3u);  // next guard, lo - 1
+          const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
           for (int j = 3; j >= 0; --j) {
             const bool t = p + static_cast<uint32_t>(j) >= lo && xs[j] <= bv;
@@ -553,8 +560,9 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
       for (uint32_t s = 0; s < V; ++s) {
         const uint32_t p = rlo + 4u * s;
         if (p < hi) {
-          if (hi - p < 4u) g = dev::pick4(xr[s], hi & 3u);  // next guard, hi
-          const float xs[4] = {xr[s].x, xr[s].y, xr[s].z, xr[s].w};
+          const float4 x = kEarly ? xr[kEarly ? s : 0] : load_level(vk, p, szk, uk);
+          if (hi - p < 4u) g = dev::pick4(x, hi & 3u);  // next guard, hi
+          const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
           for (uint32_t j = 0; j < 4; ++j) {
             const bool t = p + j < hi && xs[j] < bv;

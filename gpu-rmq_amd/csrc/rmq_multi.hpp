@@ -178,8 +178,10 @@ __device__ __forceinline__ void queue_shape(const uint32_t* __restrict__ qstart,
   dyn0 = share * waves;
 }
 
+// Eight wavefronts per SIMD (64 VGPRs): the kernel carries every variant's solver, and the
+// occupancy is worth more than the few registers the compiler then spills (measured).
 template <bool IDX>
-__global__ __launch_bounds__(kBlock) void k_query_batch(
+__global__ __launch_bounds__(kBlock, 8) void k_query_batch(
     const HierView* __restrict__ views, const uint32_t* __restrict__ variant,
     const QueryTile* __restrict__ tiles, const uint32_t* __restrict__ qstart,
     uint32_t* __restrict__ counter, const uint2* __restrict__ Q, float* __restrict__ ov,
