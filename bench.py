@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--log-n", type=int, default=26)
     ap.add_argument("--log-q", type=int, default=26)
     ap.add_argument("--lr", type=int, default=-2)
-    ap.add_argument("--alg", default="", help="16, 16/g4, 16/g64, 24/g4, 24/g64, 20, 20/c0 or 20/c1 "
+    ap.add_argument("--alg", default="", help="16, 16/g1, 16/g4, 16/g8, 16/g64, 24/g4, 24/g64, 20, 20/c0 or 20/c1 "
                                               "(empty = sweep)")
     ap.add_argument("--t", type=int, default=0, help="scan threshold (0 = sweep)")
     ap.add_argument("--mode", choices=["index", "value", "both"], default="index")
@@ -341,13 +341,15 @@ def main():
     # --- choose (alg, t): explicit, default, or a short sweep ------------------------------
     opts_of = {"16": {}, "16/g4": dict(group_lanes=4), "24/g4": dict(group_lanes=4), "20": {},
                "20/c0": dict(cg_amount_log=0), "20/c1": dict(cg_amount_log=1),
-               "16/g64": dict(group_lanes=64), "24/g64": dict(group_lanes=64)}
+               "16/g64": dict(group_lanes=64), "24/g64": dict(group_lanes=64),
+               "16/g1": dict(group_lanes=1), "16/g8": dict(group_lanes=8)}
     if args.alg and args.t:
         candidates = [(args.alg, args.t)]
     elif args.no_sweep:
         candidates = [(DEFAULT_ALG, DEFAULT_T)]
     else:
-        candidates = [(v, 1 << tl) for v in ("16", "16/g4", "24/g4") for tl in (6, 10, 15, 18)]
+        # "16" is the automatic choice (a lane per query at R = 32); g4 / g8 are lane groups
+        candidates = [(v, 1 << tl) for v in ("16", "16/g4", "16/g8", "24/g4") for tl in (6, 10, 15, 18)]
         candidates += [(v, 1 << tl) for v in ("20", "20/c1", "20/c0") for tl in (6, 10, 13, 15)]
         candidates += [(v, 1 << 18) for v in ("16/g64", "24/g64")]  # a wave per query, long scans
     sweep = []

@@ -45,6 +45,26 @@ __global__ __launch_bounds__(kBlock) void k_gpu_base(const float* __restrict__ A
 // One word set from the stream (a kernel, not a memset, so that a captured build stays valid).
 __global__ void k_set_word(uint32_t* __restrict__ w, uint32_t v) { *w = v; }
 
+// Row j >= 1 of the sparse table over the top level top[0, m): entry i (i + 2^j <= m) is the
+// leftmost minimum of [i, i + 2^j), from two entries of row j - 1 (row 1 from the level itself).
+// Entries past m - 2^j are never read and stay unwritten.
+__global__ __launch_bounds__(kBlock) void k_top_row(const float* __restrict__ top, uint32_t m,
+                                                    uint2* __restrict__ tt, uint32_t stride, uint32_t j) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t h = 1u << (j - 1u);
+  if (static_cast<uint64_t>(i) + 2u * h > m) return;
+  uint2 a, b;
+  if (j == 1) {
+    a = make_uint2(dev::f2u(top[i]), i);
+    b = make_uint2(dev::f2u(top[i + 1]), i + 1);
+  } else {
+    const uint2* prev = tt + static_cast<size_t>(j - 2u) * stride;
+    a = prev[i];
+    b = prev[i + h];
+  }
+  tt[static_cast<size_t>(j - 1u) * stride + i] = dev::u2f(b.x) < dev::u2f(a.x) ? b : a;
+}
+
 }  // namespace rmq
 
 struct rmq_hierarchy {
@@ -59,6 +79,7 @@ struct rmq_hierarchy {
   float* lv;
   uint32_t* li;
   uint32_t* zeros;  // device word: kZeroPos | kZeroNeg of level 0
+  uint2* tt = nullptr;  // sparse table over the top level (null when not built)
   HierView view;
   // 64-bit form: chunks of 2^chunk_log elements, each its own hierarchy (chunk_log = 0: one
   // hierarchy over the whole array). views / cmin / cidx are device tables: the chunks' views
@@ -79,6 +100,10 @@ constexpr uint64_t kMaxN = 0x7FFFFFFFull;  // int32 n, as the reference's CmdArg
 constexpr uint64_t kMaxN64 = 1ull << 38;    // the 64-bit form (nType = uint64_t, src/tools.h:27-29)
 constexpr uint32_t kChunkLog = 30;          // chunk size of the 64-bit form
 constexpr uint64_t kMaxGrid = 1ull << 20;
+// The top level's sparse table is built when the top level holds at least kTopTableMin
+// entries and the table fits in kTopTableMax entries of 8 bytes (64 MiB).
+constexpr uint64_t kTopTableMin = 64;
+constexpr uint64_t kTopTableMax = 1ull << 23;
 
 struct Plan {
   uint32_t R_log = 0;
@@ -115,10 +140,18 @@ rmq_status make_plan(uint64_t n, uint64_t t, int alg, const rmq_build_options* o
     if (R_log < 3 || R_log > 7) return RMQ_EUNSUPPORTED;
     if (t < 2 * (1ull << R_log)) return RMQ_ETHRESHOLD;
     if (G == 0) {
-      const uint32_t v = (1u << R_log) / 4;  // one 16-byte vector per lane per block
-      G = v < 4 ? 4 : (v > 16 ? 16 : v);
+      // a lane per query where it runs (measured fastest for uniform, lognormal and mixed
+      // lengths once long top-level ranges come from the sparse table); else one 16-byte
+      // vector per lane per block
+      const bool full = opt && (opt->flags & RMQ_BUILD_FULL);
+      const uint32_t v = (1u << R_log) / 4;
+      G = (R_log <= 6 && !full) ? 1 : (v < 4 ? 4 : (v > 16 ? 16 : v));
     }
-    if (G != 4 && G != 8 && G != 16 && G != 32 && G != 64) return RMQ_EUNSUPPORTED;
+    if (G == 1) {  // a lane per query: the lane solver on this R = 2^(g+c) hierarchy
+      if (R_log > 6 || (opt && (opt->flags & RMQ_BUILD_FULL))) return RMQ_EUNSUPPORTED;
+    } else if (G != 4 && G != 8 && G != 16 && G != 32 && G != 64) {
+      return RMQ_EUNSUPPORTED;
+    }
   }
   P->R_log = R_log;
   P->G = G;
@@ -243,6 +276,15 @@ rmq_status query_dispatch(const rmq_hierarchy* h, const uint2* Q, uint32_t q, fl
       default: return RMQ_EUNSUPPORTED;
     }
   }
+  if (h->group_lanes == 1) {  // algs 16 / 24 with a lane per query (R <= 64, pruned order)
+    switch (h->R_log) {
+      case 3: return launch_vl<3>(false, Q, q, ov, oi, H, s);
+      case 4: return launch_vl<4>(false, Q, q, ov, oi, H, s);
+      case 5: return launch_vl<5>(false, Q, q, ov, oi, H, s);
+      case 6: return launch_vl<6>(false, Q, q, ov, oi, H, s);
+      default: return RMQ_EUNSUPPORTED;
+    }
+  }
   switch (h->R_log) {
     case 3: return launch_cl_g<3>(h->group_lanes, (h->flags & RMQ_BUILD_FULL) != 0, h->alg, Q, q, ov, oi, H, s);
     case 4: return launch_cl_g<4>(h->group_lanes, (h->flags & RMQ_BUILD_FULL) != 0, h->alg, Q, q, ov, oi, H, s);
@@ -266,6 +308,30 @@ void launch_query64(const ulonglong2* Q, uint64_t q, float* ov, uint64_t* oi, co
                     uint32_t clog, const float* cv, const uint64_t* ci, hipStream_t s) {
   const uint64_t grid = (q + kBlock - 1) / kBlock;
   rmq::k_query64<RL><<<static_cast<uint32_t>(grid), kBlock, 0, s>>>(Q, q, ov, oi, views, clog, cv, ci);
+}
+
+// The sparse table over the top level m = size[top]: floor(log2 m) rows of m entries (row
+// stride rounded up to 64), one launch per row. Skipped for the reference's full order, when
+// the caller asks, for small top levels and for tables above kTopTableMax entries.
+rmq_status build_top_table(rmq_hierarchy* h, const Plan& P, const float* d_array, hipStream_t s) {
+  const uint32_t top = P.num_levels - 1;
+  const uint64_t m = P.size[top];
+  if ((h->flags & (RMQ_BUILD_FULL | RMQ_BUILD_NO_TOP_TABLE)) || m < kTopTableMin) return RMQ_OK;
+  uint32_t rows = 0;
+  while ((2ull << rows) <= m) ++rows;  // floor(log2 m)
+  const uint64_t stride = (m + 63) & ~63ull;
+  if (rows * stride > kTopTableMax) return RMQ_OK;
+  if (hipMalloc(reinterpret_cast<void**>(&h->tt), rows * stride * sizeof(uint2)) != hipSuccess) {
+    (void)hipGetLastError();
+    return RMQ_ENOMEM;
+  }
+  const float* tv = top == 0 ? d_array : h->lv + P.off[top];
+  const uint32_t grid = static_cast<uint32_t>((m + kBlock - 1) / kBlock);
+  for (uint32_t j = 1; j <= rows; ++j)
+    rmq::k_top_row<<<grid, kBlock, 0, s>>>(tv, static_cast<uint32_t>(m), h->tt, static_cast<uint32_t>(stride), j);
+  h->view.tt = h->tt;
+  h->view.tstride = static_cast<uint32_t>(stride);
+  return last_error();
 }
 
 }  // namespace
@@ -443,6 +509,11 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
       return bs;
     }
   }
+  const rmq_status ts = build_top_table(h, P, d_array, s);
+  if (ts != RMQ_OK) {
+    rmq_free(h);
+    return ts;
+  }
   *out = h;
   return RMQ_OK;
 }
@@ -473,6 +544,7 @@ void rmq_free(rmq_hierarchy* h) {
   if (h->lv) (void)hipFree(h->lv);
   if (h->li) (void)hipFree(h->li);
   if (h->zeros) (void)hipFree(h->zeros);
+  if (h->tt) (void)hipFree(h->tt);
   delete h;
 }
 

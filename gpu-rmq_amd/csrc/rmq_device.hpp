@@ -20,6 +20,8 @@ struct HierView {
   const float* lv;       // level buffer, values  (levels 1..num_levels-1)
   const uint32_t* li;    // level buffer, leftmost level-0 index of each value
   const uint32_t* zeros; // which signs of zero level 0 holds (kZeroPos | kZeroNeg), see below
+  const uint2* tt;       // sparse table over the top level (dev::top_lookup), or null
+  uint32_t tstride;      // entries per row of tt
   uint32_t num_levels;
   uint32_t size[kMaxLevels];
   uint32_t off[kMaxLevels];
@@ -185,6 +187,25 @@ __device__ __forceinline__ void publish_zeros(uint32_t* zeros, uint32_t bits) {
   const uint64_t neg = __ballot((bits & kZeroNeg) != 0u);
   const uint32_t w = (pos ? kZeroPos : 0u) | (neg ? kZeroNeg : 0u);
   if (w && (threadIdx.x & 63u) == 0) atomicOr(zeros, w);
+}
+
+// Final ranges on the top level longer than this are answered from the sparse table.
+constexpr uint32_t kTopScan = 32;
+
+// Minimum of top-level entries [a, b), b - a >= 2, from the sparse table: the entry i of row
+// j >= 1 (at tt + (j - 1) * stride) holds {value bits, leftmost position} of the minimum of
+// [i, i + 2^j). With j = floor(log2(b - a)) the entries at a and b - 2^j cover [a, b); on equal
+// values the left one holds the leftmost position, since any position of that value in the
+// right entry's span left of it lies in the left entry's span too.
+__device__ __forceinline__ void top_lookup(const uint2* __restrict__ tt, uint32_t stride, uint32_t a,
+                                           uint32_t b, float& v, uint32_t& pos) {
+  const uint32_t j = 31u - static_cast<uint32_t>(__clz(b - a));
+  const uint2* row = tt + static_cast<size_t>(j - 1u) * stride;
+  const uint2 x = row[a];
+  const uint2 y = row[b - (1u << j)];
+  const bool right = u2f(y.x) < u2f(x.x);
+  v = u2f(right ? y.x : x.x);
+  pos = right ? y.y : x.y;
 }
 
 __device__ __forceinline__ bool in_range(uint32_t p, uint32_t lo, uint32_t hi) {

@@ -176,7 +176,8 @@ __device__ __forceinline__ void fold_piece(dev::Cand& b, float& bv, const float*
   }
 }
 
-template <int G, int RL, bool IDX, class LT>
+// TOP = false leaves the top level's sparse table out (for kernels short of registers).
+template <int G, int RL, bool IDX, bool TOP = true, class LT>
 __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, uint32_t gl,
                                                        const HierView& H, const LT& T) {
   constexpr uint32_t R = 1u << RL;
@@ -204,10 +205,13 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
   const float gL = left_tail ? vK[lk - 1] : RMQ_INF;
   const float gR = right_tail ? vK[ek] : RMQ_INF;
 
-  // final range: per lane in increasing position, strict `<` keeps the leftmost
+  // final range: per lane in increasing position, strict `<` keeps the leftmost; a long one on
+  // the top level is the sparse table's (one lane looks it up, the group's choice is uniform)
   float bv = RMQ_INF;
   uint32_t bpos = dev::kNoIdx;
-  for (uint32_t p = (lk & ~(STEP - 1)) + 4 * gl; p < ek; p += STEP) {
+  const bool table = TOP && K == top && H.tt != nullptr && ek - lk > dev::kTopScan;
+  if (table && gl == 0) dev::top_lookup(H.tt, H.tstride, lk, ek, bv, bpos);
+  for (uint32_t p = (lk & ~(STEP - 1)) + 4 * gl; !table && p < ek; p += STEP) {
     if (p + 4 > lk) {
       const float4 x = dev::load4(vK, p, szK);
       if constexpr (IDX) {
@@ -265,10 +269,10 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
 }
 
 // P selects the pruned evaluation order (default) or the full one.
-template <int G, int RL, bool P, bool IDX, class LT>
+template <int G, int RL, bool P, bool IDX, bool TOP = true, class LT>
 __device__ __forceinline__ GroupAnswer solve_group(uint32_t l, uint32_t e, uint32_t gl,
                                                    const HierView& H, const LT& T) {
-  if constexpr (P) return solve_cl_pruned<G, RL, IDX>(l, e, gl, H, T);
+  if constexpr (P) return solve_cl_pruned<G, RL, IDX, TOP>(l, e, gl, H, T);
   else return solve_cl<G, RL, IDX>(l, e, gl, H);
 }
 
@@ -276,11 +280,11 @@ __device__ __forceinline__ GroupAnswer solve_group(uint32_t l, uint32_t e, uint3
 // only differ from that when the minimum is zero (+0 and -0 compare equal): the zero's sign is
 // taken from the array's zero signs, and only when the array holds both is the query answered
 // again with index tracking.
-template <int G, int RL, bool P, class LT>
+template <int G, int RL, bool P, bool TOP = true, class LT>
 __device__ __forceinline__ float value_exact(uint32_t l, uint32_t e, uint32_t gl,
                                              const HierView& H, const LT& T, uint32_t zeros) {
-  GroupAnswer a = solve_group<G, RL, P, false>(l, e, gl, H, T);
-  if (a.v == 0.0f && !dev::zero_known(zeros, a.v)) a = solve_group<G, RL, P, true>(l, e, gl, H, T);
+  GroupAnswer a = solve_group<G, RL, P, false, TOP>(l, e, gl, H, T);
+  if (a.v == 0.0f && !dev::zero_known(zeros, a.v)) a = solve_group<G, RL, P, true, TOP>(l, e, gl, H, T);
   return a.v;
 }
 
@@ -454,7 +458,7 @@ __device__ __forceinline__ float4 load_level(const float* __restrict__ v, uint32
 // to left with <=) and a right element must be strictly smaller (folded left to right with <).
 // In the final range, folded left to right with <, the first entry seeds the answer, so a range
 // of +inf answers with its leftmost entry.
-template <int RL, bool IDX, class LT>
+template <int RL, bool IDX, bool TOP = true, class LT>
 __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
   constexpr uint32_t R = 1u << RL;
   constexpr uint32_t V = R / 4;
@@ -476,7 +480,14 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
   float bv = RMQ_INF;
   uint32_t bpay = (K == 0 ? dev::kDirect : T.off(K)) + lk;
   float gL = RMQ_INF, gR = RMQ_INF;
-  {
+  if (TOP && K == top && H.tt != nullptr && ek - lk > dev::kTopScan) {
+    // a long final range on the top level: the sparse table's leftmost minimum
+    uint32_t pos;
+    dev::top_lookup(H.tt, H.tstride, lk, ek, bv, pos);
+    bpay += pos - lk;
+    if (left_tail) gL = vK[lk - 1u];
+    if (right_tail) gR = vK[ek];
+  } else {
     // guards lk - 1 and ek: in the first / last vector the scan reads unless 4-aligned
     uint32_t p = lk & ~3u;
     float4 x = load_level(vK, p, szK, K);

@@ -104,11 +104,11 @@ __device__ __forceinline__ void unit_groups(const HierView& H, const uint2* __re
       float v;
       uint32_t i = dev::kNoIdx;
       if constexpr (IDX) {
-        const GroupAnswer a = solve_cl_pruned<G, kMultiRLog, true>(lk, ek, gl, H, LevelsOf{H});
+        const GroupAnswer a = solve_cl_pruned<G, kMultiRLog, true, false>(lk, ek, gl, H, LevelsOf{H});
         v = a.v;
         i = a.idx;
       } else {
-        v = value_exact<G, kMultiRLog, true>(lk, ek, gl, H, LevelsOf{H}, *H.zeros);
+        v = value_exact<G, kMultiRLog, true, false>(lk, ek, gl, H, LevelsOf{H}, *H.zeros);
       }
       if (gl == static_cast<uint32_t>(k)) {
         my_v = v;
@@ -130,9 +130,9 @@ __device__ __forceinline__ void unit_lanes(const HierView& H, const uint2* __res
   if (local >= count) return;
   const uint2 p = dev::load_query(Q, q0 + local);
   const LevelsOf T{H};
-  GroupAnswer a = solve_vl_pruned<RL, IDX>(p.x, p.y + 1, H, T);
+  GroupAnswer a = solve_vl_pruned<RL, IDX, false>(p.x, p.y + 1, H, T);
   if constexpr (!IDX) {
-    if (a.v == 0.0f && !dev::zero_known(*H.zeros, a.v)) a = solve_vl_pruned<RL, true>(p.x, p.y + 1, H, T);
+    if (a.v == 0.0f && !dev::zero_known(*H.zeros, a.v)) a = solve_vl_pruned<RL, true, false>(p.x, p.y + 1, H, T);
   }
   if (ov) ov[q0 + local] = a.v;
   if (IDX && oi) oi[q0 + local] = a.idx;
@@ -179,7 +179,9 @@ __device__ __forceinline__ void queue_shape(const uint32_t* __restrict__ qstart,
 }
 
 // Eight wavefronts per SIMD (64 VGPRs): the kernel carries every variant's solver, and the
-// occupancy is worth more than the few registers the compiler then spills (measured).
+// occupancy is worth more than the few registers the compiler then spills (measured). For the
+// same reason its solvers scan the top level instead of reading a sparse table over it: the
+// batched build makes none, and the autotuned thresholds keep top-level scans short.
 template <bool IDX>
 __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
     const HierView* __restrict__ views, const uint32_t* __restrict__ variant,

@@ -53,15 +53,17 @@ class Plan:
     R: int
 
 
-def _options(cg_size_log, cg_amount_log, group_lanes, full=False, chunk_log=0) -> Optional[BuildOptions]:
-    if cg_size_log is None and cg_amount_log is None and not group_lanes and not full and not chunk_log:
+def _options(cg_size_log, cg_amount_log, group_lanes, full=False, chunk_log=0,
+             top_table=True) -> Optional[BuildOptions]:
+    if (cg_size_log is None and cg_amount_log is None and not group_lanes and not full and not chunk_log
+            and top_table):
         return None
     o = BuildOptions()
     o.cg_size_log = 3 if cg_size_log is None else int(cg_size_log)
     o.cg_amount_log = 2 if cg_amount_log is None else int(cg_amount_log)
     o.group_lanes = int(group_lanes or 0)
     o.flags = (_lib.RMQ_BUILD_DEFAULT_CG | (_lib.RMQ_BUILD_FULL if full else 0) |
-               _lib.RMQ_BUILD_CHUNK_LOG(chunk_log))
+               (0 if top_table else _lib.RMQ_BUILD_NO_TOP_TABLE) | _lib.RMQ_BUILD_CHUNK_LOG(chunk_log))
     return o
 
 
@@ -70,7 +72,7 @@ def _opt_ptr(o: Optional[BuildOptions]):
 
 
 def plan(n: int, t: int, alg: int = RMQ_ALG_XXX, cg_size_log=None, cg_amount_log=None,
-         group_lanes: int = 0) -> Plan:
+         group_lanes: int = 0, full: bool = False) -> Plan:
     """Host-only level plan (src/xxx.h:225-243). Raises RMQError on invalid input."""
     L = _lib.lib()
     nl = ctypes.c_uint32()
@@ -78,7 +80,7 @@ def plan(n: int, t: int, alg: int = RMQ_ALG_XXX, cg_size_log=None, cg_amount_log
     offs = (ctypes.c_uint64 * _lib.RMQ_MAX_LEVELS)()
     bf = ctypes.c_uint64()
     R = ctypes.c_uint32()
-    o = _options(cg_size_log, cg_amount_log, group_lanes)
+    o = _options(cg_size_log, cg_amount_log, group_lanes, full)
     _lib.check(L.rmq_plan(n, t, alg, _opt_ptr(o), ctypes.byref(nl), sizes, offs, ctypes.byref(bf),
                           ctypes.byref(R)), "rmq_plan")
     k = nl.value
@@ -152,7 +154,7 @@ class Hierarchy:
 
     def __init__(self, array, t: int, alg: int = RMQ_ALG_XXX, cg_size_log=None,
                  cg_amount_log=None, group_lanes: int = 0, full: bool = False, chunk_log: int = 0,
-                 stream=None):
+                 top_table: bool = True, stream=None):
         torch = _torch()
         _check_device_tensor(array, "array", (torch.float32,))
         if array.dim() != 1:
@@ -162,7 +164,7 @@ class Hierarchy:
         self.t = int(t)
         self.alg = int(alg)
         self._h = ctypes.c_void_p()
-        o = _options(cg_size_log, cg_amount_log, group_lanes, full, chunk_log)
+        o = _options(cg_size_log, cg_amount_log, group_lanes, full, chunk_log, top_table)
         st = _lib.lib().rmq_build_ex(array.data_ptr(), self.n, self.t, self.alg, _opt_ptr(o),
                                      _stream_handle(stream), ctypes.byref(self._h))
         _lib.check(st, "rmq_build")

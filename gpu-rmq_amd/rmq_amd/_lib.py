@@ -26,6 +26,7 @@ RMQ_ALG_XXX_MULTI_LOAD = 24
 RMQ_MAX_LEVELS = 20
 RMQ_BUILD_DEFAULT_CG = 0x1
 RMQ_BUILD_FULL = 0x2
+RMQ_BUILD_NO_TOP_TABLE = 0x4
 
 
 def RMQ_BUILD_CHUNK_LOG(k: int) -> int:

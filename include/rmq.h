@@ -63,18 +63,25 @@ typedef struct rmq_hierarchy rmq_hierarchy;
  *   algs 16/24: R = 2^(cg_size_log + cg_amount_log)      (src/xxx.h:208-209)
  *   alg 20:     R = 2^(cg_amount_log + 2)                 (src/xxx.h:934-935)
  * group_lanes is this library's own execution choice (lanes that cooperate on one
- * query in algs 16/24; 0 = auto; 64 = a whole wavefront per query, the long-scan form for
- * large t). It never changes results. */
+ * query in algs 16/24; 0 = auto; 1 = a lane per query with 16-byte loads, the solver of alg 20,
+ * for R <= 64 and not with RMQ_BUILD_FULL; 64 = a whole wavefront per query, the long-scan form
+ * for large t). It never changes results. */
 typedef struct {
   uint32_t cg_size_log;     /* 0 -> 3 */
   uint32_t cg_amount_log;   /* 0 -> 2 */
-  uint32_t group_lanes;     /* 0 -> auto; else 4, 8, 16, 32 or 64 */
+  uint32_t group_lanes;     /* 0 -> auto; else 1, 4, 8, 16, 32 or 64 */
   uint32_t flags;           /* RMQ_BUILD_* below */
 } rmq_build_options;
 
 #define RMQ_BUILD_DEFAULT_CG 0x1u   /* use cg_size_log = cg_amount_log = 0 literally */
 #define RMQ_BUILD_FULL 0x2u         /* read every piece of the traversal in the reference's
                                        order instead of the pruned order (same answers) */
+/* A build also keeps a sparse table over the top level (m entries): floor(log2 m) rows of m
+ * 8-byte {value, position} entries, built when m >= 64 and the table is at most 64 MiB, and not
+ * with RMQ_BUILD_FULL. A query whose remaining range on the top level is longer than 32 entries
+ * reads two table entries instead of scanning it, so large t no longer costs long scans (the
+ * reference default t = 2^15 with uniform ranges). The flag below builds none (same answers). */
+#define RMQ_BUILD_NO_TOP_TABLE 0x4u
 /* Chunk size (log2, 8..30) of the 64-bit form for this build; arrays of more than 2^31 - 1
  * elements use 2^30 without it. Setting it holds even a small array in chunks, which is how the
  * 64-bit form is tested on small inputs. */

@@ -120,8 +120,13 @@ def test_invalid_arguments():
     with pytest.raises(rmq_amd.RMQError) as e:
         rmq_amd.plan(100, 64, 16, group_lanes=3)
     assert e.value.status == rmq_amd.RMQ_EUNSUPPORTED
-    for g in (4, 8, 16, 32, 64):  # up to a whole wavefront per query
+    for g in (1, 4, 8, 16, 32, 64):  # a lane per query up to a whole wavefront per query
         rmq_amd.plan(100, 64, 16, group_lanes=g)
+    # a lane per query runs the pruned order only, on R <= 64
+    for kw in (dict(full=True), dict(cg_size_log=4, cg_amount_log=3)):
+        with pytest.raises(rmq_amd.RMQError) as e:
+            rmq_amd.plan(100, 256, 16, group_lanes=1, **kw)
+        assert e.value.status == rmq_amd.RMQ_EUNSUPPORTED
     with pytest.raises(rmq_amd.RMQError) as e:
         rmq_amd.plan(100, 64, 16, group_lanes=128)
     assert e.value.status == rmq_amd.RMQ_EUNSUPPORTED

@@ -30,6 +30,8 @@ VARIANTS = [
     (16, dict(group_lanes=16)),
     (24, dict(group_lanes=16)),
     (16, dict(group_lanes=64)),                              # a wave per query
+    (16, dict(group_lanes=1)),                               # a lane per query, R=32
+    (24, dict(group_lanes=1, cg_size_log=3, cg_amount_log=3)),  # a lane per query, R=64
     (24, dict(group_lanes=32)),
     (24, dict(group_lanes=64, cg_size_log=4, cg_amount_log=3)),  # a wave per query, R=128
     (16, dict(cg_size_log=3, cg_amount_log=0)),              # R=8
@@ -131,12 +133,39 @@ def test_every_depth(device, t):
     lr = rmq_amd.gen_queries(1 << 16, -6, n, 77)
     ref_v, ref_i = oracle.CpuHierarchy(A, 64, 5, 16).query(lr)
     for alg, opts in ((16, {}), (24, {}), (16, dict(full=True)), (16, dict(group_lanes=4)),
-                      (24, dict(group_lanes=16)), (16, dict(group_lanes=64)), (20, {})):
+                      (24, dict(group_lanes=16)), (16, dict(group_lanes=64)), (16, dict(group_lanes=1)),
+                      (20, {})):
         _say(f"depth t={t} alg={alg} opts={opts}")
         v, i = run_gpu(A, lr, alg, t, True, True, **opts)
         assert_exact(A, lr, v, i, ref_i, f"depth t={t} alg={alg} {opts}")
         v, _ = run_gpu(A, lr, alg, t, True, False, **opts)
         assert_exact(A, lr, v, None, ref_i, f"depth t={t} alg={alg} {opts} value")
+
+
+@pytest.mark.parametrize("alg,opts", [(16, {}), (16, dict(group_lanes=1)), (24, dict(group_lanes=4)),
+                                      (16, dict(group_lanes=64)), (20, dict(cg_amount_log=0)),
+                                      (20, dict(cg_amount_log=1))], ids=str)
+def test_top_table(device, alg, opts):
+    """Large t leaves long final ranges on the top level, which the sparse table over it answers:
+    the reference default t = 2^15 with uniform lengths, a one-level hierarchy (t >= n, the table
+    over the array itself) and a small top level near the 32-entry scan limit. Values are
+    quantised (many ties) with both zero signs planted; answers are the same with and without
+    the table and equal to the oracle's."""
+    rng = np.random.default_rng(alg * 7 + len(opts))
+    R = _R(alg, opts)
+    for n, t in ((1 << 20, 1 << 15), ((1 << 18) + 3, 1 << 19), (100 * R * R + 17, 128)):
+        A = np.floor(rmq_amd.gen_array(n, 11) * 512.0).astype(np.float32)
+        A[rng.integers(0, n, size=64)] = np.float32(-0.0)  # next to the +0 of A < 1/512
+        lr = np.concatenate([rmq_amd.gen_queries(1 << 14, -1, n, 5), rmq_amd.gen_queries(1 << 12, -6, n, 6),
+                             edge_queries(n, R)])
+        ref_v, ref_i = oracle.CpuHierarchy(A, 64, 5, 16).query(lr)
+        levels = rmq_amd.plan(n, t, alg, **{k: v for k, v in opts.items() if k != "group_lanes"})
+        _say(f"top table alg={alg} {opts} n={n} t={t} top={levels.sizes[-1]}")
+        for table in (True, False):
+            v, i = run_gpu(A, lr, alg, t, True, True, top_table=table, **opts)
+            assert_exact(A, lr, v, i, ref_i, f"alg={alg} {opts} n={n} t={t} table={table}")
+            v, _ = run_gpu(A, lr, alg, t, True, False, top_table=table, **opts)
+            assert_exact(A, lr, v, None, ref_i, f"alg={alg} {opts} n={n} t={t} table={table} value")
 
 
 def test_tie_vectors(device):
