@@ -42,8 +42,12 @@ __global__ __launch_bounds__(kBlock) void k_gpu_base(const float* __restrict__ A
   if (out_i) out_i[qi] = mi;
 }
 
-// One word set from the stream (a kernel, not a memset, so that a captured build stays valid).
-__global__ void k_set_word(uint32_t* __restrict__ w, uint32_t v) { *w = v; }
+// The hierarchy's small words, set from the stream (a kernel, not a memset, so that a captured
+// build stays valid): w[0] = the zero-sign word, w[4..8) = the copy of level 0's last vector.
+__global__ void k_init_words(uint32_t* __restrict__ w, uint32_t zeros, const float* __restrict__ A, uint32_t n) {
+  w[0] = zeros;
+  dev::fill_tail(A, n, reinterpret_cast<float*>(w + 4));
+}
 
 // Row j >= 1 of the sparse table over the top level top[0, m): entry i (i + 2^j <= m) is the
 // leftmost minimum of [i, i + 2^j), from two entries of row j - 1 (row 1 from the level itself).
@@ -457,7 +461,7 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   h->n = n;
   h->t = t;
   h->buffer_floats = P.buffer_floats;
-  if (hipMalloc(reinterpret_cast<void**>(&h->zeros), sizeof(uint32_t)) != hipSuccess) {
+  if (hipMalloc(reinterpret_cast<void**>(&h->zeros), 8 * sizeof(uint32_t)) != hipSuccess) {
     (void)hipGetLastError();
     delete h;
     return RMQ_ENOMEM;
@@ -486,6 +490,7 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   H.lv = h->lv;
   H.li = h->li;
   H.zeros = h->zeros;
+  H.tail = reinterpret_cast<const float*>(h->zeros + 4);
   H.num_levels = P.num_levels;
   for (uint32_t k = 0; k < rmq::kMaxLevels; ++k) {
     H.size[k] = k < P.num_levels ? static_cast<uint32_t>(P.size[k]) : 0;
@@ -493,7 +498,8 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   }
   const hipStream_t s = static_cast<hipStream_t>(stream);
   // the level-0 pass records the zero signs; with no such pass both are assumed
-  rmq::k_set_word<<<1, 1, 0, s>>>(h->zeros, P.num_levels > 1 ? 0u : (rmq::kZeroPos | rmq::kZeroNeg));
+  rmq::k_init_words<<<1, 1, 0, s>>>(h->zeros, P.num_levels > 1 ? 0u : (rmq::kZeroPos | rmq::kZeroNeg), d_array,
+                                    static_cast<uint32_t>(n));
   if (last_error() != RMQ_OK) {
     rmq_free(h);
     return RMQ_EDEVICE;

@@ -43,6 +43,7 @@ struct rmq_batch {
   float* lv;
   uint32_t* li;
   uint32_t* zeros;  // per array: kZeroPos | kZeroNeg of its level 0
+  float4* tails;    // per array: the copy of level 0's last 16-byte vector, +inf padded
   uint64_t level_floats;
   std::vector<uint32_t> variant;  // host copy
 };
@@ -114,11 +115,15 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
   if (k == 0) dev::publish_zeros(const_cast<uint32_t*>(H.zeros), zb);
 }
 
-// Every array's zero-sign word: cleared for the level-0 pass, or both signs when there is none.
+// Every array's zero-sign word (cleared for the level-0 pass, or both signs when there is none)
+// and its copy of level 0's last vector.
 __global__ void k_init_zeros(const HierView* __restrict__ views, uint32_t count) {
   const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < count)
-    *const_cast<uint32_t*>(views[a].zeros) = views[a].num_levels > 1 ? 0u : (kZeroPos | kZeroNeg);
+  if (a < count) {
+    const HierView& H = views[a];
+    *const_cast<uint32_t*>(H.zeros) = H.num_levels > 1 ? 0u : (kZeroPos | kZeroNeg);
+    dev::fill_tail(H.A, H.size[0], const_cast<float*>(H.tail));
+  }
 }
 
 }  // namespace rmq
@@ -195,6 +200,7 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
   rmq_status st = dmalloc(&b->lv, arena);
   if (st == RMQ_OK) st = dmalloc(&b->li, arena);
   if (st == RMQ_OK) st = dmalloc(&b->zeros, count);
+  if (st == RMQ_OK) st = dmalloc(&b->tails, count);
   if (st == RMQ_OK) st = dmalloc(&b->d_views, count);
   if (st == RMQ_OK) st = dmalloc(&b->d_variant, count);
   if (st != RMQ_OK) {
@@ -208,6 +214,7 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     H.lv = b->lv + lv_off[a];
     H.li = b->li + lv_off[a];
     H.zeros = b->zeros + a;
+    H.tail = reinterpret_cast<const float*>(b->tails + a);
     b->variant[a] = arrays[a].variant;
   }
 
@@ -359,6 +366,7 @@ void rmq_batch_free(rmq_batch* b) {
   if (!b) return;
   if (b->lv) (void)hipFree(b->lv);
   if (b->zeros) (void)hipFree(b->zeros);
+  if (b->tails) (void)hipFree(b->tails);
   if (b->li) (void)hipFree(b->li);
   if (b->d_views) (void)hipFree(b->d_views);
   if (b->d_variant) (void)hipFree(b->d_variant);

@@ -20,6 +20,7 @@ struct HierView {
   const float* lv;       // level buffer, values  (levels 1..num_levels-1)
   const uint32_t* li;    // level buffer, leftmost level-0 index of each value
   const uint32_t* zeros; // which signs of zero level 0 holds (kZeroPos | kZeroNeg), see below
+  const float* tail;     // level 0's last 16-byte vector A[n & ~3, n), +inf padded, 16-byte aligned
   const uint2* tt;       // sparse table over the top level (dev::top_lookup), or null
   uint32_t tstride;      // entries per row of tt
   uint32_t num_levels;
@@ -162,8 +163,18 @@ __device__ __forceinline__ uint4 load4u(const uint32_t* __restrict__ base, uint3
 }
 
 // Element j (0..3) of a vector.
-__device__ __forceinline__ float pick4(float4 x, uint32_t j) {
-  return j == 0 ? x.x : (j == 1 ? x.y : (j == 2 ? x.z : x.w));
+__device__ __forceinline__ float pick4(float4 x, uint32_t j) {  // two selects, no branch
+  const float lo = (j & 1u) ? x.y : x.x;
+  const float hi = (j & 1u) ? x.w : x.z;
+  return (j & 2u) ? hi : lo;
+}
+
+// The copy of level 0's last vector kept with a hierarchy: tail[j] = A[(n & ~3) + j] below n,
+// +inf past it (one lane writes all four).
+__device__ __forceinline__ void fill_tail(const float* __restrict__ A, uint32_t n, float* __restrict__ tail) {
+  const uint32_t p = n & ~3u;
+#pragma unroll
+  for (uint32_t j = 0; j < 4; ++j) tail[j] = p + j < n ? A[p + j] : RMQ_INF;
 }
 
 // The sign bits of the zeros among x (kZeroPos / kZeroNeg), if `ok`.

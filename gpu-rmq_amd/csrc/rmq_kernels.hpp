@@ -91,16 +91,24 @@ __device__ __forceinline__ const float* level_values(const HierView& H, uint32_t
 
 // Where a solver finds a level's offset and size (one lookup per level it visits): in the
 // HierView itself, or in a copy the kernel placed in LDS.
+// offz(k) is the offset with level 0 read as 0 (it lives in the caller's array).
 struct LevelsOf {
   const HierView& H;
   __device__ __forceinline__ uint32_t off(uint32_t k) const { return H.off[k]; }
+  __device__ __forceinline__ uint32_t offz(uint32_t k) const { return k == 0 ? 0u : H.off[k]; }
   __device__ __forceinline__ uint32_t size(uint32_t k) const { return H.size[k]; }
 };
 struct LevelsLds {
-  const uint32_t* meta;  // LDS: off[kMaxLevels], then size[kMaxLevels]
+  const uint32_t* meta;  // LDS: off[kMaxLevels] with off[0] = 0, then size[kMaxLevels] (lds_meta)
   __device__ __forceinline__ uint32_t off(uint32_t k) const { return meta[k]; }
+  __device__ __forceinline__ uint32_t offz(uint32_t k) const { return meta[k]; }
   __device__ __forceinline__ uint32_t size(uint32_t k) const { return meta[kMaxLevels + k]; }
 };
+
+// Entry i (< 2 * kMaxLevels) of the LDS copy that LevelsLds reads.
+__device__ __forceinline__ uint32_t lds_meta(const HierView& H, uint32_t i) {
+  return i < kMaxLevels ? (i == 0 ? 0u : H.off[i]) : H.size[i - kMaxLevels];
+}
 
 template <int G, int RL, bool IDX>
 __device__ __forceinline__ GroupAnswer solve_cl(uint32_t l, uint32_t e, uint32_t gl,
@@ -195,7 +203,7 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
     lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
     ek = e >> shK;
   }
-  const uint32_t offK = K == 0 ? 0u : T.off(K);
+  const uint32_t offK = T.offz(K);
   const float* vK = K == 0 ? H.A : H.lv + offK;
   const uint32_t szK = T.size(K);
   const uint32_t payK = K == 0 ? dev::kDirect : offK;
@@ -247,7 +255,7 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
   for (int k = static_cast<int>(K) - 1; k >= 0 && (la || ra); --k) {
     const uint32_t uk = static_cast<uint32_t>(k);
     const uint32_t sk = RL * uk;
-    const uint32_t ok = k == 0 ? 0u : T.off(uk);
+    const uint32_t ok = T.offz(uk);
     const float* vk = k == 0 ? H.A : H.lv + ok;
     const uint32_t szk = T.size(uk);
     const uint32_t pay = k == 0 ? dev::kDirect : ok;
@@ -298,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ 
                                                       uint32_t* __restrict__ out_i, HierView H) {
   __shared__ uint32_t s_meta[2 * kMaxLevels];
   if (threadIdx.x < 2 * kMaxLevels)
-    s_meta[threadIdx.x] = threadIdx.x < kMaxLevels ? H.off[threadIdx.x] : H.size[threadIdx.x - kMaxLevels];
+    s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
   const LevelsLds T{s_meta};
   const uint32_t lane = threadIdx.x & 63;
@@ -352,7 +360,7 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
                                                         HierView H) {
   __shared__ uint32_t s_meta[2 * kMaxLevels];
   if (threadIdx.x < 2 * kMaxLevels)
-    s_meta[threadIdx.x] = threadIdx.x < kMaxLevels ? H.off[threadIdx.x] : H.size[threadIdx.x - kMaxLevels];
+    s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
   const LevelsLds T{s_meta};
   const uint32_t gl = threadIdx.x & (G - 1);
@@ -441,12 +449,14 @@ __device__ __forceinline__ GroupAnswer solve_vl(uint32_t l, uint32_t e, const Hi
   else return GroupAnswer{bv, dev::kNoIdx};
 }
 
-// A 16-byte vector of level k at p (p % 4 == 0, p < size). Upper levels sit in the level buffer,
-// padded to 32 entries past each level's end, so their vectors need no bounds test; level 0 is
-// the caller's array of exactly n floats.
-__device__ __forceinline__ float4 load_level(const float* __restrict__ v, uint32_t p, uint32_t size, uint32_t k) {
-  if (k != 0) return *reinterpret_cast<const float4*>(v + p);
-  return dev::load4(v, p, size);
+// A 16-byte vector of level k at p (p % 4 == 0, p < size), with no branch: upper levels sit in
+// the level buffer, padded to 32 entries past each level's end, so their vectors are whole; on
+// level 0 (the caller's array of exactly n floats) the one vector that would run past n, at
+// p = n & ~3, is read from the hierarchy's copy of it, padded with +inf.
+__device__ __forceinline__ float4 load_level(const float* __restrict__ v, uint32_t p, uint32_t size, uint32_t k,
+                                             const float* __restrict__ tail) {
+  const float* src = (k != 0 || p + 4u <= size) ? v + p : tail;
+  return *reinterpret_cast<const float4*>(src);
 }
 
 // Lane-per-query pruned evaluation (the pieces of alg 20, read in the order of
@@ -458,10 +468,20 @@ __device__ __forceinline__ float4 load_level(const float* __restrict__ v, uint32
 // to left with <=) and a right element must be strictly smaller (folded left to right with <).
 // In the final range, folded left to right with <, the first entry seeds the answer, so a range
 // of +inf answers with its leftmost entry.
-template <int RL, bool IDX, bool TOP = true, class LT>
-__device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
-  constexpr uint32_t R = 1u << RL;
-  constexpr uint32_t V = R / 4;
+//
+// The solver in three resumable parts, so that a kernel can hold a query between levels:
+// vl_start reads the final range and its guards, vl_step walks both tails down one level,
+// vl_finish resolves the winner's index. A query needs vl_step while la or ra holds.
+struct VlState {
+  uint32_t l, e;
+  float bv;
+  uint32_t bpay;  // winner: level-buffer slot, or kDirect | level-0 position
+  int k;          // the level the next step reads (K - 1 down to 0)
+  bool la, ra;    // a tail's next piece can still win
+};
+
+template <int RL, bool IDX, bool TOP, class LT>
+__device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
   const uint32_t top = H.num_levels - 1;
   uint32_t K = min((31u - static_cast<uint32_t>(__clz(e - l))) / RL, top);
   uint32_t shK = RL * K;
@@ -473,12 +493,12 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
     lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
     ek = e >> shK;
   }
-  const float* vK = K == 0 ? H.A : H.lv + T.off(K);
+  const float* vK = K == 0 ? H.A : H.lv + T.offz(K);
   const uint32_t szK = T.size(K);
   const bool left_tail = l < (lk << shK);
   const bool right_tail = (ek << shK) < e;
   float bv = RMQ_INF;
-  uint32_t bpay = (K == 0 ? dev::kDirect : T.off(K)) + lk;
+  uint32_t bpay = (K == 0 ? dev::kDirect : T.offz(K)) + lk;
   float gL = RMQ_INF, gR = RMQ_INF;
   if (TOP && K == top && H.tt != nullptr && ek - lk > dev::kTopScan) {
     // a long final range on the top level: the sparse table's leftmost minimum
@@ -490,7 +510,7 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
   } else {
     // guards lk - 1 and ek: in the first / last vector the scan reads unless 4-aligned
     uint32_t p = lk & ~3u;
-    float4 x = load_level(vK, p, szK, K);
+    float4 x = load_level(vK, p, szK, K, H.tail);
     if (left_tail) gL = (lk & 3u) ? dev::pick4(x, (lk - 1u) & 3u) : vK[lk - 1u];
     const uint32_t pay0 = bpay - lk;
     for (;;) {
@@ -507,91 +527,115 @@ __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, c
       }
       p += 4;
       if (p >= ek) break;
-      x = load_level(vK, p, szK, K);
+      x = load_level(vK, p, szK, K, H.tail);
     }
     if (right_tail) gR = (ek & 3u) ? dev::pick4(x, ek & 3u) : vK[ek];
   }
-  // Both tails walk down together, one level per step, so that a step's left and right loads
-  // are in flight at once. Any interleaving is exact: a left element wins ties whenever it is
-  // folded, a right one never does, and each guard test compares with the best so far.
-  bool la = left_tail && gL <= bv;
-  bool ra = right_tail && gR < bv;
-  for (int k = static_cast<int>(K) - 1; k >= 0 && (la || ra); --k) {
-    const uint32_t uk = static_cast<uint32_t>(k);
-    const uint32_t sk = RL * uk;
-    const uint32_t ok = k == 0 ? 0u : T.off(uk);
-    const float* vk = k == 0 ? H.A : H.lv + ok;
-    const uint32_t szk = T.size(uk);
-    const uint32_t pay0 = k == 0 ? dev::kDirect : ok;
-    const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);  // left piece [lo, roundup_R(lo))
-    const uint32_t lbase = lo & ~(R - 1u);
-    const uint32_t hi = e >> sk;                                       // right piece [rlo, hi)
-    const uint32_t rlo = hi & ~(R - 1u);
-    // R <= 8: both pieces' vectors are loaded before either is folded; wider blocks load per
-    // side (holding 2 x R/4 vectors would cost the occupancy the overlap is meant to buy)
-    constexpr bool kEarly = V <= 2;
-    constexpr uint32_t VE = kEarly ? V : 1;
-    float4 xl[VE], xr[VE];
-    if constexpr (kEarly) {
+  return VlState{l, e, bv, bpay, static_cast<int>(K) - 1, left_tail && gL <= bv, right_tail && gR < bv};
+}
+
+// One level of both tails. Any interleaving of the two sides is exact: a left element wins ties
+// whenever it is folded, a right one never does, and each guard test compares with the best so
+// far; both sides' loads of a level are in flight together.
+template <int RL, bool IDX, class LT>
+__device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT& T) {
+  constexpr uint32_t R = 1u << RL;
+  constexpr uint32_t V = R / 4;
+  const uint32_t l = st.l, e = st.e;
+  float bv = st.bv;
+  uint32_t bpay = st.bpay;
+  bool la = st.la, ra = st.ra;
+  const int k = st.k;
+  const uint32_t uk = static_cast<uint32_t>(k);
+  const uint32_t sk = RL * uk;
+  const uint32_t ok = T.offz(uk);
+  const float* vk = k == 0 ? H.A : H.lv + ok;
+  const uint32_t szk = T.size(uk);
+  const uint32_t pay0 = k == 0 ? dev::kDirect : ok;
+  const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);  // left piece [lo, roundup_R(lo))
+  const uint32_t lbase = lo & ~(R - 1u);
+  const uint32_t hi = e >> sk;                                       // right piece [rlo, hi)
+  const uint32_t rlo = hi & ~(R - 1u);
+  // R <= 8: both pieces' vectors are loaded before either is folded; wider blocks load per
+  // side (holding 2 x R/4 vectors would cost the occupancy the overlap is meant to buy)
+  constexpr bool kEarly = V <= 2;
+  constexpr uint32_t VE = kEarly ? V : 1;
+  float4 xl[VE], xr[VE];
+  if constexpr (kEarly) {
 #pragma unroll
-      for (uint32_t s = 0; s < V; ++s) {
-        const uint32_t pl = lbase + 4u * s, pr = rlo + 4u * s;
-        xl[s] = la && (lo & (R - 1u)) != 0u && pl + 4u > lo ? load_level(vk, pl, szk, uk)
-                                                          : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
-        xr[s] = ra && pr < hi ? load_level(vk, pr, szk, uk) : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
-      }
-    }
-    if (la) {
-      float g = RMQ_INF;
-#pragma unroll
-      for (int s = static_cast<int>(V) - 1; s >= 0; --s) {
-        const uint32_t p = lbase + 4u * static_cast<uint32_t>(s);
-        if ((lo & (R - 1u)) != 0u && p + 4u > lo) {
-          const float4 x = kEarly ? xl[kEarly ? s : 0] : load_level(vk, p, szk, uk);
-          if (p <= lo - 1u) g = dev::pick4(x, (lo - 1u) & 3u);  // next guard, lo - 1
-          const float xs[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-          for (int j = 3; j >= 0; --j) {
-            const bool t = p + static_cast<uint32_t>(j) >= lo && xs[j] <= bv;
-            bv = t ? xs[j] : bv;
-            if constexpr (IDX) bpay = t ? pay0 + p + static_cast<uint32_t>(j) : bpay;
-          }
-        }
-      }
-      if (k == 0 || !(l < (lo << sk))) {
-        la = false;
-      } else {
-        if ((lo & 3u) == 0) g = vk[lo - 1u];
-        la = !(g > bv);
-      }
-    }
-    if (ra) {
-      float g = RMQ_INF;
-#pragma unroll
-      for (uint32_t s = 0; s < V; ++s) {
-        const uint32_t p = rlo + 4u * s;
-        if (p < hi) {
-          const float4 x = kEarly ? xr[kEarly ? s : 0] : load_level(vk, p, szk, uk);
-          if (hi - p < 4u) g = dev::pick4(x, hi & 3u);  // next guard, hi
-          const float xs[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-          for (uint32_t j = 0; j < 4; ++j) {
-            const bool t = p + j < hi && xs[j] < bv;
-            bv = t ? xs[j] : bv;
-            if constexpr (IDX) bpay = t ? pay0 + p + j : bpay;
-          }
-        }
-      }
-      if (k == 0 || !((hi << sk) < e)) {
-        ra = false;
-      } else {
-        if ((hi & 3u) == 0) g = vk[hi];
-        ra = g < bv;
-      }
+    for (uint32_t s = 0; s < V; ++s) {
+      const uint32_t pl = lbase + 4u * s, pr = rlo + 4u * s;
+      xl[s] = la && (lo & (R - 1u)) != 0u && pl + 4u > lo ? load_level(vk, pl, szk, uk, H.tail)
+                                                        : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+      xr[s] = ra && pr < hi ? load_level(vk, pr, szk, uk, H.tail) : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
     }
   }
-  if constexpr (IDX) return GroupAnswer{bv, (bpay & dev::kDirect) ? (bpay & ~dev::kDirect) : H.li[bpay]};
-  else return GroupAnswer{bv, dev::kNoIdx};
+  if (la) {
+    float g = RMQ_INF;
+#pragma unroll
+    for (int s = static_cast<int>(V) - 1; s >= 0; --s) {
+      const uint32_t p = lbase + 4u * static_cast<uint32_t>(s);
+      if ((lo & (R - 1u)) != 0u && p + 4u > lo) {
+        const float4 x = kEarly ? xl[kEarly ? s : 0] : load_level(vk, p, szk, uk, H.tail);
+        if (p <= lo - 1u) g = dev::pick4(x, (lo - 1u) & 3u);  // next guard, lo - 1
+        const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int j = 3; j >= 0; --j) {
+          const bool t = p + static_cast<uint32_t>(j) >= lo && xs[j] <= bv;
+          bv = t ? xs[j] : bv;
+          if constexpr (IDX) bpay = t ? pay0 + p + static_cast<uint32_t>(j) : bpay;
+        }
+      }
+    }
+    if (k == 0 || !(l < (lo << sk))) {
+      la = false;
+    } else {
+      if ((lo & 3u) == 0) g = vk[lo - 1u];
+      la = !(g > bv);
+    }
+  }
+  if (ra) {
+    float g = RMQ_INF;
+#pragma unroll
+    for (uint32_t s = 0; s < V; ++s) {
+      const uint32_t p = rlo + 4u * s;
+      if (p < hi) {
+        const float4 x = kEarly ? xr[kEarly ? s : 0] : load_level(vk, p, szk, uk, H.tail);
+        if (hi - p < 4u) g = dev::pick4(x, hi & 3u);  // next guard, hi
+        const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const bool t = p + j < hi && xs[j] < bv;
+          bv = t ? xs[j] : bv;
+          if constexpr (IDX) bpay = t ? pay0 + p + j : bpay;
+        }
+      }
+    }
+    if (k == 0 || !((hi << sk) < e)) {
+      ra = false;
+    } else {
+      if ((hi & 3u) == 0) g = vk[hi];
+      ra = g < bv;
+    }
+  }
+  st.bv = bv;
+  st.bpay = bpay;
+  st.la = la;
+  st.ra = ra;
+  st.k = k - 1;
+}
+
+template <bool IDX>
+__device__ __forceinline__ GroupAnswer vl_finish(const VlState& st, const HierView& H) {
+  if constexpr (IDX) return GroupAnswer{st.bv, (st.bpay & dev::kDirect) ? (st.bpay & ~dev::kDirect) : H.li[st.bpay]};
+  else return GroupAnswer{st.bv, dev::kNoIdx};
+}
+
+template <int RL, bool IDX, bool TOP = true, class LT>
+__device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
+  VlState st = vl_start<RL, IDX, TOP>(l, e, H, T);
+  while (st.la || st.ra) vl_step<RL, IDX>(st, H, T);
+  return vl_finish<IDX>(st, H);
 }
 
 template <int RL, bool P, bool IDX, class LT>
@@ -607,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q
   // level offsets and sizes in LDS: the solver looks one up per level it visits
   __shared__ uint32_t s_meta[2 * kMaxLevels];
   if (threadIdx.x < 2 * kMaxLevels)
-    s_meta[threadIdx.x] = threadIdx.x < kMaxLevels ? H.off[threadIdx.x] : H.size[threadIdx.x - kMaxLevels];
+    s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
   const LevelsLds T{s_meta};
   const uint32_t zeros = IDX ? 0u : *H.zeros;
@@ -621,6 +665,5 @@ __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q
   if (out_v) out_v[qi] = a.v;
   if (IDX && out_i) out_i[qi] = a.idx;
 }
-
 
 }  // namespace rmq
