@@ -81,7 +81,8 @@ struct rmq_hierarchy {
   uint64_t t;
   uint64_t buffer_floats;
   float* lv;
-  uint32_t* li;
+  uint8_t* lr;      // offset entries of levels 1.. (HierView::lr)
+  uint64_t rel_bytes = 0;
   uint32_t* zeros;  // device word: kZeroPos | kZeroNeg of level 0
   uint2* tt = nullptr;  // sparse table over the top level (null when not built)
   HierView view;
@@ -189,30 +190,45 @@ rmq_status last_error() { return hipGetLastError() == hipSuccess ? RMQ_OK : RMQ_
 
 // ---- build dispatch -----------------------------------------------------------------------
 template <int RL>
-void launch_build_level(const float* sv, const uint32_t* si, uint32_t ssize, float* dv,
-                        uint32_t* di, uint32_t dsize, uint32_t* zeros, hipStream_t s) {
+void launch_build_level(const float* sv, const uint8_t* srel, uint32_t ssk, uint32_t ssize, float* dv,
+                        uint8_t* drel, uint32_t dsk, uint32_t dsize, uint32_t* zeros, hipStream_t s) {
   constexpr uint32_t V = (1u << RL) / 4;
   constexpr uint32_t W = V < 64 ? V : 64;
   const uint32_t grid = grid_for(static_cast<uint64_t>(dsize) * W);
-  if (si) {
-    rmq::k_build_level<RL, true><<<grid, kBlock, 0, s>>>(sv, si, ssize, dv, di, dsize, zeros);
+  if (srel) {
+    rmq::k_build_level<RL, true><<<grid, kBlock, 0, s>>>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros);
   } else {
-    rmq::k_build_level<RL, false><<<grid, kBlock, 0, s>>>(sv, si, ssize, dv, di, dsize, zeros);
+    rmq::k_build_level<RL, false><<<grid, kBlock, 0, s>>>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros);
   }
 }
 
-rmq_status build_level(uint32_t R_log, const float* sv, const uint32_t* si, uint32_t ssize,
-                       float* dv, uint32_t* di, uint32_t dsize, uint32_t* zeros, hipStream_t s) {
+// Level k -> k + 1 of a hierarchy: values into dv, offset entries into drel; srel is null on
+// level 0 (positions are indices there).
+rmq_status build_level(uint32_t R_log, uint32_t k, const float* sv, const uint8_t* srel, uint32_t ssize,
+                       float* dv, uint8_t* drel, uint32_t dsize, uint32_t* zeros, hipStream_t s) {
+  const uint32_t ssk = R_log * k, dsk = R_log * (k + 1);
   switch (R_log) {
-    case 2: launch_build_level<2>(sv, si, ssize, dv, di, dsize, zeros, s); break;
-    case 3: launch_build_level<3>(sv, si, ssize, dv, di, dsize, zeros, s); break;
-    case 4: launch_build_level<4>(sv, si, ssize, dv, di, dsize, zeros, s); break;
-    case 5: launch_build_level<5>(sv, si, ssize, dv, di, dsize, zeros, s); break;
-    case 6: launch_build_level<6>(sv, si, ssize, dv, di, dsize, zeros, s); break;
-    case 7: launch_build_level<7>(sv, si, ssize, dv, di, dsize, zeros, s); break;
+    case 2: launch_build_level<2>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
+    case 3: launch_build_level<3>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
+    case 4: launch_build_level<4>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
+    case 5: launch_build_level<5>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
+    case 6: launch_build_level<6>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
+    case 7: launch_build_level<7>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
     default: return RMQ_EUNSUPPORTED;
   }
   return last_error();
+}
+
+// Byte offsets of the levels' offset entries (rel_bytes per level, 16-byte aligned, four entries
+// of padding each); returns the total bytes.
+uint64_t rel_layout(uint32_t R_log, uint32_t num_levels, const uint64_t* size, uint32_t* roff) {
+  uint64_t b = 0;
+  for (uint32_t k = 0; k < rmq::kMaxLevels; ++k) roff[k] = 0;
+  for (uint32_t k = 1; k < num_levels; ++k) {
+    roff[k] = static_cast<uint32_t>(b);
+    b += ((size[k] + 4) * rmq::rel_bytes(R_log * k) + 15) & ~15ull;
+  }
+  return b;
 }
 
 // ---- query dispatch -----------------------------------------------------------------------
@@ -362,7 +378,7 @@ rmq_status build_chunked(const float* d_array, uint64_t n, uint64_t t, int alg, 
   h->t = t;
   h->buffer_floats = P.buffer_floats;
   h->lv = nullptr;
-  h->li = nullptr;
+  h->lr = nullptr;
   h->zeros = nullptr;
   h->view = HierView{};
   h->view.num_levels = P.num_levels;
@@ -477,7 +493,8 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
       delete h;
       return RMQ_ENOMEM;
     }
-    if (hipMalloc(reinterpret_cast<void**>(&h->li), bytes) != hipSuccess) {
+    h->rel_bytes = rel_layout(P.R_log, P.num_levels, P.size, h->view.roff);
+    if (hipMalloc(reinterpret_cast<void**>(&h->lr), h->rel_bytes) != hipSuccess) {
       (void)hipGetLastError();
       (void)hipFree(h->lv);
       (void)hipFree(h->zeros);
@@ -488,7 +505,7 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   HierView& H = h->view;
   H.A = d_array;
   H.lv = h->lv;
-  H.li = h->li;
+  H.lr = h->lr;
   H.zeros = h->zeros;
   H.tail = reinterpret_cast<const float*>(h->zeros + 4);
   H.num_levels = P.num_levels;
@@ -506,10 +523,10 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   }
   for (uint32_t k = 0; k + 1 < P.num_levels; ++k) {
     const float* sv = k == 0 ? d_array : h->lv + P.off[k];
-    const uint32_t* si = k == 0 ? nullptr : h->li + P.off[k];
+    const uint8_t* srel = k == 0 ? nullptr : h->lr + H.roff[k];
     const rmq_status bs =
-        build_level(P.R_log, sv, si, static_cast<uint32_t>(P.size[k]), h->lv + P.off[k + 1],
-                    h->li + P.off[k + 1], static_cast<uint32_t>(P.size[k + 1]), h->zeros, s);
+        build_level(P.R_log, k, sv, srel, static_cast<uint32_t>(P.size[k]), h->lv + P.off[k + 1],
+                    h->lr + H.roff[k + 1], static_cast<uint32_t>(P.size[k + 1]), h->zeros, s);
     if (bs != RMQ_OK) {
       rmq_free(h);
       return bs;
@@ -548,7 +565,7 @@ void rmq_free(rmq_hierarchy* h) {
   if (h->cmin) (void)hipFree(h->cmin);
   if (h->cidx) (void)hipFree(h->cidx);
   if (h->lv) (void)hipFree(h->lv);
-  if (h->li) (void)hipFree(h->li);
+  if (h->lr) (void)hipFree(h->lr);
   if (h->zeros) (void)hipFree(h->zeros);
   if (h->tt) (void)hipFree(h->tt);
   delete h;
@@ -675,4 +692,7 @@ const char* rmq_version(void) { return "rmq-mi355x 0.1 gfx950"; }
 
 const rmq::HierView* rmq_internal_view(const rmq_hierarchy* h) { return h ? &h->view : nullptr; }
 
-uint64_t rmq_internal_level_bytes(const rmq_hierarchy* h) { return h ? h->buffer_floats * 8 : 0; }
+uint64_t rmq_internal_level_bytes(const rmq_hierarchy* h) { return h ? h->buffer_floats * 4 + h->rel_bytes : 0; }
+uint64_t rmq_internal_rel_layout(uint32_t R_log, uint32_t num_levels, const uint64_t* size, uint32_t* roff) {
+  return rel_layout(R_log, num_levels, size, roff);
+}

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/rmq_batch.h"
+#include "rmq_internal.hpp"
 #include "rmq_multi.hpp"
 
 using rmq::HierView;
@@ -41,10 +42,11 @@ struct rmq_batch {
   HierView* d_views;
   uint32_t* d_variant;
   float* lv;
-  uint32_t* li;
+  uint8_t* lr;      // offset entries of every array's levels 1..
   uint32_t* zeros;  // per array: kZeroPos | kZeroNeg of its level 0
   float4* tails;    // per array: the copy of level 0's last 16-byte vector, +inf padded
   uint64_t level_floats;
+  uint64_t rel_bytes;
   std::vector<uint32_t> variant;  // host copy
 };
 
@@ -89,10 +91,10 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
   const HierView& H = views[T.array];
   const uint32_t k = T.level;
   const float* sv = k == 0 ? H.A : H.lv + H.off[k];
-  const uint32_t* si = k == 0 ? nullptr : H.li + H.off[k];
+  const uint8_t* srel = k == 0 ? nullptr : H.lr + H.roff[k];
   const uint32_t ssize = H.size[k];
   float* dv = const_cast<float*>(H.lv) + H.off[k + 1];
-  uint32_t* di = const_cast<uint32_t*>(H.li) + H.off[k + 1];
+  uint8_t* drel = const_cast<uint8_t*>(H.lr) + H.roff[k + 1];
   const uint32_t gl = threadIdx.x & (W - 1);
   uint32_t zb = 0;
   for (uint32_t j = T.j0 + threadIdx.x / W; j < T.j1; j += kBlock / W) {
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
     const uint32_t pos = j * kR + 4 * gl;
     if (pos < ssize) {
       const float4 x = dev::load4(sv, pos, ssize);
-      dev::fold4_idx(bv, bi, x, si, pos, ssize, pos, ssize);
+      dev::fold4_idx(bv, bi, x, srel, RL * k, pos, pos, ssize);
       if (k == 0)
         zb |= dev::zero_bits(x.x, true) | dev::zero_bits(x.y, pos + 1 < ssize) |
               dev::zero_bits(x.z, pos + 2 < ssize) | dev::zero_bits(x.w, pos + 3 < ssize);
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
     dev::group_argmin<W>(bv, bi);
     if (gl == 0) {
       dv[j] = bv;
-      di[j] = bi;
+      dev::rel_store(drel, j, RL * (k + 1), bi);
     }
   }
   if (k == 0) dev::publish_zeros(const_cast<uint32_t*>(H.zeros), zb);
@@ -166,8 +168,8 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
 
   // per-array plans (src/xxx.h:225-243 at R = 32) and arena offsets
   std::vector<HierView> views(count);
-  std::vector<uint64_t> lv_off(count);
-  uint64_t arena = 0;
+  std::vector<uint64_t> lv_off(count), lr_off(count);
+  uint64_t arena = 0, rel_arena = 0;
   uint32_t max_levels = 1;
   for (uint32_t a = 0; a < count; ++a) {
     const rmq_batch_array& d = arrays[a];
@@ -189,6 +191,8 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     }
     lv_off[a] = arena;
     arena += (floats + 31) / 32 * 32;
+    lr_off[a] = rel_arena;
+    rel_arena += rmq_internal_rel_layout(rmq::variant_rlog(d.variant), nl, sizes, H.roff);
     max_levels = std::max(max_levels, nl);
   }
 
@@ -196,9 +200,10 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
   if (!b) return RMQ_ENOMEM;
   b->count = count;
   b->level_floats = arena;
+  b->rel_bytes = rel_arena;
   (void)hipGetDevice(&b->device);
   rmq_status st = dmalloc(&b->lv, arena);
-  if (st == RMQ_OK) st = dmalloc(&b->li, arena);
+  if (st == RMQ_OK) st = dmalloc(&b->lr, rel_arena);
   if (st == RMQ_OK) st = dmalloc(&b->zeros, count);
   if (st == RMQ_OK) st = dmalloc(&b->tails, count);
   if (st == RMQ_OK) st = dmalloc(&b->d_views, count);
@@ -212,7 +217,7 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     HierView& H = views[a];
     H.A = d_arrays + arrays[a].offset;
     H.lv = b->lv + lv_off[a];
-    H.li = b->li + lv_off[a];
+    H.lr = b->lr + lr_off[a];
     H.zeros = b->zeros + a;
     H.tail = reinterpret_cast<const float*>(b->tails + a);
     b->variant[a] = arrays[a].variant;
@@ -367,12 +372,12 @@ void rmq_batch_free(rmq_batch* b) {
   if (b->lv) (void)hipFree(b->lv);
   if (b->zeros) (void)hipFree(b->zeros);
   if (b->tails) (void)hipFree(b->tails);
-  if (b->li) (void)hipFree(b->li);
+  if (b->lr) (void)hipFree(b->lr);
   if (b->d_views) (void)hipFree(b->d_views);
   if (b->d_variant) (void)hipFree(b->d_variant);
   delete b;
 }
 
-uint64_t rmq_batch_level_bytes(const rmq_batch* b) { return b ? b->level_floats * 8 : 0; }
+uint64_t rmq_batch_level_bytes(const rmq_batch* b) { return b ? b->level_floats * 4 + b->rel_bytes : 0; }
 
 }  // extern "C"

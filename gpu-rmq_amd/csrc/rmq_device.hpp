@@ -18,7 +18,7 @@ constexpr uint32_t kMaxLevels = 20;  // XXX_MAX_LEVELS, src/constants.cuh:14
 struct HierView {
   const float* A;        // level 0: the caller's array (borrowed)
   const float* lv;       // level buffer, values  (levels 1..num_levels-1)
-  const uint32_t* li;    // level buffer, leftmost level-0 index of each value
+  const uint8_t* lr;     // winner offsets of levels 1.. (rel_load), at byte offsets roff[k]
   const uint32_t* zeros; // which signs of zero level 0 holds (kZeroPos | kZeroNeg), see below
   const float* tail;     // level 0's last 16-byte vector A[n & ~3, n), +inf padded, 16-byte aligned
   const uint2* tt;       // sparse table over the top level (dev::top_lookup), or null
@@ -26,7 +26,16 @@ struct HierView {
   uint32_t num_levels;
   uint32_t size[kMaxLevels];
   uint32_t off[kMaxLevels];
+  uint32_t roff[kMaxLevels];
 };
+
+// Where the leftmost minimum of an upper-level entry lies: entry j of level k >= 1 keeps
+// i - (j << (R_log k)), i = the leftmost level-0 index of the block's minimum, in the fewest
+// whole bytes that hold a block's span (R^k <= 2^8: one byte, <= 2^16: two, else four). A level's
+// offsets start 16-byte aligned and are padded by four entries.
+__host__ __device__ constexpr uint32_t rel_bytes(uint32_t span_log) {
+  return span_log <= 8 ? 1u : (span_log <= 16 ? 2u : 4u);
+}
 
 // A value-only answer must carry the bit pattern of A[leftmost argmin]; a plain minimum can only
 // miss it when the minimum is zero and the array holds both +0 and -0. The level-0 build pass
@@ -153,15 +162,6 @@ __device__ __forceinline__ uint2 load_query(const uint2* __restrict__ Q, uint64_
 #endif
 }
 
-__device__ __forceinline__ uint4 load4u(const uint32_t* __restrict__ base, uint32_t i, uint32_t size) {
-  if (i + 4 <= size) return *reinterpret_cast<const uint4*>(base + i);
-  uint4 r = make_uint4(kNoIdx, kNoIdx, kNoIdx, kNoIdx);
-  r.x = base[i];
-  if (i + 1 < size) r.y = base[i + 1];
-  if (i + 2 < size) r.z = base[i + 2];
-  return r;
-}
-
 // Element j (0..3) of a vector.
 __device__ __forceinline__ float pick4(float4 x, uint32_t j) {  // two selects, no branch
   const float lo = (j & 1u) ? x.y : x.x;
@@ -242,8 +242,8 @@ __device__ __forceinline__ void take(float& bv, uint32_t& bi, float v, uint32_t 
 // The pieces a query visits stand for disjoint level-0 spans, so candidates are totally ordered
 // by the level-0 start of their span ("key" = position << (R_log * level)). Comparing
 // (value, key) therefore realises the leftmost rule without reading any stored index; only the
-// winner's index is looked up at the end. "pay" says where that index lives: kDirect | position
-// for a level-0 candidate, or the slot of the level-buffer index array.
+// winner's index is looked up at the end. "pay" says where that index lives: kDirect for a
+// level-0 candidate (its key is its position), else the level k of the candidate.
 constexpr uint32_t kDirect = 0x80000000u;
 
 struct Cand {
@@ -262,13 +262,13 @@ __device__ __forceinline__ void cand_take(Cand& b, float v, uint32_t key, uint32
 }
 
 // Fold the elements of the vector at level position p that fall in [lo, hi). shift =
-// R_log * level; pay_base = kDirect at level 0, else the level's offset in the level buffer.
+// R_log * level; pay = kDirect at level 0, else the level.
 __device__ __forceinline__ void cand_fold4(Cand& b, float4 x, uint32_t p, uint32_t lo, uint32_t hi,
-                                           uint32_t shift, uint32_t pay_base) {
-  cand_take(b, x.x, (p + 0) << shift, pay_base + p + 0, in_range(p + 0, lo, hi));
-  cand_take(b, x.y, (p + 1) << shift, pay_base + p + 1, in_range(p + 1, lo, hi));
-  cand_take(b, x.z, (p + 2) << shift, pay_base + p + 2, in_range(p + 2, lo, hi));
-  cand_take(b, x.w, (p + 3) << shift, pay_base + p + 3, in_range(p + 3, lo, hi));
+                                           uint32_t shift, uint32_t pay) {
+  cand_take(b, x.x, (p + 0) << shift, pay, in_range(p + 0, lo, hi));
+  cand_take(b, x.y, (p + 1) << shift, pay, in_range(p + 1, lo, hi));
+  cand_take(b, x.z, (p + 2) << shift, pay, in_range(p + 2, lo, hi));
+  cand_take(b, x.w, (p + 3) << shift, pay, in_range(p + 3, lo, hi));
 }
 
 template <int S>
@@ -292,18 +292,43 @@ __device__ __forceinline__ void group_cand(Cand& b) {
   if constexpr (W >= 64) step_cand<32>(b);
 }
 
-// The leftmost level-0 index of the winning candidate.
-__device__ __forceinline__ uint32_t cand_index(const Cand& b, const uint32_t* __restrict__ li) {
-  return (b.pay & kDirect) ? (b.pay & ~kDirect) : li[b.pay];
+// Offset entry j of a level whose blocks span 2^span_log level-0 elements (roff: the level's
+// byte offset in lr). One aligned 4-byte load; the entry is cut out of it.
+__device__ __forceinline__ uint32_t rel_load(const uint8_t* __restrict__ lr, uint32_t roff, uint32_t j,
+                                             uint32_t span_log) {
+  const uint32_t w = rel_bytes(span_log);
+  const uint32_t a = roff + j * w;
+  const uint32_t word = *reinterpret_cast<const uint32_t*>(lr + (a & ~3u));
+  const uint32_t mask = w == 4u ? 0xFFFFFFFFu : (1u << (8u * w)) - 1u;
+  return (word >> (8u * (a & 3u))) & mask;
 }
 
-// Same fold for the level build: positions are level-k slots, idx from the source index
-// array (ix == nullptr at level 0, where the index is the position itself).
-__device__ __forceinline__ void fold4_idx(float& bv, uint32_t& bi, float4 x, const uint32_t* ix,
-                                          uint32_t i, uint32_t size, uint32_t lo, uint32_t hi) {
+// Four consecutive offset entries (pos % 4 == 0) of a level, one 4-, 8- or 16-byte load; the
+// level's padding covers the entries past its end.
+__device__ __forceinline__ uint4 rel_load4(const uint8_t* __restrict__ r, uint32_t pos, uint32_t w) {
+  if (w == 1u) {
+    const uint32_t x = *reinterpret_cast<const uint32_t*>(r + pos);
+    return make_uint4(x & 0xFFu, (x >> 8) & 0xFFu, (x >> 16) & 0xFFu, x >> 24);
+  }
+  if (w == 2u) {
+    const uint2 x = *reinterpret_cast<const uint2*>(r + 2u * pos);
+    return make_uint4(x.x & 0xFFFFu, x.x >> 16, x.y & 0xFFFFu, x.y >> 16);
+  }
+  return *reinterpret_cast<const uint4*>(r + 4u * pos);
+}
+
+// Build fold over the vector x at level-k position i, elements in [lo, hi). The candidates'
+// level-0 indices: the positions themselves on level 0 (srel null), else (position << span_log)
+// plus the level's offset entries.
+__device__ __forceinline__ void fold4_idx(float& bv, uint32_t& bi, float4 x, const uint8_t* srel,
+                                          uint32_t span_log, uint32_t i, uint32_t lo, uint32_t hi) {
   uint4 id;
-  if (ix) {
-    id = load4u(ix, i, size);
+  if (srel) {
+    id = rel_load4(srel, i, rel_bytes(span_log));
+    id.x += (i + 0) << span_log;
+    id.y += (i + 1) << span_log;
+    id.z += (i + 2) << span_log;
+    id.w += (i + 3) << span_log;
   } else {
     id = make_uint4(i, i + 1, i + 2, i + 3);
   }
@@ -311,6 +336,15 @@ __device__ __forceinline__ void fold4_idx(float& bv, uint32_t& bi, float4 x, con
   take(bv, bi, x.y, id.y, in_range(i + 1, lo, hi));
   take(bv, bi, x.z, id.z, in_range(i + 2, lo, hi));
   take(bv, bi, x.w, id.w, in_range(i + 3, lo, hi));
+}
+
+// Stores the offset entry of level position j (blocks of 2^span_log elements) for winner bi.
+__device__ __forceinline__ void rel_store(uint8_t* __restrict__ drel, uint32_t j, uint32_t span_log, uint32_t bi) {
+  const uint32_t w = rel_bytes(span_log);
+  const uint32_t rel = bi - (span_log >= 32u ? 0u : (j << span_log));
+  if (w == 1u) drel[j] = static_cast<uint8_t>(rel);
+  else if (w == 2u) reinterpret_cast<uint16_t*>(drel)[j] = static_cast<uint16_t>(rel);
+  else reinterpret_cast<uint32_t*>(drel)[j] = rel;
 }
 
 }  // namespace dev

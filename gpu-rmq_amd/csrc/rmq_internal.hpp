@@ -8,3 +8,6 @@
 const rmq::HierView* rmq_internal_view(const rmq_hierarchy* h);
 // Device bytes the hierarchy's level buffers occupy.
 uint64_t rmq_internal_level_bytes(const rmq_hierarchy* h);
+// Byte offsets (roff[k]) of the offset entries of levels 1.. of a hierarchy with these level
+// sizes and reduction factor 2^R_log; returns their total bytes.
+uint64_t rmq_internal_rel_layout(uint32_t R_log, uint32_t num_levels, const uint64_t* size, uint32_t* roff);

@@ -16,15 +16,16 @@ constexpr int kBlock = 256;  // 4 waves per workgroup
 // ---------------------------------------------------------------------------------------
 // Build: level k+1 entry j = min of level k over [j*R, min(j*R+R, size_k)) with the leftmost
 // level-0 index of that minimum (reference: xxx_build_step, src/xxx.h:91-120, which keeps
-// the value only). W = R/4 lanes share one block, each lane owns 16-byte vectors, so one
-// wave instruction streams 1 KiB of contiguous level-k data; the W lanes meet in a DPP
-// all-reduce. Grid-stride over output entries.
-// The level-0 pass (SRC_IDX false) also records which signs of zero the array holds in *zeros.
+// the value only), stored as an offset entry (rel_store). W = R/4 lanes share one block, each
+// lane owns 16-byte vectors, so one wave instruction streams 1 KiB of contiguous level-k data;
+// the W lanes meet in a DPP all-reduce. Grid-stride over output entries. ssk / dsk: log2 of
+// the level-0 span of a source / destination entry. The level-0 pass (SRC_IDX false) also
+// records which signs of zero the array holds in *zeros.
 template <int RL, bool SRC_IDX>
 __global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict__ sv,
-                                                        const uint32_t* __restrict__ si,
+                                                        const uint8_t* __restrict__ srel, uint32_t ssk,
                                                         uint32_t ssize, float* __restrict__ dv,
-                                                        uint32_t* __restrict__ di,
+                                                        uint8_t* __restrict__ drel, uint32_t dsk,
                                                         uint32_t dsize, uint32_t* __restrict__ zeros) {
   constexpr uint32_t R = 1u << RL;
   constexpr uint32_t V = R / 4;                 // 16-byte vectors per block
@@ -43,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict_
       const uint32_t pos = base + 4 * (p * W + gl);
       if (pos < ssize) {
         const float4 x = dev::load4(sv, pos, ssize);
-        dev::fold4_idx(bv, bi, x, SRC_IDX ? si : nullptr, pos, ssize, pos, ssize);
+        dev::fold4_idx(bv, bi, x, SRC_IDX ? srel : nullptr, ssk, pos, pos, ssize);
         if constexpr (!SRC_IDX) {
           zb |= dev::zero_bits(x.x, true) | dev::zero_bits(x.y, pos + 1 < ssize) |
                 dev::zero_bits(x.z, pos + 2 < ssize) | dev::zero_bits(x.w, pos + 3 < ssize);
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict_
     dev::group_argmin<W>(bv, bi);
     if (gl == 0) {
       dv[j] = bv;
-      di[j] = bi;
+      dev::rel_store(drel, static_cast<uint32_t>(j), dsk, bi);
     }
   }
   if constexpr (!SRC_IDX) dev::publish_zeros(zeros, zb);
@@ -96,18 +97,33 @@ struct LevelsOf {
   const HierView& H;
   __device__ __forceinline__ uint32_t off(uint32_t k) const { return H.off[k]; }
   __device__ __forceinline__ uint32_t offz(uint32_t k) const { return k == 0 ? 0u : H.off[k]; }
+  __device__ __forceinline__ uint32_t roff(uint32_t k) const { return H.roff[k]; }
   __device__ __forceinline__ uint32_t size(uint32_t k) const { return H.size[k]; }
 };
 struct LevelsLds {
-  const uint32_t* meta;  // LDS: off[kMaxLevels] with off[0] = 0, then size[kMaxLevels] (lds_meta)
+  const uint32_t* meta;  // LDS: off (with off[0] = 0), size, roff; kMaxLevels each (lds_meta)
   __device__ __forceinline__ uint32_t off(uint32_t k) const { return meta[k]; }
   __device__ __forceinline__ uint32_t offz(uint32_t k) const { return meta[k]; }
   __device__ __forceinline__ uint32_t size(uint32_t k) const { return meta[kMaxLevels + k]; }
+  __device__ __forceinline__ uint32_t roff(uint32_t k) const { return meta[2 * kMaxLevels + k]; }
 };
+constexpr uint32_t kMetaWords = 3 * kMaxLevels;
 
-// Entry i (< 2 * kMaxLevels) of the LDS copy that LevelsLds reads.
+// Entry i (< kMetaWords) of the LDS copy that LevelsLds reads.
 __device__ __forceinline__ uint32_t lds_meta(const HierView& H, uint32_t i) {
-  return i < kMaxLevels ? (i == 0 ? 0u : H.off[i]) : H.size[i - kMaxLevels];
+  if (i < kMaxLevels) return i == 0 ? 0u : H.off[i];
+  if (i < 2 * kMaxLevels) return H.size[i - kMaxLevels];
+  return H.roff[i - 2 * kMaxLevels];
+}
+
+// The winner's level-0 index for a group candidate: its key on level 0, else the key (the
+// level-0 start of its span) plus the level's offset entry.
+template <int RL, class LT>
+__device__ __forceinline__ uint32_t cand_index(const dev::Cand& b, const HierView& H, const LT& T) {
+  if (b.pay & dev::kDirect) return b.key;
+  const uint32_t sk = RL * b.pay;
+  const uint32_t j = sk >= 32u ? 0u : (b.key >> sk);
+  return b.key + dev::rel_load(H.lr, T.roff(b.pay), j, sk);
 }
 
 template <int G, int RL, bool IDX>
@@ -146,7 +162,7 @@ __device__ __forceinline__ GroupAnswer solve_cl(uint32_t l, uint32_t e, uint32_t
       cur_v = H.lv + H.off[lev + 1];
       cur_size = H.size[lev + 1];
       shift += RL;
-      pay = H.off[lev + 1];
+      pay = lev + 1;
     } else {
       climbing = false;
     }
@@ -160,7 +176,7 @@ __device__ __forceinline__ GroupAnswer solve_cl(uint32_t l, uint32_t e, uint32_t
   }
   if constexpr (IDX) {
     dev::group_cand<G>(b);
-    return GroupAnswer{b.v, dev::cand_index(b, H.li)};
+    return GroupAnswer{b.v, cand_index<RL>(b, H, LevelsOf{H})};
   } else {
     return GroupAnswer{dev::group_min<G>(bv), dev::kNoIdx};
   }
@@ -206,7 +222,7 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
   const uint32_t offK = T.offz(K);
   const float* vK = K == 0 ? H.A : H.lv + offK;
   const uint32_t szK = T.size(K);
-  const uint32_t payK = K == 0 ? dev::kDirect : offK;
+  const uint32_t payK = K == 0 ? dev::kDirect : K;
   const bool left_tail = l < (lk << shK);
   const bool right_tail = (ek << shK) < e;
   // guards are issued together with the final range
@@ -239,7 +255,7 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
   if constexpr (IDX) {
     b.v = bv;
     b.key = bpos == dev::kNoIdx ? dev::kNoIdx : bpos << shK;
-    b.pay = bpos == dev::kNoIdx ? dev::kNoIdx : payK + bpos;
+    b.pay = bpos == dev::kNoIdx ? dev::kNoIdx : payK;
     dev::group_cand<G>(b);
     bv = b.v;
   } else {
@@ -258,7 +274,7 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
     const uint32_t ok = T.offz(uk);
     const float* vk = k == 0 ? H.A : H.lv + ok;
     const uint32_t szk = T.size(uk);
-    const uint32_t pay = k == 0 ? dev::kDirect : ok;
+    const uint32_t pay = k == 0 ? dev::kDirect : uk;
     const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);
     const uint32_t hi = e >> sk;
     if (la) fold_piece<G, RL, IDX>(b, bv, vk, szk, lo, (lo + R - 1) & ~(R - 1), gl, sk, pay);
@@ -272,7 +288,7 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
     if (la) la = k != 0 && l < (lo << sk) && !(vk[lo - 1] > bv);  // deeper left pieces can still win
     if (ra) ra = k != 0 && (hi << sk) < e && vk[hi] < bv;
   }
-  if constexpr (IDX) return GroupAnswer{b.v, dev::cand_index(b, H.li)};
+  if constexpr (IDX) return GroupAnswer{b.v, cand_index<RL>(b, H, T)};
   else return GroupAnswer{bv, dev::kNoIdx};
 }
 
@@ -304,9 +320,8 @@ template <int G, int RL, bool P, bool IDX>
 __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ Q, uint32_t q,
                                                       float* __restrict__ out_v,
                                                       uint32_t* __restrict__ out_i, HierView H) {
-  __shared__ uint32_t s_meta[2 * kMaxLevels];
-  if (threadIdx.x < 2 * kMaxLevels)
-    s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
+  __shared__ uint32_t s_meta[kMetaWords];
+  if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
   const LevelsLds T{s_meta};
   const uint32_t lane = threadIdx.x & 63;
@@ -358,9 +373,8 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
                                                         float* __restrict__ out_v,
                                                         uint32_t* __restrict__ out_i,
                                                         HierView H) {
-  __shared__ uint32_t s_meta[2 * kMaxLevels];
-  if (threadIdx.x < 2 * kMaxLevels)
-    s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
+  __shared__ uint32_t s_meta[kMetaWords];
+  if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
   const LevelsLds T{s_meta};
   const uint32_t gl = threadIdx.x & (G - 1);
@@ -432,7 +446,7 @@ __device__ __forceinline__ GroupAnswer solve_vl(uint32_t l, uint32_t e, const Hi
         cur_v = H.lv + H.off[lev + 1];
         cur_size = H.size[lev + 1];
         shift += RL;
-        pay = H.off[lev + 1];
+        pay = lev + 1;
       } else {
         climbing = false;
       }
@@ -445,7 +459,7 @@ __device__ __forceinline__ GroupAnswer solve_vl(uint32_t l, uint32_t e, const Hi
     if constexpr (IDX) dev::cand_fold4(b, x, p, l, e, shift, pay);
     else bv = dev::fold4(bv, x, p, l, e);
   }
-  if constexpr (IDX) return GroupAnswer{b.v, dev::cand_index(b, H.li)};
+  if constexpr (IDX) return GroupAnswer{b.v, cand_index<RL>(b, H, LevelsOf{H})};
   else return GroupAnswer{bv, dev::kNoIdx};
 }
 
@@ -472,10 +486,25 @@ __device__ __forceinline__ float4 load_level(const float* __restrict__ v, uint32
 // The solver in three resumable parts, so that a kernel can hold a query between levels:
 // vl_start reads the final range and its guards, vl_step walks both tails down one level,
 // vl_finish resolves the winner's index. A query needs vl_step while la or ra holds.
+// The lane solver's winner payload for position pos of level k (blocks of 2^sk level-0
+// elements, sk = R_log k): kDirect | pos on level 0; else the level-0 span start pos << sk with
+// bit sk - 1 set, so that the level is read back from the lowest set bit. pay(pos) =
+// base + (pos << sk) with base = 2^((sk - 1) mod 32), which is kDirect on level 0. The solver's
+// levels have sk <= 30 (its final range sits at most floor(30 / R_log) levels up).
+__device__ __forceinline__ uint32_t pay_base(uint32_t sk) { return 1u << ((sk + 31u) & 31u); }
+
+template <int RL, class LT>
+__device__ __forceinline__ uint32_t pay_index(uint32_t pay, const HierView& H, const LT& T) {
+  if (pay & dev::kDirect) return pay & ~dev::kDirect;
+  const uint32_t sk = static_cast<uint32_t>(__builtin_ctz(pay)) + 1u;
+  const uint32_t s0 = pay & ~((1u << sk) - 1u);
+  return s0 + dev::rel_load(H.lr, T.roff(sk / RL), s0 >> sk, sk);
+}
+
 struct VlState {
   uint32_t l, e;
   float bv;
-  uint32_t bpay;  // winner: level-buffer slot, or kDirect | level-0 position
+  uint32_t bpay;  // winner (pay_base, pay_index)
   int k;          // the level the next step reads (K - 1 down to 0)
   bool la, ra;    // a tail's next piece can still win
 };
@@ -498,13 +527,14 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
   const bool left_tail = l < (lk << shK);
   const bool right_tail = (ek << shK) < e;
   float bv = RMQ_INF;
-  uint32_t bpay = (K == 0 ? dev::kDirect : T.offz(K)) + lk;
+  const uint32_t pb = pay_base(shK);
+  uint32_t bpay = pb + (lk << shK);
   float gL = RMQ_INF, gR = RMQ_INF;
   if (TOP && K == top && H.tt != nullptr && ek - lk > dev::kTopScan) {
     // a long final range on the top level: the sparse table's leftmost minimum
     uint32_t pos;
     dev::top_lookup(H.tt, H.tstride, lk, ek, bv, pos);
-    bpay += pos - lk;
+    bpay = pb + (pos << shK);
     if (left_tail) gL = vK[lk - 1u];
     if (right_tail) gR = vK[ek];
   } else {
@@ -512,7 +542,6 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
     uint32_t p = lk & ~3u;
     float4 x = load_level(vK, p, szK, K, H.tail);
     if (left_tail) gL = (lk & 3u) ? dev::pick4(x, (lk - 1u) & 3u) : vK[lk - 1u];
-    const uint32_t pay0 = bpay - lk;
     for (;;) {
       if constexpr (IDX) {
         const float xs[4] = {x.x, x.y, x.z, x.w};
@@ -520,7 +549,7 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
         for (uint32_t j = 0; j < 4; ++j) {
           const bool t = dev::in_range(p + j, lk, ek) && xs[j] < bv;
           bv = t ? xs[j] : bv;
-          bpay = t ? pay0 + p + j : bpay;
+          bpay = t ? pb + ((p + j) << shK) : bpay;
         }
       } else {
         bv = dev::fold4(bv, x, p, lk, ek);
@@ -551,7 +580,7 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
   const uint32_t ok = T.offz(uk);
   const float* vk = k == 0 ? H.A : H.lv + ok;
   const uint32_t szk = T.size(uk);
-  const uint32_t pay0 = k == 0 ? dev::kDirect : ok;
+  const uint32_t pb = pay_base(sk);
   const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);  // left piece [lo, roundup_R(lo))
   const uint32_t lbase = lo & ~(R - 1u);
   const uint32_t hi = e >> sk;                                       // right piece [rlo, hi)
@@ -583,7 +612,7 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
         for (int j = 3; j >= 0; --j) {
           const bool t = p + static_cast<uint32_t>(j) >= lo && xs[j] <= bv;
           bv = t ? xs[j] : bv;
-          if constexpr (IDX) bpay = t ? pay0 + p + static_cast<uint32_t>(j) : bpay;
+          if constexpr (IDX) bpay = t ? pb + ((p + static_cast<uint32_t>(j)) << sk) : bpay;
         }
       }
     }
@@ -607,7 +636,7 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
         for (uint32_t j = 0; j < 4; ++j) {
           const bool t = p + j < hi && xs[j] < bv;
           bv = t ? xs[j] : bv;
-          if constexpr (IDX) bpay = t ? pay0 + p + j : bpay;
+          if constexpr (IDX) bpay = t ? pb + ((p + j) << sk) : bpay;
         }
       }
     }
@@ -625,9 +654,9 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
   st.k = k - 1;
 }
 
-template <bool IDX>
-__device__ __forceinline__ GroupAnswer vl_finish(const VlState& st, const HierView& H) {
-  if constexpr (IDX) return GroupAnswer{st.bv, (st.bpay & dev::kDirect) ? (st.bpay & ~dev::kDirect) : H.li[st.bpay]};
+template <int RL, bool IDX, class LT>
+__device__ __forceinline__ GroupAnswer vl_finish(const VlState& st, const HierView& H, const LT& T) {
+  if constexpr (IDX) return GroupAnswer{st.bv, pay_index<RL>(st.bpay, H, T)};
   else return GroupAnswer{st.bv, dev::kNoIdx};
 }
 
@@ -635,7 +664,7 @@ template <int RL, bool IDX, bool TOP = true, class LT>
 __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
   VlState st = vl_start<RL, IDX, TOP>(l, e, H, T);
   while (st.la || st.ra) vl_step<RL, IDX>(st, H, T);
-  return vl_finish<IDX>(st, H);
+  return vl_finish<RL, IDX>(st, H, T);
 }
 
 template <int RL, bool P, bool IDX, class LT>
@@ -649,9 +678,8 @@ __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q
                                                      float* __restrict__ out_v,
                                                      uint32_t* __restrict__ out_i, HierView H) {
   // level offsets and sizes in LDS: the solver looks one up per level it visits
-  __shared__ uint32_t s_meta[2 * kMaxLevels];
-  if (threadIdx.x < 2 * kMaxLevels)
-    s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
+  __shared__ uint32_t s_meta[kMetaWords];
+  if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
   const LevelsLds T{s_meta};
   const uint32_t zeros = IDX ? 0u : *H.zeros;
