@@ -69,6 +69,45 @@ __global__ __launch_bounds__(kBlock) void k_top_row(const float* __restrict__ to
   tt[static_cast<size_t>(j - 1u) * stride + i] = dev::u2f(b.x) < dev::u2f(a.x) ? b : a;
 }
 
+// The whole table of a top level of at most kTopSmall entries in one workgroup: the current row
+// lives in LDS and is rebuilt in place, each thread reading its entries' two sources before a
+// barrier and writing after it.
+constexpr uint32_t kTopSmall = 8192;
+constexpr uint32_t kTopThreads = 1024;
+
+__global__ __launch_bounds__(kTopThreads) void k_top_table_small(const float* __restrict__ top, uint32_t m,
+                                                                 uint2* __restrict__ tt, uint32_t stride,
+                                                                 uint32_t rows) {
+  constexpr uint32_t kPer = kTopSmall / kTopThreads;
+  __shared__ uint2 cur[kTopSmall];
+  for (uint32_t i = threadIdx.x; i < m; i += kTopThreads) cur[i] = make_uint2(dev::f2u(top[i]), i);
+  __syncthreads();
+  for (uint32_t j = 1; j <= rows; ++j) {
+    const uint32_t h = 1u << (j - 1u);
+    const uint32_t cnt = m + 1u - 2u * h;  // entries i with i + 2^j <= m
+    uint2 nv[kPer];
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+      const uint32_t i = threadIdx.x + r * kTopThreads;
+      if (i < cnt) {
+        const uint2 a = cur[i], b = cur[i + h];
+        nv[r] = dev::u2f(b.x) < dev::u2f(a.x) ? b : a;
+      }
+    }
+    __syncthreads();
+    uint2* row = tt + static_cast<size_t>(j - 1u) * stride;
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+      const uint32_t i = threadIdx.x + r * kTopThreads;
+      if (i < cnt) {
+        cur[i] = nv[r];
+        row[i] = nv[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace rmq
 
 struct rmq_hierarchy {
@@ -80,6 +119,8 @@ struct rmq_hierarchy {
   uint64_t n;
   uint64_t t;
   uint64_t buffer_floats;
+  uint8_t* mem = nullptr;  // the one device allocation the pointers below live in
+  uint64_t mem_bytes = 0;
   float* lv;
   uint8_t* lr;      // offset entries of levels 1.. (HierView::lr)
   uint64_t rel_bytes = 0;
@@ -331,26 +372,45 @@ void launch_query64(const ulonglong2* Q, uint64_t q, float* ov, uint64_t* oi, co
 }
 
 // The sparse table over the top level m = size[top]: floor(log2 m) rows of m entries (row
-// stride rounded up to 64), one launch per row. Skipped for the reference's full order, when
+// stride rounded up to 64). Skipped for the reference's full order, when
 // the caller asks, for small top levels and for tables above kTopTableMax entries.
-rmq_status build_top_table(rmq_hierarchy* h, const Plan& P, const float* d_array, hipStream_t s) {
-  const uint32_t top = P.num_levels - 1;
-  const uint64_t m = P.size[top];
-  if ((h->flags & (RMQ_BUILD_FULL | RMQ_BUILD_NO_TOP_TABLE)) || m < kTopTableMin) return RMQ_OK;
+// Shape of the sparse table over the top level m = size[top] (rows = 0: none is built).
+struct TopShape {
+  uint32_t rows = 0;
+  uint64_t stride = 0;
+};
+
+TopShape top_shape(const Plan& P, uint32_t flags) {
+  TopShape ts;
+  const uint64_t m = P.size[P.num_levels - 1];
+  if ((flags & (RMQ_BUILD_FULL | RMQ_BUILD_NO_TOP_TABLE)) || m < kTopTableMin) return ts;
   uint32_t rows = 0;
   while ((2ull << rows) <= m) ++rows;  // floor(log2 m)
   const uint64_t stride = (m + 63) & ~63ull;
-  if (rows * stride > kTopTableMax) return RMQ_OK;
-  if (hipMalloc(reinterpret_cast<void**>(&h->tt), rows * stride * sizeof(uint2)) != hipSuccess) {
-    (void)hipGetLastError();
-    return RMQ_ENOMEM;
-  }
+  if (rows * stride > kTopTableMax) return ts;
+  ts.rows = rows;
+  ts.stride = stride;
+  return ts;
+}
+
+// Fills the table (h->tt, already carved out of the hierarchy's memory): one launch for a top
+// level of at most kTopSmall entries, else one launch per row.
+rmq_status build_top_table(rmq_hierarchy* h, const Plan& P, const TopShape& ts, const float* d_array,
+                           hipStream_t s) {
+  if (ts.rows == 0) return RMQ_OK;
+  const uint32_t top = P.num_levels - 1;
+  const uint64_t m = P.size[top];
   const float* tv = top == 0 ? d_array : h->lv + P.off[top];
-  const uint32_t grid = static_cast<uint32_t>((m + kBlock - 1) / kBlock);
-  for (uint32_t j = 1; j <= rows; ++j)
-    rmq::k_top_row<<<grid, kBlock, 0, s>>>(tv, static_cast<uint32_t>(m), h->tt, static_cast<uint32_t>(stride), j);
+  if (m <= rmq::kTopSmall) {
+    rmq::k_top_table_small<<<1, rmq::kTopThreads, 0, s>>>(tv, static_cast<uint32_t>(m), h->tt,
+                                                          static_cast<uint32_t>(ts.stride), ts.rows);
+  } else {
+    const uint32_t grid = static_cast<uint32_t>((m + kBlock - 1) / kBlock);
+    for (uint32_t j = 1; j <= ts.rows; ++j)
+      rmq::k_top_row<<<grid, kBlock, 0, s>>>(tv, static_cast<uint32_t>(m), h->tt, static_cast<uint32_t>(ts.stride), j);
+  }
   h->view.tt = h->tt;
-  h->view.tstride = static_cast<uint32_t>(stride);
+  h->view.tstride = static_cast<uint32_t>(ts.stride);
   return last_error();
 }
 
@@ -477,31 +537,37 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   h->n = n;
   h->t = t;
   h->buffer_floats = P.buffer_floats;
-  if (hipMalloc(reinterpret_cast<void**>(&h->zeros), 8 * sizeof(uint32_t)) != hipSuccess) {
+  // one allocation: the small words (zero signs, level 0's last vector), the level values, the
+  // offset entries and the top level's sparse table, each 256-byte aligned
+  uint64_t bytes = 0;
+  auto carve = [&bytes](uint64_t b) {
+    const uint64_t at = bytes;
+    bytes += (b + 255) & ~255ull;
+    return at;
+  };
+  const uint64_t at_words = carve(8 * sizeof(uint32_t));
+  uint64_t at_lv = 0, at_lr = 0;
+  if (P.num_levels > 1) {
+    // padded to the level alignment past the last level too, so that every 16-byte vector that
+    // starts inside a level lies inside the allocation
+    at_lv = carve(((P.buffer_floats + RMQ_LEVEL_ALIGNMENT - 1) & ~static_cast<uint64_t>(RMQ_LEVEL_ALIGNMENT - 1)) * 4);
+    h->rel_bytes = rel_layout(P.R_log, P.num_levels, P.size, h->view.roff);
+    at_lr = carve(h->rel_bytes);
+  }
+  const TopShape ts = top_shape(P, h->flags);
+  const uint64_t at_tt = carve(ts.rows * ts.stride * sizeof(uint2));
+  if (hipMalloc(reinterpret_cast<void**>(&h->mem), bytes) != hipSuccess) {
     (void)hipGetLastError();
     delete h;
     return RMQ_ENOMEM;
   }
+  h->mem_bytes = bytes;
+  h->zeros = reinterpret_cast<uint32_t*>(h->mem + at_words);
   if (P.num_levels > 1) {
-    // padded to the level alignment past the last level too, so that every 16-byte vector that
-    // starts inside a level lies inside the allocation
-    const size_t bytes = static_cast<size_t>((P.buffer_floats + RMQ_LEVEL_ALIGNMENT - 1) &
-                                             ~static_cast<uint64_t>(RMQ_LEVEL_ALIGNMENT - 1)) * 4;
-    if (hipMalloc(reinterpret_cast<void**>(&h->lv), bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      (void)hipFree(h->zeros);
-      delete h;
-      return RMQ_ENOMEM;
-    }
-    h->rel_bytes = rel_layout(P.R_log, P.num_levels, P.size, h->view.roff);
-    if (hipMalloc(reinterpret_cast<void**>(&h->lr), h->rel_bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      (void)hipFree(h->lv);
-      (void)hipFree(h->zeros);
-      delete h;
-      return RMQ_ENOMEM;
-    }
+    h->lv = reinterpret_cast<float*>(h->mem + at_lv);
+    h->lr = h->mem + at_lr;
   }
+  if (ts.rows) h->tt = reinterpret_cast<uint2*>(h->mem + at_tt);
   HierView& H = h->view;
   H.A = d_array;
   H.lv = h->lv;
@@ -532,10 +598,10 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
       return bs;
     }
   }
-  const rmq_status ts = build_top_table(h, P, d_array, s);
-  if (ts != RMQ_OK) {
+  const rmq_status tst = build_top_table(h, P, ts, d_array, s);
+  if (tst != RMQ_OK) {
     rmq_free(h);
-    return ts;
+    return tst;
   }
   *out = h;
   return RMQ_OK;
@@ -564,10 +630,7 @@ void rmq_free(rmq_hierarchy* h) {
   if (h->views) (void)hipFree(h->views);
   if (h->cmin) (void)hipFree(h->cmin);
   if (h->cidx) (void)hipFree(h->cidx);
-  if (h->lv) (void)hipFree(h->lv);
-  if (h->lr) (void)hipFree(h->lr);
-  if (h->zeros) (void)hipFree(h->zeros);
-  if (h->tt) (void)hipFree(h->tt);
+  if (h->mem) (void)hipFree(h->mem);
   delete h;
 }
 
@@ -692,7 +755,7 @@ const char* rmq_version(void) { return "rmq-mi355x 0.1 gfx950"; }
 
 const rmq::HierView* rmq_internal_view(const rmq_hierarchy* h) { return h ? &h->view : nullptr; }
 
-uint64_t rmq_internal_level_bytes(const rmq_hierarchy* h) { return h ? h->buffer_floats * 4 + h->rel_bytes : 0; }
+uint64_t rmq_internal_level_bytes(const rmq_hierarchy* h) { return h ? h->mem_bytes : 0; }
 uint64_t rmq_internal_rel_layout(uint32_t R_log, uint32_t num_levels, const uint64_t* size, uint32_t* roff) {
   return rel_layout(R_log, num_levels, size, roff);
 }

@@ -6,7 +6,7 @@
 
 // The device-side level view of a built hierarchy (valid until rmq_free).
 const rmq::HierView* rmq_internal_view(const rmq_hierarchy* h);
-// Device bytes the hierarchy's level buffers occupy.
+// Device bytes the hierarchy occupies (levels, offset entries, top-level table, small words).
 uint64_t rmq_internal_level_bytes(const rmq_hierarchy* h);
 // Byte offsets (roff[k]) of the offset entries of levels 1.. of a hierarchy with these level
 // sizes and reduction factor 2^R_log; returns their total bytes.
