@@ -1,10 +1,11 @@
-// rmq_kernels.hpp — gfx950 kernels of the hierarchical RMQ: level build, the three query
-// variants (algs 16, 24, 20) and GPU BASE (alg 2/102).
+// rmq_kernels.hpp — gfx950 kernels of the hierarchical RMQ: level build and the three query
+// variants (algs 16, 24, 20).
 //
 // Data layout in HBM: level 0 is the caller's float array (borrowed). Levels 1.. live in one
-// value buffer and one index buffer at the reference's offsets (multiples of 32 elements,
-// src/xxx.h:238), so every level starts 128-byte aligned and every R-block of R >= 4
-// elements is whole 16-byte vectors.
+// value buffer at the reference's offsets (multiples of 32 elements, src/xxx.h:238), so every
+// level starts 128-byte aligned and every R-block of R >= 4 elements is whole 16-byte vectors;
+// where each entry's minimum lies is kept beside them as a 1-, 2- or 4-byte offset in its block
+// (rel_bytes in rmq_device.hpp).
 #pragma once
 
 #include "rmq_device.hpp"
