@@ -366,9 +366,12 @@ void launch_chunk_min(const HierView* views, uint32_t chunks, uint32_t clog, flo
 
 template <int RL>
 void launch_query64(const ulonglong2* Q, uint64_t q, float* ov, uint64_t* oi, const HierView* views,
-                    uint32_t clog, const float* cv, const uint64_t* ci, hipStream_t s) {
-  const uint64_t grid = (q + kBlock - 1) / kBlock;
-  rmq::k_query64<RL><<<static_cast<uint32_t>(grid), kBlock, 0, s>>>(Q, q, ov, oi, views, clog, cv, ci);
+                    uint32_t clog, uint32_t chunks, const float* cv, const uint64_t* ci, hipStream_t s) {
+  const uint32_t grid = static_cast<uint32_t>((q + kBlock - 1) / kBlock);
+  if (chunks <= rmq::kLdsChunks)
+    rmq::k_query64<RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, views, clog, chunks, cv, ci);
+  else
+    rmq::k_query64<RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, views, clog, chunks, cv, ci);
 }
 
 // The sparse table over the top level m = size[top]: floor(log2 m) rows of m entries (row
@@ -713,14 +716,15 @@ rmq_status rmq_query64(const rmq_hierarchy* hc, const uint64_t* d_lr, uint64_t q
     }
     clog = 63;  // every position is in chunk 0
   }
+  const uint32_t chunks = h->chunk_log ? h->chunks : 1u;
   const ulonglong2* Q = reinterpret_cast<const ulonglong2*>(d_lr);
   switch (h->R_log) {
-    case 2: launch_query64<2>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
-    case 3: launch_query64<3>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
-    case 4: launch_query64<4>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
-    case 5: launch_query64<5>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
-    case 6: launch_query64<6>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
-    case 7: launch_query64<7>(Q, q, d_val, d_idx, h->views, clog, cv, ci, s); break;
+    case 2: launch_query64<2>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
+    case 3: launch_query64<3>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
+    case 4: launch_query64<4>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
+    case 5: launch_query64<5>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
+    case 6: launch_query64<6>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
+    case 7: launch_query64<7>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
     default: return RMQ_EUNSUPPORTED;
   }
   return last_error();

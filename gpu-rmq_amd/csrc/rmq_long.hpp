@@ -14,11 +14,14 @@
 
 namespace rmq {
 
-template <int RL>
-__device__ __forceinline__ GroupAnswer solve_chunk(const HierView& H, uint32_t l, uint32_t e) {
-  const LevelsOf T{H};
+template <int RL, class LT>
+__device__ __forceinline__ GroupAnswer solve_chunk(const HierView& H, const LT& T, uint32_t l, uint32_t e) {
   return solve_vl_pruned<RL, true>(l, e, H, T);
 }
+
+// Chunks whose level tables a query kernel copies into LDS (more fall back to reading them from
+// the chunks' views).
+constexpr uint32_t kLdsChunks = 64;
 
 // The minimum of every whole chunk: one lane per chunk.
 template <int RL>
@@ -28,35 +31,44 @@ __global__ __launch_bounds__(kBlock) void k_chunk_min(const HierView* __restrict
   const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= chunks) return;
   const HierView& H = views[c];
-  const GroupAnswer a = solve_chunk<RL>(H, 0, H.size[0]);
+  const GroupAnswer a = solve_chunk<RL>(H, LevelsOf{H}, 0, H.size[0]);
   cv[c] = a.v;
   ci[c] = (static_cast<uint64_t>(c) << clog) + a.idx;
 }
 
-// One lane per query; Q holds {l, r} pairs of 64-bit positions.
-template <int RL>
+// One lane per query; Q holds {l, r} pairs of 64-bit positions. LDS_META: all `chunks` level
+// tables (at most kLdsChunks) are copied into LDS first.
+template <int RL, bool LDS_META>
 __global__ __launch_bounds__(kBlock) void k_query64(const ulonglong2* __restrict__ Q, uint64_t q,
                                                     float* __restrict__ out_v, uint64_t* __restrict__ out_i,
                                                     const HierView* __restrict__ views, uint32_t clog,
-                                                    const float* __restrict__ cv,
+                                                    uint32_t chunks, const float* __restrict__ cv,
                                                     const uint64_t* __restrict__ ci) {
+  __shared__ uint32_t s_meta[LDS_META ? kLdsChunks * kMetaWords : 1];
+  if constexpr (LDS_META) {
+    for (uint32_t i = threadIdx.x; i < chunks * kMetaWords; i += kBlock)
+      s_meta[i] = lds_meta(views[i / kMetaWords], i % kMetaWords);
+    __syncthreads();
+  }
+  auto table = [&](uint32_t c) {
+    if constexpr (LDS_META) return LevelsLds{s_meta + c * kMetaWords};
+    else return LevelsOf{views[c]};
+  };
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (qi >= q) return;
   const ulonglong2 p = Q[qi];
   const uint64_t l = p.x, e = p.y + 1;
   const uint32_t cl = static_cast<uint32_t>(l >> clog), cr = static_cast<uint32_t>(p.y >> clog);
   const uint64_t bl = static_cast<uint64_t>(cl) << clog;
-  float bv;
-  uint64_t bi;
-  if (cl == cr) {
-    const GroupAnswer a = solve_chunk<RL>(views[cl], static_cast<uint32_t>(l - bl), static_cast<uint32_t>(e - bl));
-    bv = a.v;
-    bi = bl + a.idx;
-  } else {
-    const HierView& HL = views[cl];
-    const GroupAnswer a = solve_chunk<RL>(HL, static_cast<uint32_t>(l - bl), HL.size[0]);
-    bv = a.v;
-    bi = bl + a.idx;
+  // every query solves its part in its first chunk (all of it, or the chunk's suffix from l); a
+  // query across chunks then adds the whole chunks' minima and its last chunk's prefix, so a
+  // wavefront that mixes both kinds runs the solver twice, not three times
+  const HierView& HL = views[cl];
+  const uint32_t ea = cl == cr ? static_cast<uint32_t>(e - bl) : HL.size[0];
+  const GroupAnswer a = solve_chunk<RL>(HL, table(cl), static_cast<uint32_t>(l - bl), ea);
+  float bv = a.v;
+  uint64_t bi = bl + a.idx;
+  if (cl != cr) {
     for (uint32_t c = cl + 1; c < cr; ++c) {  // chunks covered whole, left to right
       const float v = cv[c];
       if (v < bv) {
@@ -65,7 +77,7 @@ __global__ __launch_bounds__(kBlock) void k_query64(const ulonglong2* __restrict
       }
     }
     const uint64_t br = static_cast<uint64_t>(cr) << clog;
-    const GroupAnswer b = solve_chunk<RL>(views[cr], 0, static_cast<uint32_t>(e - br));
+    const GroupAnswer b = solve_chunk<RL>(views[cr], table(cr), 0, static_cast<uint32_t>(e - br));
     if (b.v < bv) {
       bv = b.v;
       bi = br + b.idx;

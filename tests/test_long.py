@@ -67,6 +67,29 @@ def test_chunked_small_against_cpu_base(device, alg, opts):
             assert np.array_equal(v.view(np.uint32), A[ref.astype(np.int64)].view(np.uint32))
 
 
+def test_many_chunks(device):
+    """391 chunks of 2^8: more than the query kernel copies into LDS, so the chunks' level tables
+    are read from their views (the other form of the kernel)."""
+    import torch
+
+    rng = np.random.default_rng(391)
+    n, clog = 100003, 8
+    A = rng.integers(0, 5, size=n).astype(np.float32)
+    lr = _chunk_queries(rng, n, 1 << clog)
+    ref = _cpu_ref(A, lr)
+    dA = torch.from_numpy(A).cuda()
+    dQ = torch.from_numpy(lr.view(np.int64)).cuda()
+    for alg, opts, t in ((20, dict(cg_amount_log=0), 16), (16, {}, 64)):
+        _say(f"many chunks alg={alg} opts={opts} t={t}")
+        with rmq_amd.Hierarchy(dA, t, alg, chunk_log=clog, **opts) as h:
+            v, i = h.query64(dQ, True, True)
+            torch.cuda.synchronize()
+        i = i.cpu().numpy().view(np.uint64)
+        bad = np.nonzero(i != ref)[0]
+        assert bad.size == 0, (alg, lr[bad[:3]].tolist(), i[bad[:3]], ref[bad[:3]])
+        assert np.array_equal(v.cpu().numpy().view(np.uint32), A[ref.astype(np.int64)].view(np.uint32))
+
+
 def test_query64_on_one_piece_equals_query(device):
     import torch
 
