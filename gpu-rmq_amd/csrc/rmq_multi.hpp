@@ -78,8 +78,8 @@ __global__ void k_reset_counters(uint32_t* __restrict__ c) {
 
 // ---- query: persistent, tiles pulled from per-XCD queues ----------------------------------------
 // One 64-query unit: queries q0 .. q0 + count - 1 (count <= 64), one per lane of the wavefront.
-template <int G, bool IDX>
-__device__ __forceinline__ void unit_groups(const HierView& H, const uint2* __restrict__ Q,
+template <int G, bool IDX, class LT>
+__device__ __forceinline__ void unit_groups(const HierView& H, const LT& T, const uint2* __restrict__ Q,
                                             uint64_t q0, uint32_t count, float* __restrict__ ov,
                                             uint32_t* __restrict__ oi) {
   const uint32_t lane = threadIdx.x & 63;
@@ -104,11 +104,11 @@ __device__ __forceinline__ void unit_groups(const HierView& H, const uint2* __re
       float v;
       uint32_t i = dev::kNoIdx;
       if constexpr (IDX) {
-        const GroupAnswer a = solve_cl_pruned<G, kMultiRLog, true, false>(lk, ek, gl, H, LevelsOf{H});
+        const GroupAnswer a = solve_cl_pruned<G, kMultiRLog, true, false>(lk, ek, gl, H, T);
         v = a.v;
         i = a.idx;
       } else {
-        v = value_exact<G, kMultiRLog, true, false>(lk, ek, gl, H, LevelsOf{H}, *H.zeros);
+        v = value_exact<G, kMultiRLog, true, false>(lk, ek, gl, H, T, *H.zeros);
       }
       if (gl == static_cast<uint32_t>(k)) {
         my_v = v;
@@ -122,14 +122,13 @@ __device__ __forceinline__ void unit_groups(const HierView& H, const uint2* __re
   }
 }
 
-template <int RL, bool IDX>
-__device__ __forceinline__ void unit_lanes(const HierView& H, const uint2* __restrict__ Q,
+template <int RL, bool IDX, class LT>
+__device__ __forceinline__ void unit_lanes(const HierView& H, const LT& T, const uint2* __restrict__ Q,
                                            uint64_t q0, uint32_t count, float* __restrict__ ov,
                                            uint32_t* __restrict__ oi) {
   const uint32_t local = threadIdx.x & 63;
   if (local >= count) return;
   const uint2 p = dev::load_query(Q, q0 + local);
-  const LevelsOf T{H};
   GroupAnswer a = solve_vl_pruned<RL, IDX, false>(p.x, p.y + 1, H, T);
   if constexpr (!IDX) {
     if (a.v == 0.0f && !dev::zero_known(*H.zeros, a.v)) a = solve_vl_pruned<RL, true, false>(p.x, p.y + 1, H, T);
@@ -138,9 +137,11 @@ __device__ __forceinline__ void unit_lanes(const HierView& H, const uint2* __res
   if (IDX && oi) oi[q0 + local] = a.idx;
 }
 
-// Answers unit u (64 queries) of queue q with the variant of the unit's array.
+// Answers unit u (64 queries) of queue q with the variant of the unit's array. The wavefront
+// first copies that array's level table into its own LDS slot `meta` (its LDS operations complete
+// in order, so the previous unit's reads of the slot are done before these writes land).
 template <bool IDX>
-__device__ __forceinline__ void run_unit(const HierView* __restrict__ views,
+__device__ __forceinline__ void run_unit(uint32_t* meta, const HierView* __restrict__ views,
                                          const uint32_t* __restrict__ variant,
                                          const QueryTile* __restrict__ tiles, uint32_t first, uint32_t u,
                                          const uint2* __restrict__ Q, float* __restrict__ ov,
@@ -152,14 +153,17 @@ __device__ __forceinline__ void run_unit(const HierView* __restrict__ views,
   const uint64_t q0 = T.q0 + part;
   const uint32_t a = __builtin_amdgcn_readfirstlane(T.array);
   const HierView& H = views[a];
+  const uint32_t lane = threadIdx.x & 63u;
+  if (lane < kMetaWords) meta[lane] = lds_meta(H, lane);
+  const LevelsLds L{meta};
   switch (variant[a]) {
-    case RMQ_VARIANT_GROUP4: unit_groups<4, IDX>(H, Q, q0, cnt, ov, oi); break;
-    case RMQ_VARIANT_GROUP16: unit_groups<16, IDX>(H, Q, q0, cnt, ov, oi); break;
-    case RMQ_VARIANT_GROUP32: unit_groups<32, IDX>(H, Q, q0, cnt, ov, oi); break;
-    case RMQ_VARIANT_WAVE: unit_groups<64, IDX>(H, Q, q0, cnt, ov, oi); break;
-    case RMQ_VARIANT_LANE: unit_lanes<kMultiRLog, IDX>(H, Q, q0, cnt, ov, oi); break;
-    case RMQ_VARIANT_LANE8: unit_lanes<kLane8RLog, IDX>(H, Q, q0, cnt, ov, oi); break;
-    default: unit_groups<8, IDX>(H, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_GROUP4: unit_groups<4, IDX>(H, L, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_GROUP16: unit_groups<16, IDX>(H, L, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_GROUP32: unit_groups<32, IDX>(H, L, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_WAVE: unit_groups<64, IDX>(H, L, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_LANE: unit_lanes<kMultiRLog, IDX>(H, L, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_LANE8: unit_lanes<kLane8RLog, IDX>(H, L, Q, q0, cnt, ov, oi); break;
+    default: unit_groups<8, IDX>(H, L, Q, q0, cnt, ov, oi); break;
   }
 }
 
@@ -190,8 +194,10 @@ __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
     uint32_t* __restrict__ oi) {
   // Queues a wavefront of this workgroup found drained; the others skip them.
   __shared__ uint32_t s_drained;
+  __shared__ uint32_t s_meta[kBlock / kWaveQueries][kMetaWords];  // each wavefront's level table
   if (threadIdx.x == 0) s_drained = 0;
   __syncthreads();
+  uint32_t* meta = s_meta[threadIdx.x / kWaveQueries];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t home = blockIdx.x % kQueues;
   {
@@ -199,7 +205,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
     queue_shape(qstart, home, first, units, dyn0, share);
     const uint32_t w = (blockIdx.x / kQueues) * (kBlock / kWaveQueries) + threadIdx.x / kWaveQueries;
     for (uint32_t u = w * share; u < (w + 1) * share; ++u)
-      run_unit<IDX>(views, variant, tiles, first, u, Q, ov, oi);
+      run_unit<IDX>(meta, views, variant, tiles, first, u, Q, ov, oi);
   }
   uint32_t queue = home;
   uint32_t tried = 0;
@@ -218,7 +224,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
       continue;
     }
     const uint32_t u1 = min(u0 + kClaimUnits, units);
-    for (uint32_t u = u0; u < u1; ++u) run_unit<IDX>(views, variant, tiles, first, u, Q, ov, oi);
+    for (uint32_t u = u0; u < u1; ++u) run_unit<IDX>(meta, views, variant, tiles, first, u, Q, ov, oi);
   }
 }
 
