@@ -69,43 +69,12 @@ __global__ __launch_bounds__(kBlock) void k_top_row(const float* __restrict__ to
   tt[static_cast<size_t>(j - 1u) * stride + i] = dev::u2f(b.x) < dev::u2f(a.x) ? b : a;
 }
 
-// The whole table of a top level of at most kTopSmall entries in one workgroup: the current row
-// lives in LDS and is rebuilt in place, each thread reading its entries' two sources before a
-// barrier and writing after it.
-constexpr uint32_t kTopSmall = 8192;
-constexpr uint32_t kTopThreads = 1024;
-
+// The whole table of a top level of at most kTopSmall entries in one workgroup.
 __global__ __launch_bounds__(kTopThreads) void k_top_table_small(const float* __restrict__ top, uint32_t m,
                                                                  uint2* __restrict__ tt, uint32_t stride,
                                                                  uint32_t rows) {
-  constexpr uint32_t kPer = kTopSmall / kTopThreads;
   __shared__ uint2 cur[kTopSmall];
-  for (uint32_t i = threadIdx.x; i < m; i += kTopThreads) cur[i] = make_uint2(dev::f2u(top[i]), i);
-  __syncthreads();
-  for (uint32_t j = 1; j <= rows; ++j) {
-    const uint32_t h = 1u << (j - 1u);
-    const uint32_t cnt = m + 1u - 2u * h;  // entries i with i + 2^j <= m
-    uint2 nv[kPer];
-#pragma unroll
-    for (uint32_t r = 0; r < kPer; ++r) {
-      const uint32_t i = threadIdx.x + r * kTopThreads;
-      if (i < cnt) {
-        const uint2 a = cur[i], b = cur[i + h];
-        nv[r] = dev::u2f(b.x) < dev::u2f(a.x) ? b : a;
-      }
-    }
-    __syncthreads();
-    uint2* row = tt + static_cast<size_t>(j - 1u) * stride;
-#pragma unroll
-    for (uint32_t r = 0; r < kPer; ++r) {
-      const uint32_t i = threadIdx.x + r * kTopThreads;
-      if (i < cnt) {
-        cur[i] = nv[r];
-        row[i] = nv[r];
-      }
-    }
-    __syncthreads();
-  }
+  top_table_block(top, m, tt, stride, rows, cur);
 }
 
 }  // namespace rmq

@@ -61,6 +61,44 @@ __global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict_
   if constexpr (!SRC_IDX) dev::publish_zeros(zeros, zb);
 }
 
+// The sparse table over a top level top[0, m), m <= kTopSmall, built by one workgroup of
+// kTopThreads threads: row j >= 1 entry i (i + 2^j <= m) is the leftmost minimum of
+// [i, i + 2^j) (dev::top_lookup). The current row lives in the LDS array `cur` and is rebuilt in
+// place, each thread reading its entries' two sources before a barrier and writing after it.
+constexpr uint32_t kTopSmall = 8192;
+constexpr uint32_t kTopThreads = 1024;
+
+__device__ __forceinline__ void top_table_block(const float* __restrict__ top, uint32_t m, uint2* __restrict__ tt,
+                                                uint32_t stride, uint32_t rows, uint2* cur) {
+  constexpr uint32_t kPer = kTopSmall / kTopThreads;
+  for (uint32_t i = threadIdx.x; i < m; i += kTopThreads) cur[i] = make_uint2(dev::f2u(top[i]), i);
+  __syncthreads();
+  for (uint32_t j = 1; j <= rows; ++j) {
+    const uint32_t h = 1u << (j - 1u);
+    const uint32_t cnt = m + 1u - 2u * h;  // entries i with i + 2^j <= m
+    uint2 nv[kPer];
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+      const uint32_t i = threadIdx.x + r * kTopThreads;
+      if (i < cnt) {
+        const uint2 a = cur[i], b = cur[i + h];
+        nv[r] = dev::u2f(b.x) < dev::u2f(a.x) ? b : a;
+      }
+    }
+    __syncthreads();
+    uint2* row = tt + static_cast<size_t>(j - 1u) * stride;
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+      const uint32_t i = threadIdx.x + r * kTopThreads;
+      if (i < cnt) {
+        cur[i] = nv[r];
+        row[i] = nv[r];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Query traversal shared by algs 16 and 24 (src/xxx.h:158-187 and 493-521): r exclusive;
 // while the remaining range at this level is longer than 2R, the partial blocks
