@@ -98,13 +98,14 @@ struct rmq_hierarchy {
   HierView view;
   // 64-bit form: chunks of 2^chunk_log elements, each its own hierarchy (chunk_log = 0: one
   // hierarchy over the whole array). views / cmin / cidx are device tables: the chunks' views
-  // (or this hierarchy's own view, made by the first rmq_query64) and every chunk's minimum.
+  // and every chunk's minimum.
   uint32_t chunk_log = 0;
   uint32_t chunks = 0;
   uint64_t msize[RMQ_MAX_LEVELS] = {};  // the reference's metadata for the whole array
   uint64_t moff[RMQ_MAX_LEVELS] = {};
   rmq_hierarchy** part = nullptr;
   HierView* views = nullptr;
+  HierView* self_view = nullptr;  // one-piece hierarchy: its own view in device memory
   float* cmin = nullptr;
   uint64_t* cidx = nullptr;
 };
@@ -528,6 +529,7 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   }
   const TopShape ts = top_shape(P, h->flags);
   const uint64_t at_tt = carve(ts.rows * ts.stride * sizeof(uint2));
+  const uint64_t at_view = carve(sizeof(HierView));  // the view, on the device, for rmq_query64
   if (hipMalloc(reinterpret_cast<void**>(&h->mem), bytes) != hipSuccess) {
     (void)hipGetLastError();
     delete h;
@@ -540,6 +542,7 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
     h->lr = h->mem + at_lr;
   }
   if (ts.rows) h->tt = reinterpret_cast<uint2*>(h->mem + at_tt);
+  h->self_view = reinterpret_cast<HierView*>(h->mem + at_view);
   HierView& H = h->view;
   H.A = d_array;
   H.lv = h->lv;
@@ -574,6 +577,12 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   if (tst != RMQ_OK) {
     rmq_free(h);
     return tst;
+  }
+  // the finished view (h lives until rmq_free, so the copy's source outlives it)
+  if (hipMemcpyAsync(h->self_view, &h->view, sizeof(HierView), hipMemcpyHostToDevice, s) != hipSuccess) {
+    (void)hipGetLastError();
+    rmq_free(h);
+    return RMQ_EDEVICE;
   }
   *out = h;
   return RMQ_OK;
@@ -661,39 +670,27 @@ rmq_status rmq_gpu_base(const float* d_array, uint64_t n, const uint32_t* d_lr, 
   return last_error();
 }
 
-rmq_status rmq_query64(const rmq_hierarchy* hc, const uint64_t* d_lr, uint64_t q, float* d_val,
+rmq_status rmq_query64(const rmq_hierarchy* h, const uint64_t* d_lr, uint64_t q, float* d_val,
                        uint64_t* d_idx, void* stream) {
-  if (!hc) return RMQ_EINVAL;
+  if (!h) return RMQ_EINVAL;
   if (q == 0) return RMQ_OK;
   if (!d_lr || (!d_val && !d_idx) || (reinterpret_cast<uintptr_t>(d_lr) & 15u) != 0) return RMQ_EINVAL;
   if ((q + kBlock - 1) / kBlock > 0xFFFFFFFFull) return RMQ_EINVAL;
-  rmq_hierarchy* h = const_cast<rmq_hierarchy*>(hc);
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  uint32_t clog = h->chunk_log;
+  // a one-piece hierarchy is its own single chunk: every position is in chunk 0
+  const uint32_t clog = h->chunk_log ? h->chunk_log : 63u;
+  const uint32_t chunks = h->chunk_log ? h->chunks : 1u;
+  const HierView* views = h->chunk_log ? h->views : h->self_view;
   const float* cv = h->cmin;
   const uint64_t* ci = h->cidx;
-  if (!clog) {  // one hierarchy over the whole array: a one-entry view table, made once
-    if (!h->views) {
-      if (hipMalloc(reinterpret_cast<void**>(&h->views), sizeof(HierView)) != hipSuccess) {
-        (void)hipGetLastError();
-        h->views = nullptr;
-        return RMQ_ENOMEM;
-      }
-      if (hipMemcpyAsync(h->views, &h->view, sizeof(HierView), hipMemcpyHostToDevice, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess)
-        return RMQ_EDEVICE;
-    }
-    clog = 63;  // every position is in chunk 0
-  }
-  const uint32_t chunks = h->chunk_log ? h->chunks : 1u;
   const ulonglong2* Q = reinterpret_cast<const ulonglong2*>(d_lr);
   switch (h->R_log) {
-    case 2: launch_query64<2>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
-    case 3: launch_query64<3>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
-    case 4: launch_query64<4>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
-    case 5: launch_query64<5>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
-    case 6: launch_query64<6>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
-    case 7: launch_query64<7>(Q, q, d_val, d_idx, h->views, clog, chunks, cv, ci, s); break;
+    case 2: launch_query64<2>(Q, q, d_val, d_idx, views, clog, chunks, cv, ci, s); break;
+    case 3: launch_query64<3>(Q, q, d_val, d_idx, views, clog, chunks, cv, ci, s); break;
+    case 4: launch_query64<4>(Q, q, d_val, d_idx, views, clog, chunks, cv, ci, s); break;
+    case 5: launch_query64<5>(Q, q, d_val, d_idx, views, clog, chunks, cv, ci, s); break;
+    case 6: launch_query64<6>(Q, q, d_val, d_idx, views, clog, chunks, cv, ci, s); break;
+    case 7: launch_query64<7>(Q, q, d_val, d_idx, views, clog, chunks, cv, ci, s); break;
     default: return RMQ_EUNSUPPORTED;
   }
   return last_error();

@@ -112,8 +112,8 @@ rmq_status rmq_query(const rmq_hierarchy* h, const uint32_t* d_lr, uint64_t q,
  * src/tools.h:27-31), 16-byte aligned; indices out as uint64_t. Works on every hierarchy; it is
  * the only query of a 64-bit-form (chunked) one, on which rmq_query returns RMQ_EINVAL. A query
  * across chunks is answered from its first and last chunk's hierarchies and the minima of the
- * chunks in between, with the same leftmost rule. The first call on a one-piece hierarchy
- * allocates a small device table for it. */
+ * chunks in between, with the same leftmost rule. Allocates nothing; concurrent calls on
+ * different streams are allowed, as for rmq_query. */
 rmq_status rmq_query64(const rmq_hierarchy* h, const uint64_t* d_lr, uint64_t q,
                        float* d_val, uint64_t* d_idx, void* stream);
 
