@@ -628,14 +628,14 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
   // side (holding 2 x R/4 vectors would cost the occupancy the overlap is meant to buy)
   constexpr bool kEarly = V <= 2;
   constexpr uint32_t VE = kEarly ? V : 1;
+  // each vector is read below only under the condition it was loaded under
   float4 xl[VE], xr[VE];
   if constexpr (kEarly) {
 #pragma unroll
     for (uint32_t s = 0; s < V; ++s) {
       const uint32_t pl = lbase + 4u * s, pr = rlo + 4u * s;
-      xl[s] = la && (lo & (R - 1u)) != 0u && pl + 4u > lo ? load_level(vk, pl, szk, uk, H.tail)
-                                                        : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
-      xr[s] = ra && pr < hi ? load_level(vk, pr, szk, uk, H.tail) : make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+      if (la && (lo & (R - 1u)) != 0u && pl + 4u > lo) xl[s] = load_level(vk, pl, szk, uk, H.tail);
+      if (ra && pr < hi) xr[s] = load_level(vk, pr, szk, uk, H.tail);
     }
   }
   if (la) {
