@@ -322,6 +322,49 @@ def test_config3_large_array(device, lr_mode):
     torch.cuda.empty_cache()
 
 
+def test_array_at_int32_limit(device):
+    """n = 2^31 - 1 (the reference's largest int32 n; 8 GiB), one piece: level-0 spans of upper
+    entries reach 2^31, so the winner codes and span keys use every bit below the top one.
+    Queries: mixed lengths and ranges ending at n - 1; short ones against GPU BASE, long ones
+    against torch.argmin (first occurrence), every answer inside its range with value ==
+    A[index]."""
+    import torch
+
+    _CACHE.clear()
+    n = (1 << 31) - 1
+    _say("int32 limit: fill")
+    dA = torch.empty(n, dtype=torch.float32, device="cuda")
+    rmq_amd.fill_uniform(dA, seed=17)  # 24-bit values: long ranges hold ties at their minimum
+    lr = rmq_amd.gen_queries(1 << 16, -6, n, 17)
+    edge = np.array([[0, n - 1], [n - 1, n - 1], [n - 5, n - 1], [n - 70000, n - 1], [1 << 30, n - 1],
+                     [(1 << 30) - 3, (1 << 30) + 3], [n - (1 << 24), n - 2]], dtype=np.uint32)
+    lr = np.concatenate([edge, lr])
+    dQ = torch.from_numpy(lr.view(np.int32)).cuda()
+    short = np.nonzero((lr[:, 1].astype(np.int64) - lr[:, 0]) < (1 << 20))[0]
+    sq = torch.from_numpy(np.ascontiguousarray(lr[short]).view(np.int32)).cuda()
+    _, base_i = rmq_amd.gpu_base(dA, sq, False, True)
+    torch.cuda.synchronize()
+    base_i = base_i.cpu().numpy().view(np.uint32)
+    lo = torch.from_numpy(lr[:, 0].astype(np.int64)).cuda()
+    hi = torch.from_numpy(lr[:, 1].astype(np.int64)).cuda()
+    for alg, opts, t in ((20, dict(cg_amount_log=0), 8192), (16, {}, 1 << 15), (16, dict(group_lanes=4), 64),
+                         (20, dict(cg_amount_log=1), 1 << 17)):
+        _say(f"int32 limit alg={alg} opts={opts} t={t}")
+        with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
+            v, i = h.query(dQ, True, True)
+            torch.cuda.synchronize()
+        i64 = i.to(torch.int64) & 0xFFFFFFFF
+        assert bool(((i64 >= lo) & (i64 <= hi)).all()), (alg, opts)
+        assert torch.equal(v.view(torch.int32), dA[i64].view(torch.int32)), (alg, opts)
+        ih = i.cpu().numpy().view(np.uint32)
+        assert np.array_equal(ih[short], base_i), (alg, opts)
+        for k in range(edge.shape[0]):
+            l, r = int(lr[k, 0]), int(lr[k, 1])
+            assert int(ih[k]) == l + int(torch.argmin(dA[l:r + 1])), (alg, opts, k, l, r)
+    del dA
+    torch.cuda.empty_cache()
+
+
 def test_streams_and_reuse(device):
     """One resident hierarchy, queried concurrently on two streams and repeatedly."""
     import torch
