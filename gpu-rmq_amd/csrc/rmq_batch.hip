@@ -23,7 +23,7 @@ using rmq::kBlock;
 
 namespace {
 
-constexpr uint32_t kBuildTile = 1024;    // level-(k+1) entries per build tile
+constexpr uint32_t kBuildTile = 4096;    // level-(k+1) entries per build tile
 
 struct BuildTile {
   uint32_t array;
