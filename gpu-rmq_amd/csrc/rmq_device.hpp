@@ -201,7 +201,7 @@ __device__ __forceinline__ void publish_zeros(uint32_t* zeros, uint32_t bits) {
 }
 
 // Final ranges on the top level longer than this are answered from the sparse table.
-constexpr uint32_t kTopScan = 32;
+constexpr uint32_t kTopScan = 8;
 
 // Minimum of top-level entries [a, b), b - a >= 2, from the sparse table: the entry i of row
 // j >= 1 (at tt + (j - 1) * stride) holds {value bits, leftmost position} of the minimum of

@@ -78,7 +78,7 @@ typedef struct {
                                        order instead of the pruned order (same answers) */
 /* A build also keeps a sparse table over the top level (m entries): floor(log2 m) rows of m
  * 8-byte {value, position} entries, built when m >= 64 and the table is at most 64 MiB, and not
- * with RMQ_BUILD_FULL. A query whose remaining range on the top level is longer than 32 entries
+ * with RMQ_BUILD_FULL. A query whose remaining range on the top level is longer than 8 entries
  * reads two table entries instead of scanning it, so large t no longer costs long scans (the
  * reference default t = 2^15 with uniform ranges). The flag below builds none (same answers). */
 #define RMQ_BUILD_NO_TOP_TABLE 0x4u

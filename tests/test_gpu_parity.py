@@ -148,7 +148,7 @@ def test_every_depth(device, t):
 def test_top_table(device, alg, opts):
     """Large t leaves long final ranges on the top level, which the sparse table over it answers:
     the reference default t = 2^15 with uniform lengths, a one-level hierarchy (t >= n, the table
-    over the array itself) and a small top level near the 32-entry scan limit. Values are
+    over the array itself) and a small top level (about 100 entries). Values are
     quantised (many ties) with both zero signs planted; answers are the same with and without
     the table and equal to the oracle's."""
     rng = np.random.default_rng(alg * 7 + len(opts))
