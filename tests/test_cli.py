@@ -68,6 +68,20 @@ def test_rand_trivial_check_and_paper_constants(tmp_path, device):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("alg", [16, 20, 24, 116])
+def test_reference_default_case(alg, tmp_path, device):
+    """The reference's own small case with its defaults: n = q = 2^20, uniform lengths (lr = -1),
+    no --log_bs (t = 2^15, so long ranges end on a top level of thousands of entries), checked
+    on 2^15 random queries against GPU BASE."""
+    csv_file = tmp_path / "data.csv"
+    p = run(1 << 20, 1 << 20, -1, alg, "--seed", 27722, "--reps", 2, "--randTrivialCheck",
+            f"--save-time={csv_file}")
+    assert p.returncode == 0 and "pass" in p.stdout, p.stdout + p.stderr
+    rows = list(csv.reader(csv_file.read_text().splitlines()))
+    assert rows[1][13] == str(1 << 15) and rows[1][-1] == "5"  # pass (1) + 4 for the random check
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("alg", [20, 120, 116, 124, 102])
 def test_long_positions_check(alg, tmp_path, device):
     """--long: the reference's IS_LONG build (64-bit {l, r} pairs and indices) on an array below
