@@ -579,6 +579,41 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
   } else {
     // guards lk - 1 and ek: in the first / last vector the scan reads unless 4-aligned
     uint32_t p = lk & ~3u;
+    // R <= 8: a range below the top (at most 2R - 2 entries) or on a top level with a table (at
+    // most kTopScan) spans at most NV vectors, all loaded before any is folded; a wavefront pays
+    // for its longest range either way, so this trades a dependent loop for parallel loads.
+    // (Not in kernels without the table path: the batched kernel has no registers to spare.)
+    constexpr uint32_t kScanMax = 2 * (1u << RL) - 2 > dev::kTopScan ? 2 * (1u << RL) - 2 : dev::kTopScan;
+    constexpr uint32_t NV = (kScanMax + 3 + 3) / 4;
+    if (TOP && RL <= 3 && ek - lk <= kScanMax) {
+      const float ga = left_tail && (lk & 3u) == 0 ? vK[lk - 1u] : RMQ_INF;
+      const float gb = right_tail && (ek & 3u) == 0 ? vK[ek] : RMQ_INF;
+      float4 xv[NV];
+#pragma unroll
+      for (uint32_t v = 0; v < NV; ++v)
+        if (p + 4u * v < ek) xv[v] = load_level(vK, p + 4u * v, szK, K, H.tail);
+      if (left_tail) gL = (lk & 3u) ? dev::pick4(xv[0], (lk - 1u) & 3u) : ga;
+#pragma unroll
+      for (uint32_t v = 0; v < NV; ++v) {
+        const uint32_t pv = p + 4u * v;
+        if (pv < ek) {
+          if constexpr (IDX) {
+            const float xs[4] = {xv[v].x, xv[v].y, xv[v].z, xv[v].w};
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+              const bool t = dev::in_range(pv + j, lk, ek) && xs[j] < bv;
+              bv = t ? xs[j] : bv;
+              bpay = t ? pb + ((pv + j) << shK) : bpay;
+            }
+          } else {
+            bv = dev::fold4(bv, xv[v], pv, lk, ek);
+          }
+          if (right_tail && (ek & 3u) != 0 && ek - pv < 4u) gR = dev::pick4(xv[v], ek & 3u);
+        }
+      }
+      if (right_tail && (ek & 3u) == 0) gR = gb;
+      return VlState{l, e, bv, bpay, static_cast<int>(K) - 1, left_tail && gL <= bv, right_tail && gR < bv};
+    }
     float4 x = load_level(vK, p, szK, K, H.tail);
     if (left_tail) gL = (lk & 3u) ? dev::pick4(x, (lk - 1u) & 3u) : vK[lk - 1u];
     for (;;) {
