@@ -29,9 +29,10 @@ for _p in (ROOT, os.path.join(ROOT, "gpu-rmq_amd")):
 METRIC = "Gqueries/s + %HBM roofline at n=2^26,q=2^26; batched vs plain; serve p99"
 SEED = 27722
 LR_NAMES = {-1: "uniform-large", -2: "lognormal-medium", -3: "lognormal-small", -6: "mixed"}
-# The configuration timed when the sweep finds nothing more than 2% faster (lane per query,
-# R = 4, t = 2^13): keeps the timed kernel the one whose counters are committed under profiles/.
-DEFAULT_ALG, DEFAULT_T = "20/c0", 1 << 13
+# The configuration timed when the sweep finds nothing more than 2% faster (lane per query with
+# packed descents, R = 4, t = 2^13): keeps the timed kernel the one whose counters are committed
+# under profiles/.
+DEFAULT_ALG, DEFAULT_T = "20/c0/p", 1 << 13
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
@@ -54,8 +55,8 @@ def parse():
     ap.add_argument("--log-n", type=int, default=26)
     ap.add_argument("--log-q", type=int, default=26)
     ap.add_argument("--lr", type=int, default=-2)
-    ap.add_argument("--alg", default="", help="16, 16/g1, 16/g4, 16/g8, 16/g64, 24/g4, 24/g64, 20, 20/c0 or 20/c1 "
-                                              "(empty = sweep)")
+    ap.add_argument("--alg", default="", help="16, 16/g1, 16/g4, 16/g8, 16/g64, 24/g4, 24/g64, 20, 20/c0, 20/c1, "
+                                              "or 20/c0/p, 20/c1/p, 16/p (packed descents) (empty = sweep)")
     ap.add_argument("--t", type=int, default=0, help="scan threshold (0 = sweep)")
     ap.add_argument("--mode", choices=["index", "value", "both"], default="index")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -342,7 +343,9 @@ def main():
     opts_of = {"16": {}, "16/g4": dict(group_lanes=4), "24/g4": dict(group_lanes=4), "20": {},
                "20/c0": dict(cg_amount_log=0), "20/c1": dict(cg_amount_log=1),
                "16/g64": dict(group_lanes=64), "24/g64": dict(group_lanes=64),
-               "16/g1": dict(group_lanes=1), "16/g8": dict(group_lanes=8)}
+               "16/g1": dict(group_lanes=1), "16/g8": dict(group_lanes=8),
+               "20/c0/p": dict(cg_amount_log=0, pack=True), "20/c1/p": dict(cg_amount_log=1, pack=True),
+               "16/p": dict(pack=True)}
     if args.alg and args.t:
         candidates = [(args.alg, args.t)]
     elif args.no_sweep:
@@ -352,6 +355,8 @@ def main():
         candidates = [(v, 1 << tl) for v in ("16", "16/g4", "16/g8", "24/g4") for tl in (6, 10, 15, 18)]
         candidates += [(v, 1 << tl) for v in ("20", "20/c1", "20/c0") for tl in (6, 10, 13, 15)]
         candidates += [(v, 1 << 18) for v in ("16/g64", "24/g64")]  # a wave per query, long scans
+        # the lane solver with its descents packed onto full wavefronts
+        candidates += [(v, 1 << tl) for v in ("20/c0/p", "20/c1/p", "16/p") for tl in (10, 13, 15)]
     sweep = []
     best = None
     for var, t in candidates:

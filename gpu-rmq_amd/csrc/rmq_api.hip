@@ -280,7 +280,7 @@ rmq_status launch_cl_g(uint32_t G, bool full, int alg, const uint2* Q, uint32_t 
 }
 
 template <int RL>
-rmq_status launch_vl(bool full, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
+rmq_status launch_vl(bool full, bool packed, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
                      const HierView& H, hipStream_t s) {
   const uint32_t grid = grid_for(q);
   if (static_cast<uint64_t>(grid) * kBlock < q) return RMQ_EINVAL;
@@ -288,8 +288,13 @@ rmq_status launch_vl(bool full, const uint2* Q, uint32_t q, float* ov, uint32_t*
     if (oi) rmq::k_query_vl<RL, false, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
     else rmq::k_query_vl<RL, false, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
   } else {
-    if (oi) rmq::k_query_vl<RL, true, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-    else rmq::k_query_vl<RL, true, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    if (packed) {
+      if (oi) rmq::k_query_vlc<RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+      else rmq::k_query_vlc<RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    } else {
+      if (oi) rmq::k_query_vl<RL, true, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+      else rmq::k_query_vl<RL, true, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    }
   }
   return last_error();
 }
@@ -297,22 +302,23 @@ rmq_status launch_vl(bool full, const uint2* Q, uint32_t q, float* ov, uint32_t*
 rmq_status query_dispatch(const rmq_hierarchy* h, const uint2* Q, uint32_t q, float* ov,
                           uint32_t* oi, hipStream_t s) {
   const HierView& H = h->view;
+  const bool packed = (h->flags & RMQ_BUILD_PACK_DESCENTS) != 0;
   if (h->alg == RMQ_ALG_VECTOR_LOAD) {
     switch (h->R_log) {
-      case 2: return launch_vl<2>((h->flags & RMQ_BUILD_FULL) != 0, Q, q, ov, oi, H, s);
-      case 3: return launch_vl<3>((h->flags & RMQ_BUILD_FULL) != 0, Q, q, ov, oi, H, s);
-      case 4: return launch_vl<4>((h->flags & RMQ_BUILD_FULL) != 0, Q, q, ov, oi, H, s);
-      case 5: return launch_vl<5>((h->flags & RMQ_BUILD_FULL) != 0, Q, q, ov, oi, H, s);
-      case 6: return launch_vl<6>((h->flags & RMQ_BUILD_FULL) != 0, Q, q, ov, oi, H, s);
+      case 2: return launch_vl<2>((h->flags & RMQ_BUILD_FULL) != 0, packed, Q, q, ov, oi, H, s);
+      case 3: return launch_vl<3>((h->flags & RMQ_BUILD_FULL) != 0, packed, Q, q, ov, oi, H, s);
+      case 4: return launch_vl<4>((h->flags & RMQ_BUILD_FULL) != 0, packed, Q, q, ov, oi, H, s);
+      case 5: return launch_vl<5>((h->flags & RMQ_BUILD_FULL) != 0, packed, Q, q, ov, oi, H, s);
+      case 6: return launch_vl<6>((h->flags & RMQ_BUILD_FULL) != 0, packed, Q, q, ov, oi, H, s);
       default: return RMQ_EUNSUPPORTED;
     }
   }
   if (h->group_lanes == 1) {  // algs 16 / 24 with a lane per query (R <= 64, pruned order)
     switch (h->R_log) {
-      case 3: return launch_vl<3>(false, Q, q, ov, oi, H, s);
-      case 4: return launch_vl<4>(false, Q, q, ov, oi, H, s);
-      case 5: return launch_vl<5>(false, Q, q, ov, oi, H, s);
-      case 6: return launch_vl<6>(false, Q, q, ov, oi, H, s);
+      case 3: return launch_vl<3>(false, packed, Q, q, ov, oi, H, s);
+      case 4: return launch_vl<4>(false, packed, Q, q, ov, oi, H, s);
+      case 5: return launch_vl<5>(false, packed, Q, q, ov, oi, H, s);
+      case 6: return launch_vl<6>(false, packed, Q, q, ov, oi, H, s);
       default: return RMQ_EUNSUPPORTED;
     }
   }

@@ -768,4 +768,79 @@ __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q
   if (IDX && out_i) out_i[qi] = a.idx;
 }
 
+// Answer of a finished lane-solver state, written to query qi (value mode re-solves a zero of
+// unknown sign).
+template <int RL, bool IDX, class LT>
+__device__ __forceinline__ void vl_emit(const VlState& st, uint64_t qi, const HierView& H, const LT& T,
+                                        uint32_t zeros, float* __restrict__ out_v,
+                                        uint32_t* __restrict__ out_i) {
+  GroupAnswer a = vl_finish<RL, IDX>(st, H, T);
+  if constexpr (!IDX) {
+    if (a.v == 0.0f && !dev::zero_known(zeros, a.v)) a = solve_vl_pruned<RL, true>(st.l, st.e, H, T);
+  }
+  if (out_v) out_v[qi] = a.v;
+  if (IDX && out_i) out_i[qi] = a.idx;
+}
+
+// The pruned lane solver with its descents packed. k_query_vl's wavefront walks the tails until
+// its deepest lane is done, though most queries need at most one level (lognormal lengths at
+// R = 4: 0.7 levels per query on average, 3.8 for the deepest of 64). Here every lane reads its
+// final range and walks one level; the queries still descending are written to LDS, and after
+// the workgroup's barrier the first threads take them up in wavefront order, so the deep walks
+// run on as few wavefronts as there are such queries in groups of 64. A lane answers its query
+// as soon as it is done. The barrier keeps finished wavefronts resident, which costs a few
+// percent where few queries descend (uniform or short lengths).
+template <int RL, bool IDX>
+__global__ __launch_bounds__(kBlock) void k_query_vlc(const uint2* __restrict__ Q, uint32_t q,
+                                                      float* __restrict__ out_v,
+                                                      uint32_t* __restrict__ out_i, HierView H) {
+  constexpr uint32_t kWaves = kBlock / 64;
+  __shared__ uint32_t s_meta[kMetaWords];
+  __shared__ uint4 s_st[kBlock];   // l, e, best value bits, best payload
+  __shared__ uint2 s_aux[kBlock];  // next level | la << 8 | ra << 9, query index
+  __shared__ uint32_t s_cnt[kWaves];
+  if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
+  __syncthreads();
+  const LevelsLds T{s_meta};
+  const uint32_t zeros = IDX ? 0u : *H.zeros;
+  const uint32_t w = threadIdx.x >> 6;
+  const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  bool need = false;
+  VlState st{};
+  if (qi < q) {
+    const uint2 p = dev::load_query(Q, qi);
+    st = vl_start<RL, IDX, true>(p.x, p.y + 1, H, T);
+    if (st.la || st.ra) vl_step<RL, IDX>(st, H, T);
+    need = st.la || st.ra;  // k >= 0 whenever a tail is open
+    if (!need) vl_emit<RL, IDX>(st, qi, H, T, zeros, out_v, out_i);
+  }
+  const uint64_t m = __ballot(need);
+  if (need) {
+    const uint32_t slot = w * 64u + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+    s_st[slot] = make_uint4(st.l, st.e, dev::f2u(st.bv), st.bpay);
+    s_aux[slot] = make_uint2(static_cast<uint32_t>(st.k) | (st.la ? 0x100u : 0u) | (st.ra ? 0x200u : 0u),
+                             static_cast<uint32_t>(qi));
+  }
+  if ((threadIdx.x & 63u) == 0u) s_cnt[w] = static_cast<uint32_t>(__popcll(m));
+  __syncthreads();
+  // thread j takes the j-th parked query: wave v's entries follow those of waves 0..v-1
+  uint32_t r = threadIdx.x, slot = kBlock;
+#pragma unroll
+  for (uint32_t v = 0; v < kWaves; ++v) {
+    const uint32_t c = s_cnt[v];
+    if (slot == kBlock) {
+      if (r < c) slot = v * 64u + r;
+      else r -= c;
+    }
+  }
+  if (slot == kBlock) return;
+  const uint4 a = s_st[slot];
+  const uint2 b = s_aux[slot];
+  VlState ps{a.x, a.y, dev::u2f(a.z), a.w, static_cast<int>(b.x & 0xFFu), (b.x & 0x100u) != 0u,
+             (b.x & 0x200u) != 0u};
+  while (ps.la || ps.ra) vl_step<RL, IDX>(ps, H, T);
+  vl_emit<RL, IDX>(ps, b.y, H, T, zeros, out_v, out_i);
+}
+
 }  // namespace rmq

@@ -54,16 +54,17 @@ class Plan:
 
 
 def _options(cg_size_log, cg_amount_log, group_lanes, full=False, chunk_log=0,
-             top_table=True) -> Optional[BuildOptions]:
+             top_table=True, pack=False) -> Optional[BuildOptions]:
     if (cg_size_log is None and cg_amount_log is None and not group_lanes and not full and not chunk_log
-            and top_table):
+            and top_table and not pack):
         return None
     o = BuildOptions()
     o.cg_size_log = 3 if cg_size_log is None else int(cg_size_log)
     o.cg_amount_log = 2 if cg_amount_log is None else int(cg_amount_log)
     o.group_lanes = int(group_lanes or 0)
     o.flags = (_lib.RMQ_BUILD_DEFAULT_CG | (_lib.RMQ_BUILD_FULL if full else 0) |
-               (0 if top_table else _lib.RMQ_BUILD_NO_TOP_TABLE) | _lib.RMQ_BUILD_CHUNK_LOG(chunk_log))
+               (0 if top_table else _lib.RMQ_BUILD_NO_TOP_TABLE) |
+               (_lib.RMQ_BUILD_PACK_DESCENTS if pack else 0) | _lib.RMQ_BUILD_CHUNK_LOG(chunk_log))
     return o
 
 
@@ -154,7 +155,7 @@ class Hierarchy:
 
     def __init__(self, array, t: int, alg: int = RMQ_ALG_XXX, cg_size_log=None,
                  cg_amount_log=None, group_lanes: int = 0, full: bool = False, chunk_log: int = 0,
-                 top_table: bool = True, stream=None):
+                 top_table: bool = True, pack: bool = False, stream=None):
         torch = _torch()
         _check_device_tensor(array, "array", (torch.float32,))
         if array.dim() != 1:
@@ -164,7 +165,7 @@ class Hierarchy:
         self.t = int(t)
         self.alg = int(alg)
         self._h = ctypes.c_void_p()
-        o = _options(cg_size_log, cg_amount_log, group_lanes, full, chunk_log, top_table)
+        o = _options(cg_size_log, cg_amount_log, group_lanes, full, chunk_log, top_table, pack)
         st = _lib.lib().rmq_build_ex(array.data_ptr(), self.n, self.t, self.alg, _opt_ptr(o),
                                      _stream_handle(stream), ctypes.byref(self._h))
         _lib.check(st, "rmq_build")

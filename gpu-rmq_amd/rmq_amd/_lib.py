@@ -27,6 +27,7 @@ RMQ_MAX_LEVELS = 20
 RMQ_BUILD_DEFAULT_CG = 0x1
 RMQ_BUILD_FULL = 0x2
 RMQ_BUILD_NO_TOP_TABLE = 0x4
+RMQ_BUILD_PACK_DESCENTS = 0x8
 
 
 def RMQ_BUILD_CHUNK_LOG(k: int) -> int:

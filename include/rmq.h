@@ -82,6 +82,14 @@ typedef struct {
  * reads two table entries instead of scanning it, so large t no longer costs long scans (the
  * reference default t = 2^15 with uniform ranges). The flag below builds none (same answers). */
 #define RMQ_BUILD_NO_TOP_TABLE 0x4u
+/* Lane-per-query solver (alg 20; algs 16/24 with group_lanes 1) of rmq_query with its descents
+ * packed: every lane reads its final range and one level of its tails, and the queries that
+ * still descend are regrouped through LDS onto as few wavefronts as they fill, instead of each
+ * wavefront walking until its deepest lane is done. Same answers. Measured at n = q = 2^26:
+ * +10-12% on lognormal-medium lengths, about even on mixed lengths, 2-6% slower on uniform and
+ * short lengths (the workgroup barrier keeps finished wavefronts resident). Not with
+ * RMQ_BUILD_FULL; the other solvers and rmq_query64 ignore it. */
+#define RMQ_BUILD_PACK_DESCENTS 0x8u
 /* Chunk size (log2, 8..30) of the 64-bit form for this build; arrays of more than 2^31 - 1
  * elements use 2^30 without it. Setting it holds even a small array in chunks, which is how the
  * 64-bit form is tested on small inputs. */

@@ -41,6 +41,13 @@ VARIANTS = [
     (20, dict(cg_amount_log=4)),                             # R=64
     (16, dict(full=True)),                                 # level loop instead of up-front loads
     (24, dict(full=True, group_lanes=4)),
+    # descents packed onto full wavefronts (lane solver), and ignored by the other solvers
+    (20, dict(cg_amount_log=0, pack=True)),
+    (20, dict(cg_amount_log=1, pack=True)),
+    (20, dict(cg_amount_log=4, pack=True)),
+    (16, dict(pack=True)),
+    (24, dict(group_lanes=1, cg_size_log=3, cg_amount_log=3, pack=True)),
+    (16, dict(group_lanes=4, pack=True)),
 ]
 
 
@@ -234,7 +241,7 @@ def test_config1_plain_value(device):
     ref_v, ref_i = oracle.CpuHierarchy(A, 1 << 15, 5, 16).query(lr)
     sample = rmq_amd.gen_sample_indices(q, 1 << 12, SEED)
     assert np.array_equal(oracle.cpu_base_index(A, lr[sample]), ref_i[sample])
-    for alg, opts in VARIANTS[:5]:
+    for alg, opts in VARIANTS[:5] + [(20, dict(cg_amount_log=1, pack=True))]:
         _say(f"config1 alg={alg} opts={opts}")
         v, _ = run_gpu(A, lr, alg, 1 << 15, True, False, **opts)
         assert_exact(A, lr, v, None, ref_i, f"config1 alg={alg} {opts}")
@@ -247,8 +254,8 @@ def test_config2_index_sampled_full_n(device, t_log):
     A = _array_2_26()
     lr = _queries_2_26()[: 1 << 22]
     ref_v, ref_i = oracle.CpuHierarchy(A, 1 << 15, 5, 16).query(lr)
-    for alg, opts in ((16, {}), (24, {}), (20, {})):
-        _say(f"config2 alg={alg} t=2^{t_log}")
+    for alg, opts in ((16, {}), (24, {}), (20, {}), (20, dict(cg_amount_log=0, pack=True))):
+        _say(f"config2 alg={alg} opts={opts} t=2^{t_log}")
         v, i = run_gpu(A, lr, alg, 1 << t_log, True, True, **opts)
         assert_exact(A, lr, v, i, ref_i, f"config2 alg={alg} t=2^{t_log}")
     sample = rmq_amd.gen_sample_indices(lr.shape[0], 1 << 13, SEED)
@@ -265,16 +272,18 @@ def test_config2_full_q_properties(device):
     dA = torch.from_numpy(A).cuda()
     dQ = torch.from_numpy(lr.view(np.int32)).cuda()
     outs = {}
-    for alg in (16, 24, 20):
-        _say(f"config2 full q alg={alg}")
-        with rmq_amd.Hierarchy(dA, 1 << 15, alg) as h:
+    # the last: the timed headline configuration (R = 4, t = 2^13, packed descents)
+    for key, alg, t, opts in ((16, 16, 1 << 15, {}), (24, 24, 1 << 15, {}), (20, 20, 1 << 15, {}),
+                              ("20p", 20, 1 << 13, dict(cg_amount_log=0, pack=True))):
+        _say(f"config2 full q alg={alg} t={t} opts={opts}")
+        with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
             v, i = h.query(dQ, True, True)
             torch.cuda.synchronize()
-        outs[alg] = (v.cpu().numpy(), i.cpu().numpy().view(np.uint32))
+        outs[key] = (v.cpu().numpy(), i.cpu().numpy().view(np.uint32))
     v16, i16 = outs[16]
-    for alg in (24, 20):
-        assert np.array_equal(outs[alg][1], i16)
-        assert np.array_equal(outs[alg][0].view(np.uint32), v16.view(np.uint32))
+    for key in (24, 20, "20p"):
+        assert np.array_equal(outs[key][1], i16), key
+        assert np.array_equal(outs[key][0].view(np.uint32), v16.view(np.uint32)), key
     assert (i16 >= lr[:, 0]).all() and (i16 <= lr[:, 1]).all()
     assert np.array_equal(A[i16].view(np.uint32), v16.view(np.uint32))
     sample = rmq_amd.gen_sample_indices(lr.shape[0], 1 << 13, SEED + 1)
