@@ -593,6 +593,8 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
       for (uint32_t v = 0; v < NV; ++v)
         if (p + 4u * v < ek) xv[v] = load_level(vK, p + 4u * v, szK, K, H.tail);
       if (left_tail) gL = (lk & 3u) ? dev::pick4(xv[0], (lk - 1u) & 3u) : ga;
+      // index mode: the winner's offset from p (a constant per element), its payload once after
+      uint32_t bo = lk - p;
 #pragma unroll
       for (uint32_t v = 0; v < NV; ++v) {
         const uint32_t pv = p + 4u * v;
@@ -603,7 +605,7 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
             for (uint32_t j = 0; j < 4; ++j) {
               const bool t = dev::in_range(pv + j, lk, ek) && xs[j] < bv;
               bv = t ? xs[j] : bv;
-              bpay = t ? pb + ((pv + j) << shK) : bpay;
+              bo = t ? 4u * v + j : bo;
             }
           } else {
             bv = dev::fold4(bv, xv[v], pv, lk, ek);
@@ -611,6 +613,7 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
           if (right_tail && (ek & 3u) != 0 && ek - pv < 4u) gR = dev::pick4(xv[v], ek & 3u);
         }
       }
+      if constexpr (IDX) bpay = pb + ((p + bo) << shK);
       if (right_tail && (ek & 3u) == 0) gR = gb;
       return VlState{l, e, bv, bpay, static_cast<int>(K) - 1, left_tail && gL <= bv, right_tail && gR < bv};
     }
@@ -673,6 +676,10 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
       if (ra && pr < hi) xr[s] = load_level(vk, pr, szk, uk, H.tail);
     }
   }
+  // index mode: the winning element's slot (a constant per element; R + slot on the right),
+  // its payload formed once at the end of the step
+  constexpr uint32_t kNoSlot = 2u * R;
+  uint32_t wslot = kNoSlot;
   if (la) {
     float g = RMQ_INF;
 #pragma unroll
@@ -686,7 +693,7 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
         for (int j = 3; j >= 0; --j) {
           const bool t = p + static_cast<uint32_t>(j) >= lo && xs[j] <= bv;
           bv = t ? xs[j] : bv;
-          if constexpr (IDX) bpay = t ? pb + ((p + static_cast<uint32_t>(j)) << sk) : bpay;
+          if constexpr (IDX) wslot = t ? 4u * static_cast<uint32_t>(s) + static_cast<uint32_t>(j) : wslot;
         }
       }
     }
@@ -710,7 +717,7 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
         for (uint32_t j = 0; j < 4; ++j) {
           const bool t = p + j < hi && xs[j] < bv;
           bv = t ? xs[j] : bv;
-          if constexpr (IDX) bpay = t ? pb + ((p + j) << sk) : bpay;
+          if constexpr (IDX) wslot = t ? R + 4u * s + j : wslot;
         }
       }
     }
@@ -720,6 +727,9 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
       if ((hi & 3u) == 0) g = vk[hi];
       ra = g < bv;
     }
+  }
+  if constexpr (IDX) {
+    if (wslot != kNoSlot) bpay = pb + ((wslot < R ? lbase + wslot : rlo + (wslot - R)) << sk);
   }
   st.bv = bv;
   st.bpay = bpay;
