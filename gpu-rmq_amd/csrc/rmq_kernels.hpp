@@ -798,8 +798,9 @@ __device__ __forceinline__ void vl_emit(const VlState& st, uint64_t qi, const Hi
 // final range and walks one level; the queries still descending are written to LDS, and after
 // the workgroup's barrier the first threads take them up in wavefront order, so the deep walks
 // run on as few wavefronts as there are such queries in groups of 64. A lane answers its query
-// as soon as it is done. The barrier keeps finished wavefronts resident, which costs a few
-// percent where few queries descend (uniform or short lengths).
+// as soon as it is done. Where few queries descend (uniform lengths) or most descend only a
+// little (short lengths), the parking, the barrier and the parked queries' scattered stores
+// are not repaid: a few percent slower there.
 template <int RL, bool IDX>
 __global__ __launch_bounds__(kBlock) void k_query_vlc(const uint2* __restrict__ Q, uint32_t q,
                                                       float* __restrict__ out_v,

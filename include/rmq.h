@@ -87,8 +87,8 @@ typedef struct {
  * still descend are regrouped through LDS onto as few wavefronts as they fill, instead of each
  * wavefront walking until its deepest lane is done. Same answers. Measured at n = q = 2^26:
  * +10-12% on lognormal-medium lengths, about even on mixed lengths, 2-6% slower on uniform and
- * short lengths (the workgroup barrier keeps finished wavefronts resident). Not with
- * RMQ_BUILD_FULL; the other solvers and rmq_query64 ignore it. */
+ * short lengths, where the regrouping is not repaid. Not with RMQ_BUILD_FULL; the other solvers
+ * and rmq_query64 ignore it. */
 #define RMQ_BUILD_PACK_DESCENTS 0x8u
 /* Chunk size (log2, 8..30) of the 64-bit form for this build; arrays of more than 2^31 - 1
  * elements use 2^30 without it. Setting it holds even a small array in chunks, which is how the
