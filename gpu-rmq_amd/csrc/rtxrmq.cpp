@@ -4,7 +4,7 @@
 //   ./rtxrmq <n> <q> <lr> <alg> [--log_bs L] [--reps R] [--dev D] [--nt T] [--seed S]
 //            [--check | --trivialCheck | --randTrivialCheck] [--save-time[=file]]
 //            [--save-input-data] [--nb B] [--save-power[=file]] [--log_build_threshold[=x]]
-//            [--cg_size_log G] [--cg_amount_log C] [--group_lanes K] [--long]
+//            [--cg_size_log G] [--cg_amount_log C] [--group_lanes K] [--pack_descents] [--long]
 //
 // alg: 16 (XXX / GPU-RMQ CL), 20 (hierarchical vector load / GPU-RMQ VL), 24 (XXX multi load),
 //      2 (GPU BASE); +100 returns leftmost indices instead of values (102, 116, 120, 124).
@@ -13,7 +13,9 @@
 // bit-exactly (value bit patterns, indices) instead of |a-b| < 1e-4 (src/tools.h:261-264).
 // The runtime flags --cg_size_log / --cg_amount_log replace the reference's compile-time
 // XXX_CG_SIZE_LOG / XXX_CG_AMOUNT_LOG (CMakeLists.txt:61-67); --group_lanes picks this
-// library's lanes-per-query for algs 16/24 (0 = auto; never changes results).
+// library's lanes-per-query for algs 16/24 (0 = auto; never changes results); --pack_descents
+// sets RMQ_BUILD_PACK_DESCENTS (the lane solver regroups the queries still descending onto full
+// wavefronts; same results).
 // --long is the reference's IS_LONG build (CMakeLists.txt:57-59, src/main.cu:21-27): 64-bit
 // positions, {l, r} pairs and indices; it is implied for n > 2^31 - 1 (up to 2^38).
 
@@ -46,7 +48,7 @@ constexpr int kTrivLog = 10;      // TRIVCHECK_LOG_NUM_DIFFQ, src/main.cu:30
 enum {
   ARG_BS = 1, ARG_NB, ARG_REPS, ARG_DEV, ARG_NT, ARG_SEED, ARG_CHECK, ARG_TIME, ARG_POWER,
   ARG_TRIVIALCHECK, ARG_THRESHOLD, ARG_SAVE_INPUTDATA, ARG_RAND_TRIVIALCHECK,
-  ARG_CG_SIZE, ARG_CG_AMOUNT, ARG_GROUP_LANES, ARG_LONG
+  ARG_CG_SIZE, ARG_CG_AMOUNT, ARG_GROUP_LANES, ARG_LONG, ARG_PACK
 };
 
 const char* alg_name(int alg) {
@@ -94,6 +96,7 @@ void print_help() {
           "   --save-time=<file>        append a CSV line\n"
           "   --save-input-data         cache arrays, queries and results in $RMQ_AUX_DIR\n"
           "   --cg_size_log <g> --cg_amount_log <c> --group_lanes <k>\n"
+          "   --pack_descents           lane solver with descents packed onto full wavefronts\n"
           "   --long                    64-bit positions and queries (IS_LONG; implied for n > 2^31-1)\n");
 }
 
@@ -146,6 +149,7 @@ CmdArgs get_args(int argc, char* argv[]) {
       {"cg_size_log", required_argument, 0, ARG_CG_SIZE},
       {"cg_amount_log", required_argument, 0, ARG_CG_AMOUNT},
       {"group_lanes", required_argument, 0, ARG_GROUP_LANES},
+      {"pack_descents", no_argument, 0, ARG_PACK},
       {"long", no_argument, 0, ARG_LONG},
       {0, 0, 0, 0}};
   int opt, idx;
@@ -176,6 +180,7 @@ CmdArgs get_args(int argc, char* argv[]) {
       case ARG_CG_SIZE: a.cg_size_log = static_cast<uint32_t>(atoi(optarg)); break;
       case ARG_CG_AMOUNT: a.cg_amount_log = static_cast<uint32_t>(atoi(optarg)); break;
       case ARG_GROUP_LANES: a.group_lanes = static_cast<uint32_t>(atoi(optarg)); break;
+      case ARG_PACK: a.pack_descents = 1; break;
       case ARG_LONG: a.is_long = 1; break;
       default: break;
     }

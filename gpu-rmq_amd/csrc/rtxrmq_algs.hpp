@@ -35,6 +35,7 @@ struct CmdArgs {  // src/tools.h:26-35
   uint32_t cg_size_log = 3, cg_amount_log = 2, group_lanes = 0;  // runtime forms of the
                                                                  // compile-time constants
   int is_long = 0;  // IS_LONG (CMakeLists.txt:57-59): --long, or n > 2^31 - 1
+  int pack_descents = 0;  // RMQ_BUILD_PACK_DESCENTS
   std::string time_file, power_file;
 };
 
@@ -125,7 +126,7 @@ inline void* run_hierarchical(int alg, int64_t n, uint64_t t, uint32_t q, const 
   opt.cg_size_log = a.cg_size_log;
   opt.cg_amount_log = a.cg_amount_log;
   opt.group_lanes = a.group_lanes;
-  opt.flags = RMQ_BUILD_DEFAULT_CG;
+  opt.flags = RMQ_BUILD_DEFAULT_CG | (a.pack_descents ? RMQ_BUILD_PACK_DESCENTS : 0u);
   uint32_t R = 0, levels = 0;
   uint64_t buffer_floats = 0;
   rmq_status st = rmq_plan(n, t, alg, &opt, &levels, nullptr, nullptr, &buffer_floats, &R);

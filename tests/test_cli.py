@@ -91,6 +91,17 @@ def test_long_positions_check(alg, tmp_path, device):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("alg", [20, 120, 116])
+def test_pack_descents_check(alg, device):
+    """--pack_descents (the lane solver with its descents packed) passes the bit-exact --check
+    against GPU BASE on lognormal-medium and mixed lengths."""
+    for lr in (-2, -6):
+        p = run(1 << 20, 100000, lr, alg, "--seed", 41, "--reps", 1, "--check", "--log_bs", 10,
+                "--cg_amount_log", 0 if alg % 100 == 20 else 2, "--pack_descents")
+        assert p.returncode == 0 and "pass" in p.stdout, (lr, p.stdout + p.stderr)
+
+
+@pytest.mark.gpu
 def test_long_array_above_2e31(tmp_path, device):
     """n = 2^31 + 7 turns the 64-bit form on by itself; a random sample is checked. Short ranges
     (lr = -3): the check is GPU BASE, one lane scanning each range, so long ranges would take
