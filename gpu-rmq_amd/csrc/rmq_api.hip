@@ -49,6 +49,22 @@ __global__ void k_init_words(uint32_t* __restrict__ w, uint32_t zeros, const flo
   dev::fill_tail(A, n, reinterpret_cast<float*>(w + 4));
 }
 
+// Query validation (RMQ_BUILD_CHECK_QUERIES): *bad becomes nonzero when a query is not
+// 0 <= l <= r < n. k_set_word clears it first (a kernel, so that the pair can be captured).
+__global__ void k_set_word(uint32_t* __restrict__ w, uint32_t v) { *w = v; }
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_check_queries(const P* __restrict__ Q, uint64_t q, uint64_t n,
+                                                          uint32_t* __restrict__ bad) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  bool b = false;
+  if (i < q) {
+    const P p = Q[i];
+    b = !(p.x <= p.y && static_cast<uint64_t>(p.y) < n);
+  }
+  if (__ballot(b) != 0 && (threadIdx.x & 63u) == 0u) atomicOr(bad, 1u);
+}
+
 // Row j >= 1 of the sparse table over the top level top[0, m): entry i (i + 2^j <= m) is the
 // leftmost minimum of [i, i + 2^j), from two entries of row j - 1 (row 1 from the level itself).
 // Entries past m - 2^j are never read and stay unwritten.
@@ -299,6 +315,26 @@ rmq_status launch_vl(bool full, bool packed, const uint2* Q, uint32_t q, float* 
   return last_error();
 }
 
+// RMQ_BUILD_CHECK_QUERIES: every query inside [0, n) with l <= r, checked on the device before
+// any query kernel runs; the word lives in the hierarchy's words (a chunked one uses its first
+// chunk's). Synchronises the stream.
+template <class P>
+rmq_status check_queries(const rmq_hierarchy* h, const P* Q, uint64_t q, hipStream_t s) {
+  uint32_t* w = (h->chunk_log ? h->part[0]->zeros : h->zeros) + 8;
+  const uint64_t grid = (q + kBlock - 1) / kBlock;
+  if (grid > 0xFFFFFFFFull) return RMQ_EINVAL;
+  rmq::k_set_word<<<1, 1, 0, s>>>(w, 0u);
+  rmq::k_check_queries<P><<<static_cast<uint32_t>(grid), kBlock, 0, s>>>(Q, q, h->n, w);
+  if (last_error() != RMQ_OK) return RMQ_EDEVICE;
+  uint32_t bad = 0;
+  if (hipMemcpyAsync(&bad, w, sizeof(bad), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipGetLastError();
+    return RMQ_EDEVICE;
+  }
+  return bad ? RMQ_EINVAL : RMQ_OK;
+}
+
 rmq_status query_dispatch(const rmq_hierarchy* h, const uint2* Q, uint32_t q, float* ov,
                           uint32_t* oi, hipStream_t s) {
   const HierView& H = h->view;
@@ -524,7 +560,8 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
     bytes += (b + 255) & ~255ull;
     return at;
   };
-  const uint64_t at_words = carve(8 * sizeof(uint32_t));
+  // words: [0] zero signs, [4, 8) level 0's last vector, [8] the query check word
+  const uint64_t at_words = carve(16 * sizeof(uint32_t));
   uint64_t at_lv = 0, at_lr = 0;
   if (P.num_levels > 1) {
     // padded to the level alignment past the last level too, so that every 16-byte vector that
@@ -606,6 +643,10 @@ rmq_status rmq_query(const rmq_hierarchy* h, const uint32_t* d_lr, uint64_t q, f
   if (!d_lr || (!d_val && !d_idx)) return RMQ_EINVAL;
   if ((reinterpret_cast<uintptr_t>(d_lr) & 7u) != 0) return RMQ_EINVAL;
   if (q > 0xFFFFFFFFull) return RMQ_EINVAL;
+  if (h->flags & RMQ_BUILD_CHECK_QUERIES) {
+    const rmq_status cs = check_queries(h, reinterpret_cast<const uint2*>(d_lr), q, static_cast<hipStream_t>(stream));
+    if (cs != RMQ_OK) return cs;
+  }
   return query_dispatch(h, reinterpret_cast<const uint2*>(d_lr), static_cast<uint32_t>(q), d_val,
                         d_idx, static_cast<hipStream_t>(stream));
 }
@@ -683,6 +724,10 @@ rmq_status rmq_query64(const rmq_hierarchy* h, const uint64_t* d_lr, uint64_t q,
   if (!d_lr || (!d_val && !d_idx) || (reinterpret_cast<uintptr_t>(d_lr) & 15u) != 0) return RMQ_EINVAL;
   if ((q + kBlock - 1) / kBlock > 0xFFFFFFFFull) return RMQ_EINVAL;
   const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (h->flags & RMQ_BUILD_CHECK_QUERIES) {
+    const rmq_status cs = check_queries(h, reinterpret_cast<const ulonglong2*>(d_lr), q, s);
+    if (cs != RMQ_OK) return cs;
+  }
   // a one-piece hierarchy is its own single chunk: every position is in chunk 0
   const uint32_t clog = h->chunk_log ? h->chunk_log : 63u;
   const uint32_t chunks = h->chunk_log ? h->chunks : 1u;

@@ -28,6 +28,7 @@ RMQ_BUILD_DEFAULT_CG = 0x1
 RMQ_BUILD_FULL = 0x2
 RMQ_BUILD_NO_TOP_TABLE = 0x4
 RMQ_BUILD_PACK_DESCENTS = 0x8
+RMQ_BUILD_CHECK_QUERIES = 0x10
 
 
 def RMQ_BUILD_CHUNK_LOG(k: int) -> int:

@@ -90,6 +90,12 @@ typedef struct {
  * short lengths, where the regrouping is not repaid. Not with RMQ_BUILD_FULL; the other solvers
  * and rmq_query64 ignore it. */
 #define RMQ_BUILD_PACK_DESCENTS 0x8u
+/* Debug validation of queries: rmq_query and rmq_query64 on this hierarchy first check every
+ * query (0 <= l <= r < n) on the device and return RMQ_EINVAL, answering nothing, if one is out
+ * of range (the reference does not check them, src/main.cu:25-26). The check synchronises the
+ * stream and uses one device word of the hierarchy, so checked calls on one hierarchy must not
+ * overlap (a call on another stream could see the other call's result). */
+#define RMQ_BUILD_CHECK_QUERIES 0x10u
 /* Chunk size (log2, 8..30) of the 64-bit form for this build; arrays of more than 2^31 - 1
  * elements use 2^30 without it. Setting it holds even a small array in chunks, which is how the
  * 64-bit form is tested on small inputs. */

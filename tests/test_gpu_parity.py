@@ -414,3 +414,43 @@ def test_error_paths_on_device(device):
         L = rmq_amd.lib()
         q = torch.zeros((4, 2), dtype=torch.int32, device="cuda")
         assert L.rmq_query(h._h, q.data_ptr(), 4, None, None, None) == rmq_amd.RMQ_EINVAL
+
+
+def test_check_queries_mode(device):
+    """RMQ_BUILD_CHECK_QUERIES: valid queries are answered as without it; a set holding one query
+    outside 0 <= l <= r < n is refused with RMQ_EINVAL and nothing is written (the bad query is
+    never handed to a query kernel). Covers rmq_query and rmq_query64 (one piece and chunked)."""
+    import torch
+
+    rng = np.random.default_rng(17)
+    n = 50000
+    A = rng.integers(0, 7, size=n).astype(np.float32)
+    lr = np.sort(rng.integers(0, n, size=(3000, 2)), axis=1).astype(np.uint32)
+    ref_i = oracle.cpu_base_index(A, lr)
+    dA = torch.from_numpy(A).cuda()
+    for alg, opts in ((20, dict(cg_amount_log=0, pack=True)), (16, {}), (24, dict(group_lanes=4))):
+        _say(f"check_queries alg={alg} opts={opts}")
+        with rmq_amd.Hierarchy(dA, _min_t(alg, opts), alg, check_queries=True, **opts) as h:
+            v, i = h.query(torch.from_numpy(lr.view(np.int32)).cuda(), True, True)
+            torch.cuda.synchronize()
+            assert_exact(A, lr, v.cpu().numpy(), i.cpu().numpy().view(np.uint32), ref_i, f"checked alg={alg}")
+            for bad in ([5, 4], [0, n], [n, n], [n - 1, 0xFFFFFFFF]):
+                lb = lr.copy()
+                lb[1234] = bad
+                out_v = torch.full((lb.shape[0],), 7.0, device="cuda")
+                with pytest.raises(rmq_amd.RMQError) as e:
+                    h.query(torch.from_numpy(lb.view(np.int32)).cuda(), True, False, out_val=out_v)
+                assert e.value.status == rmq_amd.RMQ_EINVAL, bad
+                assert bool((out_v == 7.0).all()), "a refused call wrote answers"
+    lr64 = lr.astype(np.uint64)
+    for clog in (0, 12):
+        _say(f"check_queries query64 chunk_log={clog}")
+        with rmq_amd.Hierarchy(dA, 16, 20, cg_amount_log=0, chunk_log=clog, check_queries=True) as h:
+            v, i = h.query64(torch.from_numpy(lr64.view(np.int64)).cuda(), True, True)
+            torch.cuda.synchronize()
+            assert np.array_equal(i.cpu().numpy().view(np.uint64), ref_i.astype(np.uint64))
+            lb = lr64.copy()
+            lb[77] = [0, n]
+            with pytest.raises(rmq_amd.RMQError) as e:
+                h.query64(torch.from_numpy(lb.view(np.int64)).cuda(), True, True)
+            assert e.value.status == rmq_amd.RMQ_EINVAL
