@@ -319,9 +319,9 @@ def test_config3_large_array(device, lr_mode):
     ref_v, ref_i = oracle.CpuHierarchy(A, 64, 5, 16).query(lr)
     dA = torch.from_numpy(A).cuda()
     dQ = torch.from_numpy(lr.view(np.int32)).cuda()
-    for alg, t in ((16, 64), (24, 1 << 15), (20, 32)):
-        _say(f"config3 lr={lr_mode} alg={alg} t={t}")
-        opts = dict(cg_amount_log=1) if alg == 20 else {}
+    for alg, t, opts in ((16, 64, {}), (24, 1 << 15, {}), (20, 32, dict(cg_amount_log=1)),
+                         (20, 1 << 13, dict(cg_amount_log=0, pack=True))):
+        _say(f"config3 lr={lr_mode} alg={alg} t={t} opts={opts}")
         with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
             v, i = h.query(dQ, True, True)
             torch.cuda.synchronize()
@@ -357,7 +357,7 @@ def test_array_at_int32_limit(device):
     lo = torch.from_numpy(lr[:, 0].astype(np.int64)).cuda()
     hi = torch.from_numpy(lr[:, 1].astype(np.int64)).cuda()
     for alg, opts, t in ((20, dict(cg_amount_log=0), 8192), (16, {}, 1 << 15), (16, dict(group_lanes=4), 64),
-                         (20, dict(cg_amount_log=1), 1 << 17)):
+                         (20, dict(cg_amount_log=1), 1 << 17), (20, dict(cg_amount_log=0, pack=True), 8192)):
         _say(f"int32 limit alg={alg} opts={opts} t={t}")
         with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
             v, i = h.query(dQ, True, True)
