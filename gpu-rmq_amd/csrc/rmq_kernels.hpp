@@ -835,23 +835,30 @@ __global__ __launch_bounds__(kBlock) void k_query_vlc(const uint2* __restrict__ 
   }
   if ((threadIdx.x & 63u) == 0u) s_cnt[w] = static_cast<uint32_t>(__popcll(m));
   __syncthreads();
-  // thread j takes the j-th parked query: wave v's entries follow those of waves 0..v-1
-  uint32_t r = threadIdx.x, slot = kBlock;
+  uint32_t c[kWaves], total = 0;
 #pragma unroll
   for (uint32_t v = 0; v < kWaves; ++v) {
-    const uint32_t c = s_cnt[v];
-    if (slot == kBlock) {
-      if (r < c) slot = v * 64u + r;
-      else r -= c;
-    }
+    c[v] = s_cnt[v];
+    total += c[v];
   }
-  if (slot == kBlock) return;
-  const uint4 a = s_st[slot];
-  const uint2 b = s_aux[slot];
-  VlState ps{a.x, a.y, dev::u2f(a.z), a.w, static_cast<int>(b.x & 0xFFu), (b.x & 0x100u) != 0u,
-             (b.x & 0x200u) != 0u};
-  while (ps.la || ps.ra) vl_step<RL, IDX>(ps, H, T);
-  vl_emit<RL, IDX>(ps, b.y, H, T, zeros, out_v, out_i);
+  // thread j takes the j-th parked query: wave v's entries follow those of waves 0..v-1 (one
+  // pass: at most kBlock are parked)
+  for (uint32_t j = threadIdx.x; j < total; j += kBlock) {
+    uint32_t r = j, slot = kBlock;
+#pragma unroll
+    for (uint32_t v = 0; v < kWaves; ++v) {
+      if (slot == kBlock) {
+        if (r < c[v]) slot = v * 64u + r;
+        else r -= c[v];
+      }
+    }
+    const uint4 a = s_st[slot];
+    const uint2 b = s_aux[slot];
+    VlState ps{a.x, a.y, dev::u2f(a.z), a.w, static_cast<int>(b.x & 0xFFu), (b.x & 0x100u) != 0u,
+               (b.x & 0x200u) != 0u};
+    while (ps.la || ps.ra) vl_step<RL, IDX>(ps, H, T);
+    vl_emit<RL, IDX>(ps, b.y, H, T, zeros, out_v, out_i);
+  }
 }
 
 }  // namespace rmq
