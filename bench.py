@@ -309,8 +309,10 @@ def cpu_baseline(A, lr, target_s=12.0):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "design is described in DESIGN.md and is not run.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): queries are "
+              "independent and a hierarchy is read-only, so more GPUs would each answer a share "
+              "of the queries on their own copy, but that form is not built or run here.",
+              file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
