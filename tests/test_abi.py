@@ -112,6 +112,29 @@ def test_plan_of_the_64bit_form():
     assert all(o % 32 == 0 for o in p.offsets[1:])
 
 
+def test_execution_flags_leave_the_plan_unchanged():
+    """RMQ_BUILD_PACK_DESCENTS and RMQ_BUILD_CHECK_QUERIES change how queries run, not the
+    hierarchy: rmq_plan returns the same levels with and without them, for every algorithm."""
+    L = _lib.lib()
+
+    def plan(n, t, alg, c, flags):
+        o = _lib.BuildOptions(3, c, 0, _lib.RMQ_BUILD_DEFAULT_CG | flags)
+        nl, bf, R = ctypes.c_uint32(), ctypes.c_uint64(), ctypes.c_uint32()
+        sizes = (ctypes.c_uint64 * _lib.RMQ_MAX_LEVELS)()
+        offs = (ctypes.c_uint64 * _lib.RMQ_MAX_LEVELS)()
+        st = L.rmq_plan(n, t, alg, ctypes.byref(o), ctypes.byref(nl), sizes, offs, ctypes.byref(bf),
+                        ctypes.byref(R))
+        assert st == rmq_amd.RMQ_OK
+        return list(sizes[:nl.value]), list(offs[:nl.value]), bf.value, R.value
+
+    every = _lib.RMQ_BUILD_PACK_DESCENTS | _lib.RMQ_BUILD_CHECK_QUERIES
+    for n, t, alg, c in ((1 << 26, 1 << 13, 20, 0), (1 << 20, 1 << 15, 16, 2), (100003, 64, 24, 1),
+                         ((1 << 33) + 3, 1 << 13, 20, 1)):
+        base = plan(n, t, alg, c, 0)
+        for flags in (_lib.RMQ_BUILD_PACK_DESCENTS, _lib.RMQ_BUILD_CHECK_QUERIES, every):
+            assert plan(n, t, alg, c, flags) == base, (n, t, alg, c, flags)
+
+
 def test_invalid_arguments():
     for n, t, alg in ((0, 64, 16), ((1 << 38) + 1, 64, 16), (100, 64, 17), (100, 0, 16), (100, 64, 2)):
         with pytest.raises(rmq_amd.RMQError) as e:
