@@ -48,6 +48,7 @@ struct rmq_batch {
   uint64_t level_floats;
   uint64_t rel_bytes;
   std::vector<uint32_t> variant;  // host copy
+  bool lanes_only = false;        // every array runs RMQ_VARIANT_LANE or RMQ_VARIANT_LANE8
 };
 
 struct rmq_batch_plan {
@@ -222,6 +223,9 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     H.tail = reinterpret_cast<const float*>(b->tails + a);
     b->variant[a] = arrays[a].variant;
   }
+  b->lanes_only = std::all_of(b->variant.begin(), b->variant.end(), [](uint32_t v) {
+    return v == RMQ_VARIANT_LANE || v == RMQ_VARIANT_LANE8;
+  });
 
   // build tiles, grouped by source level and reduction factor (R = 32 first, then R = 8)
   std::vector<BuildTile> tiles;
@@ -312,7 +316,7 @@ rmq_status rmq_batch_query(const rmq_batch* b, const rmq_batch_plan* p, const ui
   const hipStream_t s = static_cast<hipStream_t>(stream);
   const uint2* Q = reinterpret_cast<const uint2*>(d_lr);
   rmq::launch_multi_query(b->d_views, b->d_variant, p->d_tiles, p->d_qstart, p->d_counter, p->ntiles,
-                          Q, d_val, d_idx, s);
+                          Q, d_val, d_idx, s, b->lanes_only);
   return err();
 }
 

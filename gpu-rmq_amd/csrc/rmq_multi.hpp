@@ -140,7 +140,7 @@ __device__ __forceinline__ void unit_lanes(const HierView& H, const LT& T, const
 // Answers unit u (64 queries) of queue q with the variant of the unit's array. The wavefront
 // first copies that array's level table into its own LDS slot `meta` (its LDS operations complete
 // in order, so the previous unit's reads of the slot are done before these writes land).
-template <bool IDX>
+template <bool IDX, bool LANES>
 __device__ __forceinline__ void run_unit(uint32_t* meta, const HierView* __restrict__ views,
                                          const uint32_t* __restrict__ variant,
                                          const QueryTile* __restrict__ tiles, uint32_t first, uint32_t u,
@@ -156,6 +156,11 @@ __device__ __forceinline__ void run_unit(uint32_t* meta, const HierView* __restr
   const uint32_t lane = threadIdx.x & 63u;
   if (lane < kMetaWords) meta[lane] = lds_meta(H, lane);
   const LevelsLds L{meta};
+  if constexpr (LANES) {  // every array of the launch runs a lane variant
+    if (variant[a] == RMQ_VARIANT_LANE8) unit_lanes<kLane8RLog, IDX>(H, L, Q, q0, cnt, ov, oi);
+    else unit_lanes<kMultiRLog, IDX>(H, L, Q, q0, cnt, ov, oi);
+    return;
+  }
   switch (variant[a]) {
     case RMQ_VARIANT_GROUP4: unit_groups<4, IDX>(H, L, Q, q0, cnt, ov, oi); break;
     case RMQ_VARIANT_GROUP16: unit_groups<16, IDX>(H, L, Q, q0, cnt, ov, oi); break;
@@ -186,7 +191,7 @@ __device__ __forceinline__ void queue_shape(const uint32_t* __restrict__ qstart,
 // occupancy is worth more than the few registers the compiler then spills (measured). For the
 // same reason its solvers scan the top level instead of reading a sparse table over it: the
 // batched build makes none, and the autotuned thresholds keep top-level scans short.
-template <bool IDX>
+template <bool IDX, bool LANES>
 __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
     const HierView* __restrict__ views, const uint32_t* __restrict__ variant,
     const QueryTile* __restrict__ tiles, const uint32_t* __restrict__ qstart,
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
     queue_shape(qstart, home, first, units, dyn0, share);
     const uint32_t w = (blockIdx.x / kQueues) * (kBlock / kWaveQueries) + threadIdx.x / kWaveQueries;
     for (uint32_t u = w * share; u < (w + 1) * share; ++u)
-      run_unit<IDX>(meta, views, variant, tiles, first, u, Q, ov, oi);
+      run_unit<IDX, LANES>(meta, views, variant, tiles, first, u, Q, ov, oi);
   }
   uint32_t queue = home;
   uint32_t tried = 0;
@@ -224,7 +229,7 @@ __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
       continue;
     }
     const uint32_t u1 = min(u0 + kClaimUnits, units);
-    for (uint32_t u = u0; u < u1; ++u) run_unit<IDX>(meta, views, variant, tiles, first, u, Q, ov, oi);
+    for (uint32_t u = u0; u < u1; ++u) run_unit<IDX, LANES>(meta, views, variant, tiles, first, u, Q, ov, oi);
   }
 }
 
@@ -275,19 +280,30 @@ inline uint32_t persistent_grid(const void* kernel) {
 // Counter reset + persistent query launch over the ntiles tiles of one plan. The grid is the
 // resident maximum, or fewer workgroups when a small launch would leave most wavefronts with
 // nothing to claim (each workgroup's first round claims kClaimUnits units per wavefront).
+template <bool IDX, bool LANES>
+inline void launch_batch_kernel(uint32_t small, const HierView* views, const uint32_t* variant,
+                                const QueryTile* tiles, const uint32_t* qstart, uint32_t* counter,
+                                const uint2* Q, float* ov, uint32_t* oi, hipStream_t s) {
+  static const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(k_query_batch<IDX, LANES>));
+  k_query_batch<IDX, LANES><<<std::min(grid, small), kBlock, 0, s>>>(views, variant, tiles, qstart, counter, Q, ov, oi);
+}
+
+// lanes_only: every array of the launch runs RMQ_VARIANT_LANE or RMQ_VARIANT_LANE8, so the
+// kernel without the group solvers (fewer registers) serves it.
 inline void launch_multi_query(const HierView* views, const uint32_t* variant, const QueryTile* tiles,
                                const uint32_t* qstart, uint32_t* counter, uint32_t ntiles,
-                               const uint2* Q, float* ov, uint32_t* oi, hipStream_t s) {
+                               const uint2* Q, float* ov, uint32_t* oi, hipStream_t s,
+                               bool lanes_only = false) {
   constexpr uint32_t kTilesPerRound = kBlock / kWaveQueries * kClaimUnits / kUnitsPerTile;
   const uint32_t want = (ntiles + kTilesPerRound - 1) / kTilesPerRound;
   const uint32_t small = (want + kQueues - 1) / kQueues * kQueues;
   k_reset_counters<kQueues><<<1, 64, 0, s>>>(counter);
   if (oi) {
-    static const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(k_query_batch<true>));
-    k_query_batch<true><<<std::min(grid, small), kBlock, 0, s>>>(views, variant, tiles, qstart, counter, Q, ov, oi);
+    if (lanes_only) launch_batch_kernel<true, true>(small, views, variant, tiles, qstart, counter, Q, ov, oi, s);
+    else launch_batch_kernel<true, false>(small, views, variant, tiles, qstart, counter, Q, ov, oi, s);
   } else {
-    static const uint32_t grid = persistent_grid(reinterpret_cast<const void*>(k_query_batch<false>));
-    k_query_batch<false><<<std::min(grid, small), kBlock, 0, s>>>(views, variant, tiles, qstart, counter, Q, ov, oi);
+    if (lanes_only) launch_batch_kernel<false, true>(small, views, variant, tiles, qstart, counter, Q, ov, oi, s);
+    else launch_batch_kernel<false, false>(small, views, variant, tiles, qstart, counter, Q, ov, oi, s);
   }
 }
 

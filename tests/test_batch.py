@@ -110,6 +110,51 @@ def test_batch_equals_plain_and_oracle(device, want):
 
 
 @pytest.mark.gpu
+def test_batch_lane_variants_only(device):
+    """A batch whose arrays all run lane variants takes the kernel built without the group
+    solvers: answers equal CPU BASE in value and index mode, and its graph replays match."""
+    import torch
+
+    w, dA, dQ = _device_workload(120, (4, 18), (2, 12), seed=31, kind="mixed")
+    A_host = dA.cpu().numpy()
+    rng = np.random.default_rng(31)
+    for a in range(0, 120, 9):  # ties and signed zeros in some arrays
+        o, n = int(w.offsets[a]), int(w.sizes[a])
+        A_host[o:o + n] = rng.integers(0, 3, size=n).astype(np.float32) * \
+            np.where(rng.random(n) < 0.5, np.float32(-1), np.float32(1))
+    dA.copy_(torch.from_numpy(A_host))
+    var = ["lane8" if a % 3 else "lane" for a in range(120)]
+    t = [64 << (a % 6) for a in range(120)]
+    with rmq_amd.Batch(dA, w.offsets, w.sizes, t, var) as b:
+        plan = b.plan(w.qoffsets)
+        bv, bi = plan.query(dQ, want_val=True, want_idx=True)
+        vv, _ = plan.query(dQ, want_val=True, want_idx=False)
+        torch.cuda.synchronize()
+        gv = torch.full_like(bv, -1.0)
+        gi = torch.full_like(bi, -1)
+        graph = plan.capture(dQ, gv, gi)
+        for rep in range(3):
+            gv.fill_(-1.0)
+            gi.fill_(-1)
+            torch.cuda.synchronize()
+            graph.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(gv.view(torch.int32), bv.view(torch.int32)), rep
+            assert torch.equal(gi, bi), rep
+    bv = bv.cpu().numpy()
+    vv = vv.cpu().numpy()
+    bi = bi.cpu().numpy().view(np.uint32)
+    for a in range(120):
+        o, n = int(w.offsets[a]), int(w.sizes[a])
+        q0, q1 = int(w.qoffsets[a]), int(w.qoffsets[a + 1])
+        Aa = A_host[o:o + n]
+        ref_i = oracle.cpu_base_index(Aa, w.queries[q0:q1])
+        assert np.array_equal(bi[q0:q1], ref_i), (a, var[a])
+        assert np.array_equal(bv[q0:q1].view(np.uint32), Aa[ref_i].view(np.uint32)), (a, var[a])
+        assert np.array_equal(vv[q0:q1].view(np.uint32), Aa[ref_i].view(np.uint32)), (a, var[a])
+
+
+@pytest.mark.gpu
 def test_batch_graph_replays(device):
     """The captured launch replays correctly several times, with eager work in between."""
     import torch
