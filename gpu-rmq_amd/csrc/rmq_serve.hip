@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <deque>
@@ -183,9 +184,13 @@ rmq_status launch(rmq_server* s, Slot& sl) {
   std::memcpy(sl.h_tiles, tiles.data(), tiles.size() * sizeof(rmq::QueryTile));
   std::memcpy(sl.h_qstart, qstart.data(), qstart.size() * 4);
   const hipStream_t st = sl.stream;
+  // every request of the batch on a lane-variant array: the kernel without the group solvers
+  const bool lanes_only = std::all_of(ids.begin(), ids.end(), [s](uint32_t id) {
+    return s->h_variant[id] == RMQ_VARIANT_LANE || s->h_variant[id] == RMQ_VARIANT_LANE8;
+  });
   rmq::launch_multi_query(s->d_views, s->d_variant, sl.d_tiles, sl.d_qstart, sl.d_counter,
                           static_cast<uint32_t>(tiles.size()), reinterpret_cast<const uint2*>(sl.d_q), sl.d_val,
-                          sl.want_idx ? sl.d_idx : nullptr, st);
+                          sl.want_idx ? sl.d_idx : nullptr, st, lanes_only);
   bool ok = hipGetLastError() == hipSuccess;
   ok = ok && hipEventRecord(sl.done, st) == hipSuccess;
   sl.total = total;
