@@ -187,10 +187,12 @@ __device__ __forceinline__ void queue_shape(const uint32_t* __restrict__ qstart,
   dyn0 = share * waves;
 }
 
-// Eight wavefronts per SIMD (64 VGPRs): the kernel carries every variant's solver, and the
-// occupancy is worth more than the few registers the compiler then spills (measured). For the
-// same reason its solvers scan the top level instead of reading a sparse table over it: the
-// batched build makes none, and the autotuned thresholds keep top-level scans short.
+// Eight wavefronts per SIMD (64 VGPRs): with every variant's solver (LANES = false) the
+// occupancy is worth more than the few registers the compiler then spills (measured); LANES =
+// true, for launches whose arrays all run lane variants, leaves the group solvers out and fits
+// in 48 registers without spills. For the same reason its solvers scan the top level instead of
+// reading a sparse table over it: the batched build makes none, and the autotuned thresholds
+// keep top-level scans short.
 template <bool IDX, bool LANES>
 __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
     const HierView* __restrict__ views, const uint32_t* __restrict__ variant,
