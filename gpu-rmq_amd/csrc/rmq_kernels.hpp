@@ -668,6 +668,12 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
   constexpr uint32_t VE = kEarly ? V : 1;
   // each vector is read below only under the condition it was loaded under
   float4 xl[VE], xr[VE];
+  // the next guards, where they lie outside the vectors read (4-aligned piece ends), are loaded
+  // with them rather than after the fold: same entries, same conditions, one round trip less
+  const bool lnext = la && k != 0 && l < (lo << sk);
+  const bool rnext = ra && k != 0 && (hi << sk) < e;
+  const float gl0 = lnext && (lo & 3u) == 0 ? vk[lo - 1u] : RMQ_INF;
+  const float gr0 = rnext && (hi & 3u) == 0 ? vk[hi] : RMQ_INF;
   if constexpr (kEarly) {
 #pragma unroll
     for (uint32_t s = 0; s < V; ++s) {
@@ -697,10 +703,10 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
         }
       }
     }
-    if (k == 0 || !(l < (lo << sk))) {
+    if (!lnext) {
       la = false;
     } else {
-      if ((lo & 3u) == 0) g = vk[lo - 1u];
+      if ((lo & 3u) == 0) g = gl0;
       la = !(g > bv);
     }
   }
@@ -721,10 +727,10 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
         }
       }
     }
-    if (k == 0 || !((hi << sk) < e)) {
+    if (!rnext) {
       ra = false;
     } else {
-      if ((hi & 3u) == 0) g = vk[hi];
+      if ((hi & 3u) == 0) g = gr0;
       ra = g < bv;
     }
   }
