@@ -6,11 +6,18 @@ lognormal-medium lengths (lr = -2, the reference generator), index-returning mod
 2^10..2^18 and the algorithms 16 / 24 / 20; the fastest (alg, t) is then timed. A "step" is one
 query pass over all q queries with the hierarchy and the queries already resident in HBM.
 
-Prints ONE JSON line with the throughput (Gqueries/s), the roofline of the query kernel
-(algorithmic bytes from the reference traversal, HIP-event kernel time) and a CPU baseline
-(the oracle's CPU BASE IDX, alg 100, on a bounded sample, all host threads).
+Prints ONE JSON line with the throughput (Gqueries/s), the roofline of the timed query kernel
+and a CPU baseline (the oracle's CPU BASE IDX, alg 100, on a bounded sample of the same queries).
+The roofline names the resource that binds the kernel: per-launch VALU / SALU wave-instructions
+and HBM bytes come from rocprofv3 counter runs of the same kernel and workload (committed in
+profiles/counters.json), divided by the kernel time measured live with HIP events; the
+reference traversal's algorithmic bytes are reported beside it as `traversal_equiv`. Beside the
+headline the line carries the uniform-length workload (lr = -1), the batched workload (4096
+arrays, one persistent launch, against per-array plain launches) and the serving curve (p50 /
+p99 latency and Gq/s over offered loads, driven by the C++ harness gpu-rmq_amd/serve_load).
 
   python bench.py [--steps K] [--warmup W] [--alg A --t T] [--lr L] [--no-cpu]
+  python bench.py --batched-only     # the batched build + query alone (profiling runs)
 """
 
 from __future__ import annotations
@@ -33,18 +40,56 @@ LR_NAMES = {-1: "uniform-large", -2: "lognormal-medium", -3: "lognormal-small", 
 # packed descents, R = 4, t = 2^13): keeps the timed kernel the one whose counters are committed
 # under profiles/.
 DEFAULT_ALG, DEFAULT_T = "20/c0/p", 1 << 13
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters.json")
+SERVE_LOAD = os.path.join(ROOT, "gpu-rmq_amd", "serve_load")
 
 
-def measured_traffic(key):
-    """HBM bytes per launch (copy-calibrated FETCH_SIZE + WRITE_SIZE) recorded for this exact
-    kernel and workload by a rocprofv3 counter run, or None."""
+def measured_counters(key):
+    """Per-launch counters recorded for this exact kernel and workload by rocprofv3 runs
+    (copy-calibrated FETCH_SIZE + WRITE_SIZE bytes, SQ_INSTS_VALU / SQ_INSTS_SALU / SQ_WAVES),
+    or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
-            entry = json.load(f).get(key)
+        with open(COUNTERS_FILE) as f:
+            return json.load(f).get(key)
     except (OSError, ValueError):
         return None
-    return None if entry is None else entry["bytes_per_launch"]
+
+
+def roofline_block(key, bytes_total, q, ev_ms):
+    """The roofline of the timed kernel: every resource with a measured per-launch figure, as a
+    fraction of its peak at the live kernel time; `bound` is the one closest to its peak."""
+    from rmq_amd import roofline
+
+    ks = ev_ms / 1e3
+    trav_gbps = bytes_total / ks / 1e9
+    trav = {"bytes_per_query": round(bytes_total / q, 3), "gbps": round(trav_gbps, 1),
+            "frac_of_hbm": round(trav_gbps / roofline.HBM_PEAK_GBPS, 4),
+            "note": "the reference traversal's level-0 bytes (R0 = 32, 64-byte sectors): the "
+                    "pruned kernel reads far fewer, so this is a model figure, not a bound"}
+    c = measured_counters(key) or {}
+    res = {}
+    if "bytes_per_launch" in c:
+        g = c["bytes_per_launch"] / ks / 1e9
+        res["hbm"] = (g, roofline.HBM_PEAK_GBPS, "GB/s")
+    if "valu_insts" in c:
+        g = c["valu_insts"] / ks / 1e9
+        res["valu_issue"] = (g, roofline.VALU_PEAK_GINST, "Ginst/s")
+    if "salu_insts" in c:
+        g = c["salu_insts"] / ks / 1e9
+        res["salu_issue"] = (g, roofline.SALU_PEAK_GINST, "Ginst/s")
+    out = {"bound": "unmeasured", "achieved": None, "peak": None, "unit": None, "frac": None}
+    if res:
+        bound = max(res, key=lambda k: res[k][0] / res[k][1])
+        a, p, u = res[bound]
+        out = {"bound": bound, "achieved": round(a, 2), "peak": p, "unit": u, "frac": round(a / p, 4)}
+    out["traffic"] = c.get("bytes_per_launch")
+    out["resources"] = {k: {"achieved": round(v[0], 2), "peak": v[1], "unit": v[2], "frac": round(v[0] / v[1], 4)}
+                        for k, v in res.items()}
+    out["counters_key"] = key
+    out["counters_kernel_ns"] = c.get("avg_ns")
+    out["kernel_ms"] = round(ev_ms, 4)
+    out["traversal_equiv"] = trav
+    return out
 
 
 def parse():
@@ -64,9 +109,14 @@ def parse():
                     help=f"time the default alg {DEFAULT_ALG}, t={DEFAULT_T} without a sweep")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-uniform", action="store_true", help="skip the uniform-length leg")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched build and its query launch (no per-array baseline, no "
+                         "graph, no other leg): the form rocprofv3 counter runs use")
     ap.add_argument("--batch-arrays", type=int, default=4096)
-    ap.add_argument("--serve-requests", type=int, default=6000)
-    ap.add_argument("--serve-rate", type=float, default=20000.0, help="offered requests/s")
+    ap.add_argument("--serve-rates", default="20000,50000,100000,200000,400000",
+                    help="offered requests/s of the serving curve (comma list)")
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival span of each point")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (calibrates rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE against known bytes)")
@@ -111,10 +161,12 @@ def time_queries(h, dQ, want_val, want_idx, out_v, out_i, steps, warmup):
     return wall, ev0.elapsed_time(ev1) / steps
 
 
-def batched_leg(arrays: int):
+def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True):
     """4096 arrays (sizes log-U[2^10, 2^24], mixed lengths, queries log-U[2^6, 2^16] per array):
     one batched build + one persistent query launch vs per-array plain launches (rmq_build +
-    rmq_query per array); identical answers required."""
+    rmq_query per array); identical answers required. baseline / graph_too = False leave the
+    per-array launches and the graph out (counter collection then covers only the batched
+    build and query)."""
     import numpy as np
     import torch
 
@@ -161,122 +213,158 @@ def batched_leg(arrays: int):
     build_b = timed(batched_build)
     plan = state["b"].plan(w.qoffsets)
     query_b = timed(lambda: plan.query(dQ, False, True, None, out_b))
-    graph = plan.capture(dQ, None, out_b)
-    query_g = timed(lambda: graph.launch())
-    # per-array plain launches through the C ABI, issued from C loops (same t per array,
-    # alg 16, R = 32): rmq_build_each / rmq_query_each
-    out_p = torch.empty(Q, dtype=torch.int32, device="cuda")
-    pstate = {}
-
-    def plain_build():
-        if "p" in pstate:
-            pstate["p"].close()
-        pstate["p"] = rmq_amd.PlainMany(dA, w.offsets, w.sizes, ts)
-
-    build_p = timed(plain_build, reps=1)
-    query_p = timed(lambda: pstate["p"].query(dQ, w.qoffsets, out_p))
-    same = bool(torch.equal(out_b, out_p))
-    pstate["p"].close()
-    graph.close()
-    state["b"].close()
-    assert same, "batched answers differ from per-array plain launches"
-    return {
+    res = {
         "workload": f"{arrays} arrays, sizes log-U[2^10,2^24] ({w.total_elements / 1e9:.2f} G "
                     f"elements), queries log-U[2^6,2^16] per array ({Q} total), mixed lengths, "
                     "index mode, autotuned per array",
         "queries": Q,
-        "batched_query_ms": round(query_b, 3), "batched_graph_ms": round(query_g, 3),
-        "plain_query_ms": round(query_p, 3),
-        "query_speedup": round(query_p / query_g, 2),
-        "batched_build_ms": round(build_b, 3), "plain_build_ms": round(build_p, 3),
-        "end_to_end_speedup": round((build_p + query_p) / (build_b + query_g), 2),
-        "batched_gqs": round(Q / query_g / 1e6, 3),
-        "identical": same,
+        "variants": {v: var.count(v) for v in sorted(set(var))},
+        "batched_query_ms": round(query_b, 3),
+        "batched_build_ms": round(build_b, 3),
+        "batched_gqs": round(Q / query_b / 1e6, 3),
+    }
+    if graph_too:
+        graph = plan.capture(dQ, None, out_b)
+        query_g = timed(lambda: graph.launch())
+        res["batched_graph_ms"] = round(query_g, 3)
+        res["batched_gqs"] = round(Q / min(query_g, query_b) / 1e6, 3)
+    if baseline:
+        # per-array plain launches through the C ABI, issued from C loops (same t per array,
+        # alg 16, R = 32): rmq_build_each / rmq_query_each
+        out_p = torch.empty(Q, dtype=torch.int32, device="cuda")
+        pstate = {}
+
+        def plain_build():
+            if "p" in pstate:
+                pstate["p"].close()
+            pstate["p"] = rmq_amd.PlainMany(dA, w.offsets, w.sizes, ts)
+
+        build_p = timed(plain_build, reps=1)
+        query_p = timed(lambda: pstate["p"].query(dQ, w.qoffsets, out_p))
+        if graph_too:
+            graph.launch()
+            torch.cuda.synchronize()
+        same = bool(torch.equal(out_b, out_p))
+        pstate["p"].close()
+        assert same, "batched answers differ from per-array plain launches"
+        best_b = res.get("batched_graph_ms", query_b)
+        res.update({"plain_query_ms": round(query_p, 3), "plain_build_ms": round(build_p, 3),
+                    "query_speedup": round(query_p / best_b, 2),
+                    "end_to_end_speedup": round((build_p + query_p) / (build_b + best_b), 2),
+                    "identical": same})
+    if graph_too:
+        graph.close()
+    state["b"].close()
+    return res
+
+
+def serve_leg(rates, seconds: float):
+    """64 resident arrays (sizes log-U[2^20, 2^28]), each tuned for the mix of lengths it serves;
+    requests with Zipf(1.1) targets, 1..2^16 queries (log-uniform) of small / medium / large
+    lengths, Poisson arrivals at each offered rate. The open-loop client is the C++ harness
+    gpu-rmq_amd/serve_load (submit / poll over the C ABI, latency = completion seen by poll -
+    scheduled arrival); it prints one JSON line per offered rate."""
+    import subprocess
+
+    import numpy as np
+
+    from rmq_amd import autotune, workloads
+
+    if not os.path.exists(SERVE_LOAD):
+        raise RuntimeError(f"{SERVE_LOAD} is missing: build with make -C gpu-rmq_amd")
+    rng = np.random.default_rng(SEED)
+    sizes = np.exp2(rng.uniform(20, 28, size=64)).astype(np.int64)
+    mrng = np.random.default_rng(SEED + 7)  # length samples for tuning only
+    specs = []
+    for n in sizes:
+        # a resident array answers every length class: tune for the mix it will see
+        mix = np.concatenate([workloads.query_lengths(mrng, int(n), 256, k)
+                              for k in ("small", "medium", "large")])
+        var, t = autotune.choose(int(n), mix)
+        specs.append(f"{int(n)}:{t}:{var}")
+    cmd = [SERVE_LOAD, "--arrays", ",".join(specs), "--rates", ",".join(str(int(r)) for r in rates),
+           "--seconds", str(seconds), "--seed", str(SEED)]
+    log("serving: " + " ".join(cmd[:1] + cmd[3:]))
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    sys.stderr.write(p.stderr)
+    if p.returncode != 0:
+        raise RuntimeError(f"serve_load failed ({p.returncode}): {p.stdout[-2000:]}")
+    points = [json.loads(line) for line in p.stdout.splitlines() if line.startswith("{")]
+    header = points.pop(0)
+    return {
+        "workload": "64 resident arrays log-U[2^20,2^28], Poisson arrivals, Zipf(1.1) targets, "
+                    "1..2^16 queries/request (log-uniform), small/medium/large lognormal lengths, "
+                    "index mode; open-loop C++ client (gpu-rmq_amd/serve_load)",
+        "resident_gb": header.get("resident_gb"),
+        "curve": points,
     }
 
 
-def serve_leg(requests: int, rate: float):
-    """64 resident arrays (sizes log-U[2^20, 2^28]), Poisson arrivals at `rate` requests/s, array
-    by Zipf(1.1), 1..2^16 queries per request (log-uniform), small/medium/large lengths.
-    Latency = completion (seen by poll) - arrival."""
-    import gc
-
+def uniform_leg(dA, n, steps, warmup):
+    """The uniform-length workload (lr = -1, L ~ U_int[1, n]) on the same array: the lane
+    solver at R = 4, t = 2^10 (packed and not), R = 8, t = 2^13, and alg 16 at the reference
+    default t = 2^15; the fastest is reported, alg 16 at t = 2^15 beside it. Index mode."""
     import numpy as np
     import torch
 
     import rmq_amd
-    from rmq_amd import autotune, workloads
 
-    rng = np.random.default_rng(SEED)
-    sizes = np.exp2(rng.uniform(20, 28, size=64)).astype(np.int64)
-    mrng = np.random.default_rng(SEED + 7)  # length samples for tuning only
-    with rmq_amd.Server(slots=3, max_batch_queries=1 << 21) as srv:
-        ids = []
-        for a, n in enumerate(sizes):
-            # a resident array answers every length class: tune for the mix it will see
-            mix = np.concatenate([workloads.query_lengths(mrng, int(n), 256, k)
-                                  for k in ("small", "medium", "large")])
-            var, t = autotune.choose(int(n), mix)
-            ids.append(srv.admit_uniform(int(n), seed=SEED + a, t=t, variant=var))
-        ranks = np.arange(1, 65, dtype=np.float64)
-        p = ranks ** -1.1
-        p /= p.sum()
-        targets = rng.choice(64, size=requests, p=p)
-        counts = np.exp2(rng.uniform(0, 16, size=requests)).astype(np.int64)
-        kinds = rng.choice(["small", "medium", "large"], size=requests)
-        reqs = [workloads.make_queries(rng, int(sizes[targets[i]]), int(counts[i]), str(kinds[i]))
-                for i in range(requests)]
-        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=requests))
-        # answers land in one pre-faulted pool (a serving process reuses its buffers)
-        offs = np.concatenate([[0], np.cumsum(counts)])
-        pool = np.ones(int(offs[-1]), dtype=np.uint32)
-        # warm-up: every resident array answers a few queries before the clock starts
-        for a in range(64):
-            srv.submit(ids[a], workloads.make_queries(rng, int(sizes[a]), 256, "medium"), False, True)
-        srv.drain()
+    q = n
+    lr = rmq_amd.gen_queries(q, -1, n, SEED)
+    dQ = torch.from_numpy(lr.view(np.int32)).cuda()
+    out_i = torch.empty(q, dtype=torch.int32, device="cuda")
+    runs = []
+    for var, t in (("20/c0", 1 << 10), ("20/c0/p", 1 << 10), ("20/c1", 1 << 13), ("16", 1 << 15)):
+        alg = int(var.split("/")[0])
+        with rmq_amd.Hierarchy(dA, t, alg, **OPTS_OF[var]) as h:
+            torch.cuda.synchronize()
+            wall, ev = time_queries(h, dQ, False, True, None, out_i, steps, warmup)
+        runs.append({"alg": var, "t": t, "ms": round(ev, 4), "gqs": round(q / ev / 1e6, 3)})
+        log(f"uniform alg={var} t={t}: {ev:.3f} ms/launch, {q / ev / 1e6:.2f} Gq/s")
+    best = min(runs, key=lambda r: r["ms"])
+    alg = int(best["alg"].split("/")[0])
+    with rmq_amd.Hierarchy(dA, best["t"], alg, **OPTS_OF[best["alg"]]) as h:
+        h.query(dQ, False, True, None, out_i)
         torch.cuda.synchronize()
-        gc.collect()
-        gc.disable()  # the driver loop below is the client; keep its collector out of the clock
-        log(f"serving: {requests} requests at {rate:.0f}/s over 64 resident arrays "
-            f"({srv.stats()['resident_bytes'] / 1e9:.1f} GB)")
-        done_at = {}
-        ticket_of = []
-        t0 = time.perf_counter()
-        i = 0
-        while len(done_at) < requests:
-            now = time.perf_counter() - t0
-            while i < requests and arrivals[i] <= now:
-                tk, _, _ = srv.submit(ids[targets[i]], reqs[i], False, True,
-                                      out_idx=pool[offs[i]:offs[i + 1]])
-                ticket_of.append(tk)
-                i += 1
-            for tk in srv.poll():
-                done_at[tk] = time.perf_counter() - t0
-        wall = time.perf_counter() - t0
-        gc.enable()
-        st = srv.stats()
-    lat = np.array([done_at[tk] - arrivals[k] for k, tk in enumerate(ticket_of)]) * 1e3
-    return {
-        "workload": f"64 resident arrays log-U[2^20,2^28], {requests} requests, Poisson "
-                    f"{rate:.0f}/s, Zipf(1.1) targets, 1..2^16 queries/request, index mode",
-        "p50_ms": round(float(np.percentile(lat, 50)), 3),
-        "p90_ms": round(float(np.percentile(lat, 90)), 3),
-        "p99_ms": round(float(np.percentile(lat, 99)), 3),
-        "max_ms": round(float(lat.max()), 3),
-        "gqs": round(float(counts.sum()) / wall / 1e9, 4),
-        "offered_gqs": round(float(counts.sum()) / float(arrivals[-1]) / 1e9, 4),
-        "batches": int(st["batches"]), "requests": requests,
-    }
+    sidx = rmq_amd.gen_sample_indices(q, 1 << 14, SEED + 4)
+    sidx = sidx[(lr[sidx, 1].astype(np.int64) - lr[sidx, 0]) < (1 << 22)]
+    sq = torch.from_numpy(np.ascontiguousarray(lr[sidx]).view(np.int32)).cuda()
+    _, bi = rmq_amd.gpu_base(dA, sq, False, True)
+    sel = torch.from_numpy(sidx.astype(np.int64)).cuda()
+    assert torch.equal(out_i[sel], bi), "uniform-length output differs from GPU BASE"
+    a16 = [r for r in runs if r["alg"] == "16"][0]
+    return {"workload": f"n=2^{n.bit_length() - 1}, q=2^{q.bit_length() - 1}, lr=-1 (uniform lengths), index mode",
+            "gqs": best["gqs"], "alg": best["alg"], "t": best["t"], "ms": best["ms"],
+            "alg16_t2e15_gqs": a16["gqs"], "runs": runs,
+            "spot_check": f"{sidx.size} sampled queries shorter than 2^22 equal GPU BASE"}
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "os_cpu_count": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
 
 
 def cpu_baseline(A, lr, target_s=12.0):
-    """Oracle CPU BASE IDX (alg 100 restated) on a bounded random sample, all host threads."""
+    """Oracle CPU BASE IDX (alg 100 restated) on a bounded random sample, the process's OpenMP
+    threads (OMP_PROC_BIND=spread)."""
     import numpy as np
 
     import oracle  # the CPU baseline leg is the only place bench.py touches the oracle
     import rmq_amd
 
     threads = oracle.num_threads()
+    host = host_cpu()
     q = lr.shape[0]
     probe = lr[rmq_amd.gen_sample_indices(q, 2048, SEED + 1)]
     t0 = time.perf_counter()
@@ -288,6 +376,9 @@ def cpu_baseline(A, lr, target_s=12.0):
     oracle.cpu_base_index(A, sample)
     dt = time.perf_counter() - t0
     gq = k / dt / 1e9
+    cap = (f"; affinity allows {host['affinity_cpus']} of the host's {host['os_cpu_count']} CPUs "
+           f"and OMP_NUM_THREADS caps the pool at {threads}"
+           if host["affinity_cpus"] != threads or host["os_cpu_count"] != threads else "")
     # the oracle's own CPU hierarchy (same levels as the GPU build) on the full query set
     t0 = time.perf_counter()
     h = oracle.CpuHierarchy(A, 1 << 15, 5, 16)
@@ -298,12 +389,22 @@ def cpu_baseline(A, lr, target_s=12.0):
     return {
         "value": gq, "unit": "Gqueries/s", "cores": threads, "kind": "port",
         "sample": f"CPU BASE IDX (alg 100) over {k} queries drawn from the {q}-query set "
-                  f"({dt:.1f} s wall, {threads} OpenMP threads, extrapolated per query)",
+                  f"({dt:.1f} s wall, {threads} OpenMP threads, OMP_PROC_BIND="
+                  f"{host['omp_proc_bind']}, {host['model']}{cap}; extrapolated per query)",
+        "host": host,
     }, {
         "value": q / th / 1e9, "unit": "Gqueries/s", "cores": threads,
         "kind": "own CPU hierarchical (oracle), not HRMQ", "build_ms": tb * 1e3,
         "sample": f"all {q} queries, R=32, t=2^15",
     }
+
+
+OPTS_OF = {"16": {}, "16/g4": dict(group_lanes=4), "24/g4": dict(group_lanes=4), "20": {},
+           "20/c0": dict(cg_amount_log=0), "20/c1": dict(cg_amount_log=1),
+           "16/g64": dict(group_lanes=64), "24/g64": dict(group_lanes=64),
+           "16/g1": dict(group_lanes=1), "16/g8": dict(group_lanes=8),
+           "20/c0/p": dict(cg_amount_log=0, pack=True), "20/c1/p": dict(cg_amount_log=1, pack=True),
+           "16/p": dict(pack=True)}
 
 
 def main():
@@ -314,8 +415,14 @@ def main():
               "of the queries on their own copy, but that form is not built or run here.",
               file=sys.stderr)
         sys.exit(2)
+    # before any OpenMP runtime starts (torch loads one too): the CPU baseline's threads spread
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
     if args.copy_probe:
         copy_probe()
+        return
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(args.batch_arrays, baseline=False, graph_too=False)}),
+              flush=True)
         return
 
     import numpy as np
@@ -342,12 +449,6 @@ def main():
     torch.cuda.synchronize()
 
     # --- choose (alg, t): explicit, default, or a short sweep ------------------------------
-    opts_of = {"16": {}, "16/g4": dict(group_lanes=4), "24/g4": dict(group_lanes=4), "20": {},
-               "20/c0": dict(cg_amount_log=0), "20/c1": dict(cg_amount_log=1),
-               "16/g64": dict(group_lanes=64), "24/g64": dict(group_lanes=64),
-               "16/g1": dict(group_lanes=1), "16/g8": dict(group_lanes=8),
-               "20/c0/p": dict(cg_amount_log=0, pack=True), "20/c1/p": dict(cg_amount_log=1, pack=True),
-               "16/p": dict(pack=True)}
     if args.alg and args.t:
         candidates = [(args.alg, args.t)]
     elif args.no_sweep:
@@ -363,7 +464,7 @@ def main():
     best = None
     for var, t in candidates:
         alg = int(var.split("/")[0])
-        with rmq_amd.Hierarchy(dA, t, alg, **opts_of[var]) as h:
+        with rmq_amd.Hierarchy(dA, t, alg, **OPTS_OF[var]) as h:
             torch.cuda.synchronize()
             wall, ev = time_queries(h, dQ, want_val, want_idx, out_v, out_i,
                                     steps=3 if len(candidates) > 1 else 1, warmup=1)
@@ -378,10 +479,10 @@ def main():
     alg = int(var.split("/")[0])
 
     # --- timed region --------------------------------------------------------------------------
-    with rmq_amd.Hierarchy(dA, t, alg, **opts_of[var]) as h:
+    with rmq_amd.Hierarchy(dA, t, alg, **OPTS_OF[var]) as h:
         torch.cuda.synchronize()
         tb0 = time.perf_counter()
-        with rmq_amd.Hierarchy(dA, t, alg, **opts_of[var]) as h2:
+        with rmq_amd.Hierarchy(dA, t, alg, **OPTS_OF[var]) as h2:
             torch.cuda.synchronize()
             build_ms = (time.perf_counter() - tb0) * 1e3
             meta = h2.metadata()
@@ -400,8 +501,7 @@ def main():
         if want_val:
             assert torch.equal(out_v[sel].view(torch.int32), bv.view(torch.int32))
     gqs = q / (wall_ms / 1e3) / 1e9
-    achieved = bytes_total / (ev_ms / 1e3) / 1e9
-    traffic_key = f"alg={var} t={t} n=2^{args.log_n} q=2^{args.log_q} lr={lr_mode} {args.mode}"
+    counters_key = f"alg={var} t={t} n=2^{args.log_n} q=2^{args.log_q} lr={lr_mode} {args.mode}"
 
     result = {
         "metric": METRIC,
@@ -421,26 +521,25 @@ def main():
                         f"q=2^{args.log_q}, lr={lr_mode} ({LR_NAMES.get(lr_mode, lr_mode)})",
             "alg": var, "t": t, "R": meta.R, "levels": meta.sizes, "n": n, "q": q,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 2),
-            "peak": roofline.HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": round(achieved / roofline.HBM_PEAK_GBPS, 4),
-            "traffic": measured_traffic(traffic_key),
-            "traffic_key": traffic_key,
-            "bytes_per_query": round(bytes_total / q, 3),
-            "kernel_ms": round(ev_ms, 4),
-        },
+        "roofline": roofline_block(counters_key, bytes_total, q, ev_ms),
         "build_ms": round(build_ms, 3),
         "sweep": sweep,
     }
+    del out_v
+    if not args.no_uniform:
+        log("uniform-length leg")
+        del dQ
+        result["uniform"] = uniform_leg(dA, n, steps=max(3, args.steps // 2), warmup=1)
+    del dA
+    torch.cuda.empty_cache()
     if not args.no_batched:
         log("batched leg")
         result["batched"] = batched_leg(args.batch_arrays)
+        torch.cuda.empty_cache()
     if not args.no_serve:
         log("serving leg")
-        result["serve"] = serve_leg(args.serve_requests, args.serve_rate)
+        result["serve"] = serve_leg([float(r) for r in args.serve_rates.split(",") if r],
+                                    args.serve_seconds)
     if not args.no_cpu:
         log("CPU baseline")
         cb, ch = cpu_baseline(A, lr)

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -132,6 +133,9 @@ constexpr uint64_t kMaxN = 0x7FFFFFFFull;  // int32 n, as the reference's CmdArg
 constexpr uint64_t kMaxN64 = 1ull << 38;    // the 64-bit form (nType = uint64_t, src/tools.h:27-29)
 constexpr uint32_t kChunkLog = 30;          // chunk size of the 64-bit form
 constexpr uint64_t kMaxGrid = 1ull << 20;
+// Queries one launch of a one-lane-per-query kernel answers (kMaxGrid workgroups of kBlock
+// lanes); rmq_query and rmq_gpu_base split larger sets into launches of this many.
+constexpr uint64_t kLaunchQueries = kMaxGrid * kBlock;
 // The top level's sparse table is built when the top level holds at least kTopTableMin
 // entries and the table fits in kTopTableMax entries of 8 bytes (64 MiB).
 constexpr uint64_t kTopTableMin = 64;
@@ -647,8 +651,15 @@ rmq_status rmq_query(const rmq_hierarchy* h, const uint32_t* d_lr, uint64_t q, f
     const rmq_status cs = check_queries(h, reinterpret_cast<const uint2*>(d_lr), q, static_cast<hipStream_t>(stream));
     if (cs != RMQ_OK) return cs;
   }
-  return query_dispatch(h, reinterpret_cast<const uint2*>(d_lr), static_cast<uint32_t>(q), d_val,
-                        d_idx, static_cast<hipStream_t>(stream));
+  // one launch per kLaunchQueries queries (the one-lane-per-query kernels have no grid stride)
+  const uint2* Q = reinterpret_cast<const uint2*>(d_lr);
+  for (uint64_t q0 = 0; q0 < q; q0 += kLaunchQueries) {
+    const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(kLaunchQueries, q - q0));
+    const rmq_status st = query_dispatch(h, Q + q0, cnt, d_val ? d_val + q0 : nullptr, d_idx ? d_idx + q0 : nullptr,
+                                         static_cast<hipStream_t>(stream));
+    if (st != RMQ_OK) return st;
+  }
+  return RMQ_OK;
 }
 
 void rmq_free(rmq_hierarchy* h) {
@@ -710,11 +721,14 @@ rmq_status rmq_gpu_base(const float* d_array, uint64_t n, const uint32_t* d_lr, 
   if (q == 0) return RMQ_OK;
   if (!d_lr || (!d_val && !d_idx) || q > 0xFFFFFFFFull) return RMQ_EINVAL;
   if ((reinterpret_cast<uintptr_t>(d_lr) & 7u) != 0) return RMQ_EINVAL;
-  const uint32_t grid = grid_for(q);
-  if (static_cast<uint64_t>(grid) * kBlock < q) return RMQ_EINVAL;
-  rmq::k_gpu_base<<<grid, kBlock, 0, static_cast<hipStream_t>(stream)>>>(
-      d_array, reinterpret_cast<const uint2*>(d_lr), static_cast<uint32_t>(q), d_val, d_idx);
-  return last_error();
+  const uint2* Q = reinterpret_cast<const uint2*>(d_lr);
+  for (uint64_t q0 = 0; q0 < q; q0 += kLaunchQueries) {
+    const uint64_t cnt = std::min<uint64_t>(kLaunchQueries, q - q0);
+    rmq::k_gpu_base<<<grid_for(cnt), kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+        d_array, Q + q0, static_cast<uint32_t>(cnt), d_val ? d_val + q0 : nullptr, d_idx ? d_idx + q0 : nullptr);
+    if (last_error() != RMQ_OK) return RMQ_EDEVICE;
+  }
+  return RMQ_OK;
 }
 
 rmq_status rmq_query64(const rmq_hierarchy* h, const uint64_t* d_lr, uint64_t q, float* d_val,

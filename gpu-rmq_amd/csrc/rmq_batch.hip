@@ -87,7 +87,7 @@ template <int RL>
 __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restrict__ views,
                                                         const BuildTile* __restrict__ tiles) {
   constexpr uint32_t kR = 1u << RL;
-  constexpr uint32_t W = kR / 4;  // lanes per block: 8 at R = 32, 2 at R = 8
+  constexpr uint32_t W = kR / 4;  // lanes per block: 8 at R = 32, 2 at R = 8, 1 at R = 4
   const BuildTile T = tiles[blockIdx.x];
   const HierView& H = views[T.array];
   const uint32_t k = T.level;
@@ -223,15 +223,14 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     H.tail = reinterpret_cast<const float*>(b->tails + a);
     b->variant[a] = arrays[a].variant;
   }
-  b->lanes_only = std::all_of(b->variant.begin(), b->variant.end(), [](uint32_t v) {
-    return v == RMQ_VARIANT_LANE || v == RMQ_VARIANT_LANE8;
-  });
+  b->lanes_only = std::all_of(b->variant.begin(), b->variant.end(), [](uint32_t v) { return rmq::variant_is_lane(v); });
 
-  // build tiles, grouped by source level and reduction factor (R = 32 first, then R = 8)
+  // build tiles, grouped by source level and reduction factor (R = 32, then R = 8, then R = 4)
+  constexpr uint32_t kRLogs[3] = {rmq::kMultiRLog, rmq::kLane8RLog, rmq::kLane4RLog};
   std::vector<BuildTile> tiles;
-  std::vector<uint32_t> start;  // [level][2] -> tile index
+  std::vector<uint32_t> start;  // [level][3] -> tile index
   for (uint32_t k = 0; k + 1 < max_levels; ++k) {
-    for (uint32_t rl : {rmq::kMultiRLog, rmq::kLane8RLog}) {
+    for (uint32_t rl : kRLogs) {
       start.push_back(static_cast<uint32_t>(tiles.size()));
       for (uint32_t a = 0; a < count; ++a) {
         if (rmq::variant_rlog(arrays[a].variant) != rl || views[a].num_levels <= k + 1) continue;
@@ -261,8 +260,11 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
   for (size_t g = 0; ok && g + 1 < start.size(); ++g) {
     const uint32_t t0 = start[g], t1 = start[g + 1];
     if (t1 == t0) continue;
-    if (g % 2 == 0) rmq::k_build_batch<rmq::kMultiRLog><<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0);
-    else rmq::k_build_batch<rmq::kLane8RLog><<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0);
+    switch (kRLogs[g % 3]) {
+      case rmq::kMultiRLog: rmq::k_build_batch<rmq::kMultiRLog><<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0); break;
+      case rmq::kLane8RLog: rmq::k_build_batch<rmq::kLane8RLog><<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0); break;
+      default: rmq::k_build_batch<rmq::kLane4RLog><<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0); break;
+    }
     ok = hipGetLastError() == hipSuccess;
   }
   // the host vectors die here: wait for the copies (and the build) before returning

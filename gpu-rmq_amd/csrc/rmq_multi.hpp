@@ -27,24 +27,30 @@ namespace rmq {
 
 constexpr uint32_t kMultiRLog = 5;       // R = 32: group variants and RMQ_VARIANT_LANE
 constexpr uint32_t kLane8RLog = 3;       // R = 8: RMQ_VARIANT_LANE8
+constexpr uint32_t kLane4RLog = 2;       // R = 4: RMQ_VARIANT_LANE4
 constexpr uint32_t kMultiR = 1u << kMultiRLog;
 
 // Reduction factor (log2) and plan parameters of a variant.
 __host__ __device__ inline uint32_t variant_rlog(uint32_t variant) {
-  return variant == RMQ_VARIANT_LANE8 ? kLane8RLog : kMultiRLog;
+  return variant == RMQ_VARIANT_LANE8 ? kLane8RLog : (variant == RMQ_VARIANT_LANE4 ? kLane4RLog : kMultiRLog);
 }
 
 inline bool variant_ok(uint32_t v) {
   return v == RMQ_VARIANT_GROUP4 || v == RMQ_VARIANT_GROUP8 || v == RMQ_VARIANT_GROUP16 ||
          v == RMQ_VARIANT_GROUP32 || v == RMQ_VARIANT_WAVE || v == RMQ_VARIANT_LANE ||
-         v == RMQ_VARIANT_LANE8;
+         v == RMQ_VARIANT_LANE8 || v == RMQ_VARIANT_LANE4;
+}
+
+// A variant with one lane per query (the lanes-only kernel serves launches of these alone).
+__host__ __device__ inline bool variant_is_lane(uint32_t v) {
+  return v == RMQ_VARIANT_LANE || v == RMQ_VARIANT_LANE8 || v == RMQ_VARIANT_LANE4;
 }
 
 // The plain-ABI build parameters that produce the hierarchy a variant reads.
 inline void variant_build(uint32_t variant, int* alg, rmq_build_options* o) {
-  if (variant == RMQ_VARIANT_LANE8) {
-    *alg = RMQ_ALG_VECTOR_LOAD;  // R = 4 * 2^1
-    *o = rmq_build_options{0, 1, 0, RMQ_BUILD_DEFAULT_CG};
+  if (variant == RMQ_VARIANT_LANE8 || variant == RMQ_VARIANT_LANE4) {
+    *alg = RMQ_ALG_VECTOR_LOAD;  // R = 4 * 2^1 or 4 * 2^0
+    *o = rmq_build_options{0, variant == RMQ_VARIANT_LANE8 ? 1u : 0u, 0, RMQ_BUILD_DEFAULT_CG};
   } else {
     *alg = RMQ_ALG_XXX;          // R = 2^(3 + 2)
     *o = rmq_build_options{3, 2, variant == RMQ_VARIANT_LANE ? 0u : variant, 0};
@@ -157,7 +163,9 @@ __device__ __forceinline__ void run_unit(uint32_t* meta, const HierView* __restr
   if (lane < kMetaWords) meta[lane] = lds_meta(H, lane);
   const LevelsLds L{meta};
   if constexpr (LANES) {  // every array of the launch runs a lane variant
-    if (variant[a] == RMQ_VARIANT_LANE8) unit_lanes<kLane8RLog, IDX>(H, L, Q, q0, cnt, ov, oi);
+    const uint32_t v = variant[a];
+    if (v == RMQ_VARIANT_LANE4) unit_lanes<kLane4RLog, IDX>(H, L, Q, q0, cnt, ov, oi);
+    else if (v == RMQ_VARIANT_LANE8) unit_lanes<kLane8RLog, IDX>(H, L, Q, q0, cnt, ov, oi);
     else unit_lanes<kMultiRLog, IDX>(H, L, Q, q0, cnt, ov, oi);
     return;
   }
@@ -168,6 +176,7 @@ __device__ __forceinline__ void run_unit(uint32_t* meta, const HierView* __restr
     case RMQ_VARIANT_WAVE: unit_groups<64, IDX>(H, L, Q, q0, cnt, ov, oi); break;
     case RMQ_VARIANT_LANE: unit_lanes<kMultiRLog, IDX>(H, L, Q, q0, cnt, ov, oi); break;
     case RMQ_VARIANT_LANE8: unit_lanes<kLane8RLog, IDX>(H, L, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_LANE4: unit_lanes<kLane4RLog, IDX>(H, L, Q, q0, cnt, ov, oi); break;
     default: unit_groups<8, IDX>(H, L, Q, q0, cnt, ov, oi); break;
   }
 }
@@ -290,7 +299,7 @@ inline void launch_batch_kernel(uint32_t small, const HierView* views, const uin
   k_query_batch<IDX, LANES><<<std::min(grid, small), kBlock, 0, s>>>(views, variant, tiles, qstart, counter, Q, ov, oi);
 }
 
-// lanes_only: every array of the launch runs RMQ_VARIANT_LANE or RMQ_VARIANT_LANE8, so the
+// lanes_only: every array of the launch runs a lane variant (variant_is_lane), so the
 // kernel without the group solvers (fewer registers) serves it.
 inline void launch_multi_query(const HierView* views, const uint32_t* variant, const QueryTile* tiles,
                                const uint32_t* qstart, uint32_t* counter, uint32_t ntiles,

@@ -27,6 +27,7 @@ struct Resident {
   bool live = false;
   float* owned = nullptr;  // device array owned by the server (HOST / UNIFORM admission)
   rmq_hierarchy* h = nullptr;
+  uint64_t n = 0;          // elements: submit refuses pairs outside [0, n)
   uint64_t bytes = 0;
   uint32_t users = 0;      // pending + in-flight requests
 };
@@ -185,9 +186,8 @@ rmq_status launch(rmq_server* s, Slot& sl) {
   std::memcpy(sl.h_qstart, qstart.data(), qstart.size() * 4);
   const hipStream_t st = sl.stream;
   // every request of the batch on a lane-variant array: the kernel without the group solvers
-  const bool lanes_only = std::all_of(ids.begin(), ids.end(), [s](uint32_t id) {
-    return s->h_variant[id] == RMQ_VARIANT_LANE || s->h_variant[id] == RMQ_VARIANT_LANE8;
-  });
+  const bool lanes_only =
+      std::all_of(ids.begin(), ids.end(), [s](uint32_t id) { return rmq::variant_is_lane(s->h_variant[id]); });
   rmq::launch_multi_query(s->d_views, s->d_variant, sl.d_tiles, sl.d_qstart, sl.d_counter,
                           static_cast<uint32_t>(tiles.size()), reinterpret_cast<const uint2*>(sl.d_q), sl.d_val,
                           sl.want_idx ? sl.d_idx : nullptr, st, lanes_only);
@@ -306,6 +306,7 @@ rmq_status rmq_server_admit(rmq_server* s, int source, const float* data, uint64
     return st;
   }
   r.live = true;
+  r.n = n;
   r.bytes = rmq_internal_level_bytes(r.h) + (r.owned ? n * 4 : 0);
   s->stats.resident++;
   s->stats.resident_bytes += r.bytes;
@@ -331,11 +332,22 @@ rmq_status rmq_server_submit(rmq_server* s, int32_t array_id, const uint32_t* h_
     return RMQ_EINVAL;
   if (array_id < 0 || static_cast<size_t>(array_id) >= s->arrays.size() || !s->arrays[array_id].live)
     return RMQ_EINVAL;
+  // every pair is checked while it is copied: one bad pair refuses the whole request before
+  // anything is queued, so no kernel ever reads outside a resident array
+  const uint64_t n = s->arrays[array_id].n;
   Request r;
+  r.lr.resize(2 * q);
+  uint32_t bad = 0;
+  for (uint64_t i = 0; i < q; ++i) {
+    const uint32_t l = h_lr[2 * i], rr = h_lr[2 * i + 1];
+    bad |= static_cast<uint32_t>(l > rr) | static_cast<uint32_t>(rr >= n);
+    r.lr[2 * i] = l;
+    r.lr[2 * i + 1] = rr;
+  }
+  if (bad) return RMQ_EINVAL;
   r.ticket = s->next_ticket++;
   r.array = array_id;
   r.q = q;
-  r.lr.assign(h_lr, h_lr + 2 * q);
   r.h_val = h_val;
   r.h_idx = h_idx;
   s->arrays[array_id].users++;

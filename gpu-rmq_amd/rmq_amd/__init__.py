@@ -324,7 +324,7 @@ def gpu_base64(array, lr, want_val: bool = True, want_idx: bool = False, stream=
 
 # ---- batched form (include/rmq_batch.h) ------------------------------------------------------
 
-VARIANTS = {"lane": _lib.RMQ_VARIANT_LANE, "lane8": _lib.RMQ_VARIANT_LANE8,
+VARIANTS = {"lane": _lib.RMQ_VARIANT_LANE, "lane8": _lib.RMQ_VARIANT_LANE8, "lane4": _lib.RMQ_VARIANT_LANE4,
             "group4": _lib.RMQ_VARIANT_GROUP4, "group8": _lib.RMQ_VARIANT_GROUP8,
             "group16": _lib.RMQ_VARIANT_GROUP16, "group32": _lib.RMQ_VARIANT_GROUP32,
             "wave": _lib.RMQ_VARIANT_WAVE}

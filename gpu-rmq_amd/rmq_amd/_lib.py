@@ -36,6 +36,7 @@ def RMQ_BUILD_CHUNK_LOG(k: int) -> int:
     return (k & 0xFF) << 8
 RMQ_VARIANT_LANE = 1
 RMQ_VARIANT_LANE8 = 2
+RMQ_VARIANT_LANE4 = 3
 RMQ_VARIANT_GROUP4 = 4
 RMQ_VARIANT_GROUP8 = 8
 RMQ_VARIANT_GROUP16 = 16

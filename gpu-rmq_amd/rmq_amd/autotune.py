@@ -1,7 +1,8 @@
 """Autotuner for the multi-array (batched / serving) query kernel.
 
 For each (array size bucket, range-length class) it times the execution variants (one lane per
-query, or 4 / 8 / 16 cooperating lanes = 64 / 128 / 256 bytes per group load) and the scan
+query on an R = 4, 8 or 32 hierarchy, or 4 / 8 cooperating lanes or a whole wavefront per query on
+an R = 32 hierarchy) and the scan
 threshold t (hierarchy depth) on synthetic arrays, and persists the fastest choice as JSON
 (tuning_gfx950.json next to this file). `choose` maps an array and its query lengths to the
 persisted choice. Results never depend on the choice; only speed does.
@@ -23,7 +24,7 @@ import numpy as np
 from . import workloads
 
 TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_gfx950.json")
-VARIANTS = ("lane8", "lane", "group4", "group8", "wave")
+VARIANTS = ("lane4", "lane8", "lane", "group4", "group8", "wave")
 THRESHOLDS = (64, 256, 1024, 4096, 32768)
 TOTAL_QUERIES = 1 << 23
 SIZE_LOGS = tuple(range(10, 29, 2))

@@ -4,11 +4,14 @@
         --stats DIR/kt_kernel_stats.csv --fetch DIR/fetch_counter_collection.csv \
         --write DIR/write_counter_collection.csv --copy-fetch DIR/copyf_counter_collection.csv \
         --copy-write DIR/copyw_counter_collection.csv [--l2 DIR/l2_counter_collection.csv] \
-        [--sq DIR/sq_counter_collection.csv] [--traffic-json profiles/traffic.json --traffic-key K] \
+        [--sq DIR/sq_counter_collection.csv] [--insts DIR/insts_counter_collection.csv] \
+        [--counters-json profiles/counters.json --counters-key K] \
         --command "..." > profiles/<kernel>_<workload>.md
 
---traffic-json records the measured bytes per dispatch under key K (the bench's traffic_key for
-the same kernel and workload), so that bench.py reports it as roofline.traffic.
+--counters-json records, under key K (bench.py's counters_key for the same kernel and workload),
+the measured HBM bytes per dispatch and, with --insts (a pass over SQ_INSTS_VALU SQ_INSTS_SALU
+SQ_WAVES), the wave-instructions per dispatch, so that bench.py reports roofline.traffic and the
+instruction-issue roofline of the timed kernel from them.
 
 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what the same counters read for a
 plain device copy of known size (bench.py --copy-probe: 1 GiB read + 1 GiB written per launch).
@@ -21,6 +24,8 @@ import collections
 import csv
 import json
 import os
+
+from . import roofline
 
 
 def _per_kernel(path, counter):
@@ -62,8 +67,10 @@ def main():
     ap.add_argument("--copy-write", required=True)
     ap.add_argument("--l2")
     ap.add_argument("--sq", help="counter CSV with VALUBusy, VALUUtilization, OccupancyPercent")
-    ap.add_argument("--traffic-json")
-    ap.add_argument("--traffic-key")
+    ap.add_argument("--insts", help="counter CSV with SQ_INSTS_VALU, SQ_INSTS_SALU, SQ_WAVES")
+    ap.add_argument("--counters-json")
+    ap.add_argument("--counters-key")
+    ap.add_argument("--note", action="append", default=[], help="a line printed after the table")
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("--title", default="")
     a = ap.parse_args()
@@ -113,16 +120,42 @@ def main():
             vals = _per_kernel(a.sq, counter)
             if any(a.kernel in k for k in vals):
                 out.append(f"| {label} | {_match(vals, a.kernel)[1]:.1f} |")
+    insts = {}
+    if a.insts:
+        for counter in ("SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_WAVES"):
+            vals = _per_kernel(a.insts, counter)
+            if any(a.kernel in k for k in vals):
+                insts[counter] = _match(vals, a.kernel)[1]
+        waves = insts.get("SQ_WAVES")
+        for counter, peak, label in (("SQ_INSTS_VALU", roofline.VALU_PEAK_GINST, "VALU"),
+                                     ("SQ_INSTS_SALU", roofline.SALU_PEAK_GINST, "SALU")):
+            if counter in insts:
+                n = insts[counter]
+                per_wave = f", {n / waves:.0f} per wavefront" if waves else ""
+                out.append(f"| {label} wave-instructions per dispatch | {n:.4g}{per_wave} |")
+                out.append(f"| {label} issue at average duration | {n / avg_ns:.1f} G/s = "
+                           f"{n / avg_ns / peak:.3f} of the {peak:.1f} G/s peak (one per CU per cycle, "
+                           f"{roofline.CUS} CUs at {roofline.CLOCK_GHZ} GHz) |")
+        if waves:
+            out.append(f"| wavefronts per dispatch | {waves:.0f} ({q / waves:.1f} queries each) |")
+    out.append(f"| HBM fraction from measured traffic | {(fetch_b + write_b) / avg_ns / 8000:.3f} of 8 TB/s |")
+    for line in a.note:
+        out.append("")
+        out.append(line)
     print("\n".join(out))
-    if a.traffic_json and a.traffic_key:
+    if a.counters_json and a.counters_key:
         table = {}
-        if os.path.exists(a.traffic_json):
-            with open(a.traffic_json) as f:
+        if os.path.exists(a.counters_json):
+            with open(a.counters_json) as f:
                 table = json.load(f)
-        table[a.traffic_key] = {"bytes_per_launch": round(fetch_b + write_b),
-                                "fetch_bytes": round(fetch_b), "write_bytes": round(write_b),
-                                "kernel": name, "avg_ns": round(avg_ns)}
-        with open(a.traffic_json, "w") as f:
+        entry = {"bytes_per_launch": round(fetch_b + write_b), "fetch_bytes": round(fetch_b),
+                 "write_bytes": round(write_b), "kernel": name, "avg_ns": round(avg_ns)}
+        for counter, key in (("SQ_INSTS_VALU", "valu_insts"), ("SQ_INSTS_SALU", "salu_insts"),
+                             ("SQ_WAVES", "waves")):
+            if counter in insts:
+                entry[key] = round(insts[counter])
+        table[a.counters_key] = entry
+        with open(a.counters_json, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)
 
 

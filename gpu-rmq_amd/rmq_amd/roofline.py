@@ -17,6 +17,17 @@ import numpy as np
 R0 = 32
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
 
+# Instruction issue of MI355X: 256 CUs at 2.4 GHz. A CU has four SIMDs of 16 lanes; a wave64
+# VALU instruction holds its SIMD for 4 cycles and the sequencer issues at most one VALU and one
+# SALU instruction per CU per cycle, so both peaks are one wave-instruction per CU per cycle.
+# The counters SQ_INSTS_VALU / SQ_INSTS_SALU count wave-instructions (one per wavefront that
+# issues it, whatever its active lanes), so per-dispatch counts divided by the kernel time are
+# directly comparable with these peaks.
+CUS = 256
+CLOCK_GHZ = 2.4
+VALU_PEAK_GINST = CUS * CLOCK_GHZ  # 614.4 G wave-instructions/s
+SALU_PEAK_GINST = CUS * CLOCK_GHZ
+
 
 def _sec(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     s = (b // 16) - (a // 16) + 1

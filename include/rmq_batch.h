@@ -6,8 +6,9 @@
  * kernel launch per hierarchy level (over all arrays at once) and their uneven query batches
  * are answered by ONE persistent kernel launch that pulls 256-query tiles from a device-side
  * work queue. Every array has its own scan threshold t and execution variant, chosen by the
- * caller (e.g. from the persisted autotuner table); the variant fixes R: 8 for
- * RMQ_VARIANT_LANE8 (the reference's VL paper setting), 32 (the CL default) otherwise. Results
+ * caller (e.g. from the persisted autotuner table); the variant fixes R: 4 for RMQ_VARIANT_LANE4
+ * (the reference's VL with cg_amount_log 0), 8 for RMQ_VARIANT_LANE8 (the reference's VL paper
+ * setting), 32 (the CL default) otherwise. Results
  * are identical to rmq_build + rmq_query per array (value = A[leftmost argmin], leftmost index).
  *
  * Layout: all arrays live in one device buffer d_arrays; array a occupies
@@ -34,13 +35,15 @@ enum {
   RMQ_VARIANT_GROUP32 = 32,  /* 32 lanes */
   RMQ_VARIANT_WAVE = 64,     /* a whole wavefront per query (long final ranges, large t) */
   RMQ_VARIANT_LANE = 1,      /* one lane per query, 16-byte loads, R = 32 hierarchy */
-  RMQ_VARIANT_LANE8 = 2      /* one lane per query, 16-byte loads, R = 8 hierarchy */
+  RMQ_VARIANT_LANE8 = 2,     /* one lane per query, 16-byte loads, R = 8 hierarchy */
+  RMQ_VARIANT_LANE4 = 3      /* one lane per query, 16-byte loads, R = 4 hierarchy */
 };
 
 typedef struct {
   uint64_t offset;   /* element offset of the array in d_arrays (multiple of 4) */
   uint32_t n;        /* 1 <= n <= 2^31 - 1 */
-  uint32_t t;        /* scan threshold: >= 64 (2R at R = 32), or >= 8 for RMQ_VARIANT_LANE8 */
+  uint32_t t;        /* scan threshold: >= 64 (2R at R = 32); >= 8 for RMQ_VARIANT_LANE8, >= 4 for
+                        RMQ_VARIANT_LANE4 (R) */
   uint32_t variant;  /* RMQ_VARIANT_* */
   uint32_t reserved;
 } rmq_batch_array;

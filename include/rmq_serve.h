@@ -52,9 +52,9 @@ rmq_status rmq_server_create(const rmq_server_options* opt, rmq_server** out);
 void rmq_server_destroy(rmq_server* s);
 
 /* Build and keep resident an array of n floats with scan threshold t and execution variant
- * RMQ_VARIANT_* (R = 8 for RMQ_VARIANT_LANE8, t >= 8; R = 32 otherwise, t >= 64). Returns
- * RMQ_ENOMEM, with nothing allocated, when the array (if the server owns it) and its levels do
- * not fit in free HBM minus the reserve. */
+ * RMQ_VARIANT_* (R = 4 for RMQ_VARIANT_LANE4, t >= 4; R = 8 for RMQ_VARIANT_LANE8, t >= 8;
+ * R = 32 otherwise, t >= 64). Returns RMQ_ENOMEM, with nothing allocated, when the array (if the
+ * server owns it) and its levels do not fit in free HBM minus the reserve. */
 rmq_status rmq_server_admit(rmq_server* s, int source, const float* data, uint64_t n, uint64_t seed,
                             uint32_t t, uint32_t variant, int32_t* array_id);
 
@@ -63,7 +63,10 @@ rmq_status rmq_server_evict(rmq_server* s, int32_t array_id);
 
 /* Queue q queries (host {l, r} pairs, copied now) on a resident array. Answers are written to
  * the caller's host buffers h_val / h_idx (either may be NULL, not both) before the ticket is
- * reported done. q must be 1..max_batch_queries. */
+ * reported done. q must be 1..max_batch_queries. Every pair must satisfy 0 <= l <= r < n of the
+ * array: the pairs are checked while they are copied, and a request holding any other pair is
+ * refused with RMQ_EINVAL, with nothing queued (the batch kernel is shared by all requests in
+ * flight, so a bad pair is never handed to it). */
 rmq_status rmq_server_submit(rmq_server* s, int32_t array_id, const uint32_t* h_lr, uint64_t q,
                              float* h_val, uint32_t* h_idx, uint64_t* ticket);
 

@@ -4,6 +4,7 @@ Batched answers must equal per-array plain launches (rmq_build + rmq_query) and 
 CPU BASE bit-exactly, for every execution variant and scan threshold."""
 
 import ctypes
+import sys
 
 import numpy as np
 import pytest
@@ -22,7 +23,7 @@ def test_batch_abi_rejects_bad_arguments():
     assert L.rmq_batch_build(0x1000, arr, 0, None, ctypes.byref(out)) == rmq_amd.RMQ_EINVAL
     arr[0].t = 63
     assert L.rmq_batch_build(0x1000, arr, 1, None, ctypes.byref(out)) == rmq_amd.RMQ_ETHRESHOLD
-    arr[0].t, arr[0].variant = 64, 3
+    arr[0].t, arr[0].variant = 64, 5
     assert L.rmq_batch_build(0x1000, arr, 1, None, ctypes.byref(out)) == rmq_amd.RMQ_EUNSUPPORTED
     arr[0].variant, arr[0].offset = 8, 2
     assert L.rmq_batch_build(0x1000, arr, 1, None, ctypes.byref(out)) == rmq_amd.RMQ_EINVAL
@@ -80,9 +81,9 @@ def test_batch_equals_plain_and_oracle(device, want):
         A_host[o:o + n] = rng.integers(0, 3, size=n).astype(np.float32) * \
             np.where(rng.random(n) < 0.5, np.float32(-1), np.float32(1))
     dA.copy_(torch.from_numpy(A_host))
-    variants = ["group8", "group4", "group16", "lane", "lane8", "group32", "wave"]
+    variants = ["group8", "group4", "group16", "lane", "lane8", "group32", "wave", "lane4"]
     t = [64 << (a % 5) for a in range(96)]
-    var = [variants[a % 7] for a in range(96)]
+    var = [variants[a % 8] for a in range(96)]
     wv, wi = want in ("value",), want in ("index",)
     with rmq_amd.Batch(dA, w.offsets, w.sizes, t, var) as b:
         plan = b.plan(w.qoffsets)
@@ -100,7 +101,8 @@ def test_batch_equals_plain_and_oracle(device, want):
         if wi:
             assert np.array_equal(bi[q0:q1], ref_i), (a, var[a])
         # per-array plain launch through the C ABI gives the same answers
-        plain = dict(alg=20, cg_amount_log=1) if var[a] == "lane8" else dict(alg=16)
+        plain = {"lane8": dict(alg=20, cg_amount_log=1), "lane4": dict(alg=20, cg_amount_log=0)}.get(
+            var[a], dict(alg=16))
         with rmq_amd.Hierarchy(dA[o:o + n], t[a], **plain) as h:
             pv, pi = h.query(dQ[q0:q1], True, True)
             torch.cuda.synchronize()
@@ -123,7 +125,7 @@ def test_batch_lane_variants_only(device):
         A_host[o:o + n] = rng.integers(0, 3, size=n).astype(np.float32) * \
             np.where(rng.random(n) < 0.5, np.float32(-1), np.float32(1))
     dA.copy_(torch.from_numpy(A_host))
-    var = ["lane8" if a % 3 else "lane" for a in range(120)]
+    var = [("lane", "lane8", "lane4")[a % 3] for a in range(120)]
     t = [64 << (a % 6) for a in range(120)]
     with rmq_amd.Batch(dA, w.offsets, w.sizes, t, var) as b:
         plan = b.plan(w.qoffsets)
@@ -179,3 +181,77 @@ def test_batch_graph_replays(device):
             junk = torch.rand(1 << 20, device="cuda").sum()  # eager work between replays
             torch.cuda.synchronize()
             assert float(junk) > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_config4_full_workload(device):
+    """The batched workload at full size, exactly as bench.py runs it: 4096 arrays with sizes
+    log-U[2^10, 2^24] (about 7.3 G elements), log-U[2^6, 2^16] queries per array (about 39 M),
+    mixed lengths, variant and t autotuned per array. The eager launch (value + index, value
+    only) and two graph replays equal the per-array plain launches (rmq_build_each /
+    rmq_query_each) on every query; every array's answers equal CPU BASE (the oracle) on a sample
+    of its queries, and every 16th array's equal the oracle hierarchy on all of its queries."""
+    import torch
+
+    from rmq_amd import autotune
+
+    seed = 27722
+    w = workloads.batch_workload(arrays=4096, log_n=(10, 24), log_q=(6, 16), kind="mixed", seed=seed)
+    total = int(w.offsets[-1] + w.sizes[-1] + 64)
+    dA = torch.empty(total, dtype=torch.float32, device="cuda")
+    rmq_amd.fill_uniform(dA, seed=seed)
+    dQ = torch.from_numpy(w.queries.view(np.int32)).cuda()
+    choices = []
+    for a in range(4096):
+        q = w.queries[w.qoffsets[a]:w.qoffsets[a + 1]]
+        choices.append(autotune.choose(int(w.sizes[a]), q[:, 1].astype(np.int64) - q[:, 0] + 1))
+    var = [c[0] for c in choices]
+    ts = [c[1] for c in choices]
+    Q = int(w.qoffsets[-1])
+    torch.cuda.synchronize()
+    sys.stderr.write(f"[gpu-case] config 4: {Q} queries, variants {sorted(set(var))}\n")
+    with rmq_amd.Batch(dA, w.offsets, w.sizes, ts, var) as b:
+        plan = b.plan(w.qoffsets)
+        bv, bi = plan.query(dQ, True, True)
+        vv, _ = plan.query(dQ, True, False)
+        torch.cuda.synchronize()
+        gi = torch.full_like(bi, -1)
+        graph = plan.capture(dQ, None, gi)
+        for rep in range(2):
+            gi.fill_(-1)
+            torch.cuda.synchronize()
+            graph.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(gi, bi), rep
+            junk = torch.rand(1 << 20, device="cuda").sum()  # eager work between replays
+            torch.cuda.synchronize()
+            assert float(junk) > 0
+        del gi
+    sys.stderr.write("[gpu-case] config 4: per-array plain launches\n")
+    with rmq_amd.PlainMany(dA, w.offsets, w.sizes, ts) as pm:
+        out_p = torch.empty(Q, dtype=torch.int32, device="cuda")
+        pm.query(dQ, w.qoffsets, out_p)
+        torch.cuda.synchronize()
+    assert torch.equal(out_p, bi)
+    del out_p
+    base = torch.from_numpy(np.repeat(w.offsets, w.qcounts)).cuda()
+    at = dA[base + (bi.to(torch.int64) & 0xFFFFFFFF)]
+    assert torch.equal(at.view(torch.int32), bv.view(torch.int32))
+    assert torch.equal(at.view(torch.int32), vv.view(torch.int32))
+    del base, at, vv, bv
+    bi_h = bi.cpu().numpy().view(np.uint32)
+    sys.stderr.write("[gpu-case] config 4: oracle\n")
+    rng = np.random.default_rng(seed + 1)
+    for a in range(4096):
+        o, n = int(w.offsets[a]), int(w.sizes[a])
+        q0, q1 = int(w.qoffsets[a]), int(w.qoffsets[a + 1])
+        Aa = dA[o:o + n].cpu().numpy()
+        lr = w.queries[q0:q1]
+        got = bi_h[q0:q1]
+        s = rng.choice(q1 - q0, size=min(16, q1 - q0), replace=False)
+        assert np.array_equal(got[s], oracle.cpu_base_index(Aa, lr[s])), (a, var[a], ts[a])
+        if a % 16 == 0:
+            assert np.array_equal(got, oracle.CpuHierarchy(Aa, 64, 5, 16).query(lr)[1]), (a, var[a], ts[a])
+    del dA, dQ, bi
+    torch.cuda.empty_cache()

@@ -454,3 +454,93 @@ def test_check_queries_mode(device):
             with pytest.raises(rmq_amd.RMQError) as e:
                 h.query64(torch.from_numpy(lb.view(np.int64)).cuda(), True, True)
             assert e.value.status == rmq_amd.RMQ_EINVAL
+
+
+@pytest.mark.timeout(400)
+def test_uniform_lengths_2_26(device):
+    """The north-star workload: n = q = 2^26, uniform lengths (lr = -1, the paper's sweep), value
+    and index mode. Alg 16 at the reference default t = 2^15, alg 20 at R = 8 and the bench's
+    uniform-length configuration (alg 20, R = 4, t = 2^10), each packed and not: 2^22 queries
+    against the oracle hierarchy (pinned to CPU BASE in test_oracle.py) plus a CPU BASE sample;
+    the full 2^26 queries by size-independent properties (every configuration gives the same
+    answer, the index lies in [l, r], the value is A[index])."""
+    import torch
+
+    _CACHE.clear()
+    n = q = 1 << 26
+    A = _array_2_26()
+    lr = rmq_amd.gen_queries(q, -1, n, SEED)
+    sub = lr[: 1 << 22]
+    ref_v, ref_i = oracle.CpuHierarchy(A, 1 << 15, 5, 16).query(sub)
+    sample = rmq_amd.gen_sample_indices(sub.shape[0], 384, SEED + 5)
+    assert np.array_equal(oracle.cpu_base_index(A, sub[sample]), ref_i[sample])
+    dA = torch.from_numpy(A).cuda()
+    dQ = torch.from_numpy(lr.view(np.int32)).cuda()
+    dS = dQ[: sub.shape[0]]
+    lo = torch.from_numpy(lr[:, 0].astype(np.int64)).cuda()
+    hi = torch.from_numpy(lr[:, 1].astype(np.int64)).cuda()
+    first = None
+    for alg, t, opts in ((16, 1 << 15, {}), (20, 1 << 13, dict(cg_amount_log=1)),
+                         (20, 1 << 10, dict(cg_amount_log=0)), (20, 1 << 10, dict(cg_amount_log=0, pack=True)),
+                         (24, 1 << 15, dict(group_lanes=4))):
+        _say(f"uniform 2^26 alg={alg} t={t} opts={opts}")
+        with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
+            v, i = h.query(dS, True, True)
+            vv, _ = h.query(dS, True, False)
+            torch.cuda.synchronize()
+            assert_exact(A, sub, v.cpu().numpy(), i.cpu().numpy().view(np.uint32), ref_i,
+                         f"uniform alg={alg} t={t} {opts}")
+            assert_exact(A, sub, vv.cpu().numpy(), None, ref_i, f"uniform value alg={alg} t={t} {opts}")
+            fv, fi = h.query(dQ, True, True)
+            torch.cuda.synchronize()
+        fi64 = fi.to(torch.int64) & 0xFFFFFFFF
+        assert bool(((fi64 >= lo) & (fi64 <= hi)).all()), (alg, t, opts)
+        assert torch.equal(fv.view(torch.int32), dA[fi64].view(torch.int32)), (alg, t, opts)
+        if first is None:
+            first = (fv, fi)
+        else:
+            assert torch.equal(fi, first[1]), (alg, t, opts)
+            assert torch.equal(fv.view(torch.int32), first[0].view(torch.int32)), (alg, t, opts)
+    del dA, dQ, lo, hi, first
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.timeout(300)
+def test_query_count_above_one_launch(device):
+    """q = 2^28 + 4099 queries (more than one launch of the one-lane-per-query kernels holds):
+    the call is split into launches of 2^28, and every answer equals GPU BASE's (itself split the
+    same way), for alg 16 and the packed lane solver; a sample, the launch boundary and the last
+    queries are checked against CPU BASE."""
+    import torch
+
+    _CACHE.clear()
+    rng = np.random.default_rng(29)
+    n = 100003
+    A = rng.integers(0, 1000, size=n).astype(np.float32)  # many ties
+    q = (1 << 28) + 4099
+    dA = torch.from_numpy(A).cuda()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(29)
+    ln = torch.randint(1, 300, (q,), device="cuda", dtype=torch.int32, generator=g)
+    l = (torch.rand(q, device="cuda", generator=g) * (n - ln).to(torch.float32)).to(torch.int32)
+    l = torch.minimum(l, n - ln)
+    dQ = torch.stack([l, l + ln - 1], dim=1).contiguous()
+    del ln, l
+    _say("q > 2^28: GPU BASE")
+    _, bi = rmq_amd.gpu_base(dA, dQ, False, True)
+    torch.cuda.synchronize()
+    edge = np.concatenate([np.arange((1 << 28) - 3, (1 << 28) + 3), np.arange(q - 8, q),
+                           rmq_amd.gen_sample_indices(q, 2000, 3).astype(np.int64)])
+    lr_edge = dQ[torch.from_numpy(edge).cuda()].cpu().numpy().view(np.uint32)
+    ref = oracle.cpu_base_index(A, lr_edge)
+    assert np.array_equal(bi[torch.from_numpy(edge).cuda()].cpu().numpy().view(np.uint32), ref)
+    for alg, t, opts in ((16, 64, {}), (20, 16, dict(cg_amount_log=0, pack=True))):
+        _say(f"q > 2^28: alg={alg} t={t} opts={opts}")
+        with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
+            v, i = h.query(dQ, True, True)
+            torch.cuda.synchronize()
+        assert torch.equal(i, bi), (alg, opts)
+        assert torch.equal(v.view(torch.int32), dA[i.to(torch.int64)].view(torch.int32)), (alg, opts)
+        del v, i
+    del dQ, bi
+    torch.cuda.empty_cache()

@@ -3,6 +3,7 @@ batching. Every answer must equal the oracle's, and a request must get the same 
 and when batched with others."""
 
 import ctypes
+import sys
 
 import numpy as np
 import pytest
@@ -40,9 +41,9 @@ def _host_arrays(rng, count, lo, hi):
 def test_serve_matches_oracle_and_is_order_independent(device):
     rng = np.random.default_rng(17)
     arrays = _host_arrays(rng, 12, 1000, 300000)
-    variants = ["lane", "group8", "group4", "group16", "lane8", "wave"]
+    variants = ["lane", "group8", "group4", "group16", "lane8", "wave", "lane4"]
     with rmq_amd.Server(slots=3, max_batch_queries=1 << 16) as srv:
-        ids = [srv.admit_host(A, t=64 << (i % 4), variant=variants[i % 6])
+        ids = [srv.admit_host(A, t=64 << (i % 4), variant=variants[i % 7])
                for i, A in enumerate(arrays)]
         reqs = []
         for k in range(150):
@@ -106,3 +107,106 @@ def test_admission_by_free_memory_and_eviction(device):
         assert srv.stats()["resident"] == 0
         with pytest.raises(rmq_amd.RMQError):
             srv.submit(small, lr, True, False)  # evicted
+
+
+@pytest.mark.gpu
+def test_submit_refuses_pairs_outside_the_array(device):
+    """A request holding one pair with l > r or r >= n is refused with RMQ_EINVAL and nothing is
+    queued (the shared batch kernel never sees it); valid requests around it are answered."""
+    rng = np.random.default_rng(8)
+    A = rng.random(5000, dtype=np.float32)
+    with rmq_amd.Server(slots=2, max_batch_queries=1 << 12) as srv:
+        aid = srv.admit_host(A, t=64, variant="lane")
+        good = workloads.make_queries(rng, A.size, 300, "medium")
+        for bad in ([7, 6], [0, 5000], [4999, 5000], [0xFFFFFFFF, 0xFFFFFFFF]):
+            lr = good.copy()
+            lr[123] = bad
+            with pytest.raises(rmq_amd.RMQError) as e:
+                srv.submit(aid, lr, True, True)
+            assert e.value.status == rmq_amd.RMQ_EINVAL, bad
+        st = srv.stats()
+        assert st["submitted"] == 0 and st["batches"] == 0
+        t, v, i = srv.submit(aid, good, True, True)
+        assert srv.drain() == [t]
+        ref = oracle.cpu_base_index(A, good)
+        assert np.array_equal(i, ref) and np.array_equal(v.view(np.uint32), A[ref].view(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_config5_full_workload(device):
+    """The serving workload at full size, as bench.py runs it: 64 resident arrays with sizes
+    log-U[2^20, 2^28] (about 3 G elements), each tuned for the mix of lengths it serves;
+    1500 requests with Zipf(1.1) targets and 1..2^16 queries (log-uniform) of small, medium or
+    large lengths, submitted in bursts while earlier batches are in flight, so that they coalesce.
+    Every answer (index and value) equals the oracle hierarchy's, a sample of the short queries
+    equals CPU BASE's, and 20 probe requests get the same answers alone as in the mix."""
+    import torch
+
+    from rmq_amd import autotune
+
+    seed = 27722
+    rng = np.random.default_rng(seed)
+    sizes = np.exp2(rng.uniform(20, 28, size=64)).astype(np.int64)
+    mrng = np.random.default_rng(seed + 7)
+    dev = []
+    with rmq_amd.Server(slots=3, max_batch_queries=1 << 21) as srv:
+        ids = []
+        for a, n in enumerate(sizes):
+            d = torch.empty(int(n), dtype=torch.float32, device="cuda")
+            rmq_amd.fill_uniform(d, seed=seed + a)
+            mix = np.concatenate([workloads.query_lengths(mrng, int(n), 256, k)
+                                  for k in ("small", "medium", "large")])
+            var, t = autotune.choose(int(n), mix)
+            ids.append(srv.admit_device(d, t=t, variant=var))
+            dev.append(d)
+        ranks = np.arange(1, 65, dtype=np.float64)
+        p = ranks ** -1.1
+        p /= p.sum()
+        nreq = 1500
+        targets = rng.choice(64, size=nreq, p=p)
+        counts = np.exp2(rng.uniform(0, 16, size=nreq)).astype(np.int64)
+        kinds = rng.choice(["small", "medium", "large"], size=nreq)
+        reqs = [workloads.make_queries(rng, int(sizes[targets[i]]), int(counts[i]), str(kinds[i]))
+                for i in range(nreq)]
+        sys.stderr.write(f"[gpu-case] config 5: {int(counts.sum())} queries in {nreq} requests\n")
+        alone = []
+        for i in range(20):
+            t, v, x = srv.submit(ids[targets[i]], reqs[i], True, True)
+            assert srv.drain() == [t]
+            alone.append((v, x))
+        out = []
+        done = set()
+        for i in range(nreq):
+            out.append(srv.submit(ids[targets[i]], reqs[i], True, i % 2 == 0))
+            if i % 5 == 4:
+                done.update(srv.poll())
+        done.update(srv.drain())
+        assert done == {t for t, _, _ in out}
+        st = srv.stats()
+        assert st["completed"] == st["submitted"] == nreq + 20 and st["in_flight"] == 0
+        assert st["batches"] < 20 + nreq // 2, st  # requests were batched together
+    for i in range(20):
+        assert np.array_equal(out[i][1].view(np.uint32), alone[i][0].view(np.uint32)), i
+        if out[i][2] is not None:
+            assert np.array_equal(out[i][2], alone[i][1]), i
+    sys.stderr.write("[gpu-case] config 5: oracle\n")
+    for a in range(64):
+        mine = np.nonzero(targets == a)[0]
+        if mine.size == 0:
+            continue
+        A = dev[a].cpu().numpy()
+        lr = np.concatenate([reqs[i] for i in mine])
+        ref = oracle.CpuHierarchy(A, 64, 5, 16).query(lr)[1]
+        k = 0
+        for i in mine:
+            q = reqs[i].shape[0]
+            _, v, x = out[i]
+            assert np.array_equal(v.view(np.uint32), A[ref[k:k + q]].view(np.uint32)), (a, i)
+            if x is not None:
+                assert np.array_equal(x, ref[k:k + q]), (a, i)
+            k += q
+        short = np.nonzero((lr[:, 1].astype(np.int64) - lr[:, 0]) < (1 << 16))[0][:500]
+        assert np.array_equal(oracle.cpu_base_index(A, lr[short]), ref[short]), a
+    del dev
+    torch.cuda.empty_cache()
