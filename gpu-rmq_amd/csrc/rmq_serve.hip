@@ -280,6 +280,12 @@ rmq_status rmq_server_admit(rmq_server* s, int source, const float* data, uint64
   }
   Resident& r = s->arrays[id];
   const float* A = data;
+  // a borrowed device array may still be written by work on other streams (the caller's fill or
+  // copy): the build below runs on the server's own non-blocking stream, so wait for the device
+  if (source == RMQ_ADMIT_DEVICE && hipDeviceSynchronize() != hipSuccess) {
+    (void)hipGetLastError();
+    return RMQ_EDEVICE;
+  }
   if (source != RMQ_ADMIT_DEVICE) {
     if (!dalloc(reinterpret_cast<void**>(&r.owned), n * 4)) return RMQ_ENOMEM;
     if (source == RMQ_ADMIT_HOST) {

@@ -172,6 +172,16 @@ def lib() -> ctypes.CDLL:
                 f"librmq.so not found at {LIB_PATH}; build it with `make -C gpu-rmq_amd` "
                 "(or __graft_entry__.build())"
             )
+        # One HIP runtime per process: torch bundles its own libamdhip64 (SONAME libamdhip64.so.7)
+        # and loads it under the file name libamdhip64.so through its RPATH, while librmq.so asks
+        # for libamdhip64.so.7. Loaded after torch, librmq.so binds to torch's copy by SONAME;
+        # loaded first, it would bring in the system copy and torch would then load its own as a
+        # second runtime, with neither seeing the other's streams or allocations. So torch (the
+        # package's source of device memory and streams) is imported first when it is installed.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

@@ -35,7 +35,9 @@ typedef struct {
 } rmq_server_options;
 
 enum {
-  RMQ_ADMIT_DEVICE = 0,   /* `data` is a device array, borrowed until eviction */
+  RMQ_ADMIT_DEVICE = 0,   /* `data` is a device array, borrowed until eviction; admit first waits
+                             for all work on the device, so writes to it issued before the call
+                             on any stream are complete when it is built */
   RMQ_ADMIT_HOST = 1,     /* `data` is a host array, copied into server-owned device memory */
   RMQ_ADMIT_UNIFORM = 2   /* server-owned device memory filled by rmq_fill_uniform(seed) */
 };

@@ -132,6 +132,31 @@ def test_submit_refuses_pairs_outside_the_array(device):
         assert np.array_equal(i, ref) and np.array_equal(v.view(np.uint32), A[ref].view(np.uint32))
 
 
+def _explain_mismatch(dA, A, lr, v, x, ref, bad, a, i, srv, aid):
+    """Which side differs on the mismatching queries of one request: GPU BASE and a plain
+    rmq_query (lane solver, R = 8, t = 64) on the same device array and the request submitted
+    again alone, printed beside the server's and the oracle's answers."""
+    import torch
+
+    sel = lr[bad[:8]]
+    dQ = torch.from_numpy(np.ascontiguousarray(sel).view(np.int32)).cuda()
+    bv, bi = rmq_amd.gpu_base(dA, dQ, True, True)
+    with rmq_amd.Hierarchy(dA, 64, 20, cg_amount_log=1) as h:
+        pv, pi = h.query(dQ, True, True)
+        torch.cuda.synchronize()
+    _, av, _ = srv.submit(aid, lr, True, False)
+    srv.drain()
+    sys.stderr.write(f"[gpu-case] config 5 mismatch: array {a} (n={A.size}) request {i}: {bad.size} of "
+                     f"{lr.shape[0]} values differ (index requested: {x is not None}); first {bad[:8].tolist()}\n")
+    for k in range(sel.shape[0]):
+        j = int(bad[k])
+        sys.stderr.write(f"  q={j} lr={sel[k].tolist()} server={v[j]!r} ({v.view(np.uint32)[j]:#x}) "
+                         f"oracle={A[ref[j]]!r} @ {int(ref[j])} gpu_base={float(bv[k])!r} @ {int(bi[k])} "
+                         f"plain={float(pv[k])!r} @ {int(pi[k])} resubmitted alone={av[j]!r}\n")
+    sys.stderr.write(f"  resubmitted alone: {int(np.sum(av.view(np.uint32) != A[ref].view(np.uint32)))} differ\n")
+    sys.stderr.flush()
+
+
 @pytest.mark.gpu
 @pytest.mark.timeout(900)
 def test_config5_full_workload(device):
@@ -186,27 +211,30 @@ def test_config5_full_workload(device):
         st = srv.stats()
         assert st["completed"] == st["submitted"] == nreq + 20 and st["in_flight"] == 0
         assert st["batches"] < 20 + nreq // 2, st  # requests were batched together
-    for i in range(20):
-        assert np.array_equal(out[i][1].view(np.uint32), alone[i][0].view(np.uint32)), i
-        if out[i][2] is not None:
-            assert np.array_equal(out[i][2], alone[i][1]), i
-    sys.stderr.write("[gpu-case] config 5: oracle\n")
-    for a in range(64):
-        mine = np.nonzero(targets == a)[0]
-        if mine.size == 0:
-            continue
-        A = dev[a].cpu().numpy()
-        lr = np.concatenate([reqs[i] for i in mine])
-        ref = oracle.CpuHierarchy(A, 64, 5, 16).query(lr)[1]
-        k = 0
-        for i in mine:
-            q = reqs[i].shape[0]
-            _, v, x = out[i]
-            assert np.array_equal(v.view(np.uint32), A[ref[k:k + q]].view(np.uint32)), (a, i)
-            if x is not None:
-                assert np.array_equal(x, ref[k:k + q]), (a, i)
-            k += q
-        short = np.nonzero((lr[:, 1].astype(np.int64) - lr[:, 0]) < (1 << 16))[0][:500]
-        assert np.array_equal(oracle.cpu_base_index(A, lr[short]), ref[short]), a
+        for i in range(20):
+            assert np.array_equal(out[i][1].view(np.uint32), alone[i][0].view(np.uint32)), i
+            if out[i][2] is not None:
+                assert np.array_equal(out[i][2], alone[i][1]), i
+        sys.stderr.write("[gpu-case] config 5: oracle\n")
+        for a in range(64):
+            mine = np.nonzero(targets == a)[0]
+            if mine.size == 0:
+                continue
+            A = dev[a].cpu().numpy()
+            lr = np.concatenate([reqs[i] for i in mine])
+            ref = oracle.CpuHierarchy(A, 64, 5, 16).query(lr)[1]
+            k = 0
+            for i in mine:
+                q = reqs[i].shape[0]
+                _, v, x = out[i]
+                bad = np.nonzero(v.view(np.uint32) != A[ref[k:k + q]].view(np.uint32))[0]
+                if bad.size:
+                    _explain_mismatch(dev[a], A, reqs[i], v, x, ref[k:k + q], bad, a, i, srv, ids[a])
+                assert bad.size == 0, (a, i)
+                if x is not None:
+                    assert np.array_equal(x, ref[k:k + q]), (a, i)
+                k += q
+            short = np.nonzero((lr[:, 1].astype(np.int64) - lr[:, 0]) < (1 << 16))[0][:500]
+            assert np.array_equal(oracle.cpu_base_index(A, lr[short]), ref[short]), a
     del dev
     torch.cuda.empty_cache()
