@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <numeric>
 #include <vector>
@@ -129,11 +130,130 @@ __global__ void k_init_zeros(const HierView* __restrict__ views, uint32_t count)
   }
 }
 
+// ---- the batched query without a plan (rmq_batch_query_noalloc) ----------------------------------
+// Its work queue lives in the file-scope array `scratch` below instead of a plan's allocation:
+// the queue counters, the queue starts, every array's first tile and the 256-query tiles, built
+// on the device from the caller's query offsets by the two kernels that follow.
+constexpr uint32_t kStaticArrays = RMQ_BATCH_NOALLOC_MAX_ARRAYS;
+constexpr uint32_t kStaticTiles = RMQ_BATCH_NOALLOC_MAX_TILES;
+constexpr size_t kScrCounters = 0;                                  // kCounterWords words
+constexpr size_t kScrQstart = kScrCounters + kCounterWords * 4;     // kQueues + 1 words
+constexpr size_t kScrTileStart = kScrQstart + 256;                  // kStaticArrays + 1 words
+constexpr size_t kScrTiles = (kScrTileStart + 4 * (kStaticArrays + 1) + 255) & ~size_t(255);
+constexpr size_t kScratchBytes = kScrTiles + kStaticTiles * sizeof(QueryTile);
+constexpr uint32_t kTileThreads = 1024;
+
+// One workgroup: tile counts per array (ceil(q_a / 256), clamped so that a wrong offset cannot
+// overflow), their exclusive prefix sum (the first tile of each array, and the total at
+// [count]), the queue starts (make_query_tiles's rule: queue k starts at the first tile with
+// at least k/8 of all queries before it) and zeroed queue counters. A layout of more than
+// kStaticTiles tiles (offsets that do not match the caller's total) becomes an empty one.
+__global__ __launch_bounds__(kTileThreads) void k_static_tile_starts(const uint64_t* __restrict__ qoff,
+                                                                     uint32_t count, uint8_t* __restrict__ scr) {
+  uint32_t* counter = reinterpret_cast<uint32_t*>(scr + kScrCounters);
+  uint32_t* qstart = reinterpret_cast<uint32_t*>(scr + kScrQstart);
+  uint32_t* ts = reinterpret_cast<uint32_t*>(scr + kScrTileStart);
+  __shared__ uint64_t part[kTileThreads];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t per = (count + kTileThreads - 1) / kTileThreads;
+  const uint32_t a0 = min(count, tid * per), a1 = min(count, a0 + per);
+  auto tiles_of = [&](uint32_t a) -> uint64_t {
+    const uint64_t c = qoff[a + 1] - qoff[a];
+    const uint64_t t = c / kTileQueries + (c % kTileQueries != 0);
+    return t < kStaticTiles + 1ull ? t : kStaticTiles + 1ull;
+  };
+  uint64_t sum = 0;
+  for (uint32_t a = a0; a < a1; ++a) sum += tiles_of(a);
+  part[tid] = sum;
+  __syncthreads();
+  for (uint32_t d = 1; d < kTileThreads; d <<= 1) {  // inclusive scan
+    const uint64_t x = tid >= d ? part[tid - d] : 0ull;
+    __syncthreads();
+    part[tid] += x;
+    __syncthreads();
+  }
+  const uint64_t total_tiles = part[kTileThreads - 1];
+  const bool ok = total_tiles <= kStaticTiles;
+  uint64_t base = part[tid] - sum;
+  for (uint32_t a = a0; a < a1; ++a) {
+    ts[a] = static_cast<uint32_t>(base < 0xFFFFFFFFull ? base : 0xFFFFFFFFull);
+    base += tiles_of(a);
+  }
+  const uint32_t T = ok ? static_cast<uint32_t>(total_tiles) : 0u;
+  if (tid == 0) ts[count] = T;
+  if (tid < kQueues) counter[tid * kCounterStride] = 0u;
+  __syncthreads();  // ts[] complete (workgroup-scope fence for global memory)
+  if (tid <= kQueues) {
+    uint32_t qs = T;
+    const uint64_t q0 = qoff[0], total = qoff[count] - q0;
+    if (tid == 0 || !ok) {
+      qs = 0u;
+    } else if (tid < kQueues) {
+      const uint64_t x = q0 + (total * tid + kQueues - 1) / kQueues;  // the first query of queue tid
+      if (x < qoff[count]) {
+        uint32_t lo = 0, hi = count - 1;  // the array holding query x: smallest a with qoff[a + 1] > x
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) / 2;
+          if (qoff[mid + 1] > x) hi = mid;
+          else lo = mid + 1;
+        }
+        const uint64_t into = x - qoff[lo];
+        qs = ts[lo] + static_cast<uint32_t>(into / kTileQueries + (into % kTileQueries != 0));
+      }
+    }
+    qstart[tid] = qs;
+  }
+}
+
+// One thread per tile t < ts[count]: its array (the last one whose first tile is <= t) and its
+// queries.
+__global__ __launch_bounds__(kBlock) void k_static_tiles(const uint64_t* __restrict__ qoff, uint32_t count,
+                                                         uint8_t* __restrict__ scr) {
+  const uint32_t* ts = reinterpret_cast<const uint32_t*>(scr + kScrTileStart);
+  QueryTile* tiles = reinterpret_cast<QueryTile*>(scr + kScrTiles);
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= ts[count]) return;
+  uint32_t lo = 0, hi = count;  // ts[lo] <= t < ts[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (ts[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  const uint64_t q0 = qoff[lo] + static_cast<uint64_t>(kTileQueries) * (t - ts[lo]);
+  const uint64_t left = qoff[lo + 1] - q0;
+  tiles[t] = QueryTile{lo, static_cast<uint32_t>(left < kTileQueries ? left : kTileQueries), q0};
+}
+
 }  // namespace rmq
+
+// The work queue of rmq_batch_query_noalloc on each GPU: one copy per device, held until the
+// process ends. Host code never uses this name as an address (it is a host-side stand-in); the
+// device address comes from hipGetSymbolAddress, once per device (static_scratch below).
+static __device__ __align__(256) uint8_t scratch[rmq::kScratchBytes];
 
 namespace {
 
 rmq_status err() { return hipGetLastError() == hipSuccess ? RMQ_OK : RMQ_EDEVICE; }
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the first call there.
+uint8_t* static_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    g_scratch[dev] = static_cast<uint8_t*>(p);
+  }
+  return g_scratch[dev];
+}
 
 template <typename T>
 rmq_status dmalloc(T** p, size_t count) {
@@ -319,6 +439,27 @@ rmq_status rmq_batch_query(const rmq_batch* b, const rmq_batch_plan* p, const ui
   const uint2* Q = reinterpret_cast<const uint2*>(d_lr);
   rmq::launch_multi_query(b->d_views, b->d_variant, p->d_tiles, p->d_qstart, p->d_counter, p->ntiles,
                           Q, d_val, d_idx, s, b->lanes_only);
+  return err();
+}
+
+rmq_status rmq_batch_query_noalloc(const rmq_batch* b, const uint64_t* d_qoffsets, uint64_t total_queries,
+                                   const uint32_t* d_lr, float* d_val, uint32_t* d_idx, void* stream) {
+  if (!b || !d_qoffsets || (reinterpret_cast<uintptr_t>(d_qoffsets) & 7u) != 0) return RMQ_EINVAL;
+  if (total_queries == 0) return RMQ_OK;
+  if (!d_lr || (!d_val && !d_idx) || (reinterpret_cast<uintptr_t>(d_lr) & 7u) != 0) return RMQ_EINVAL;
+  // every array has at most q_a / 256 + 1 tiles
+  const uint64_t max_tiles = total_queries / rmq::kTileQueries + b->count;
+  if (b->count > rmq::kStaticArrays || max_tiles > rmq::kStaticTiles) return RMQ_EINVAL;
+  uint8_t* scr = static_scratch();
+  if (!scr) return RMQ_EDEVICE;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  rmq::k_static_tile_starts<<<1, rmq::kTileThreads, 0, s>>>(d_qoffsets, b->count, scr);
+  rmq::k_static_tiles<<<static_cast<uint32_t>((max_tiles + kBlock - 1) / kBlock), kBlock, 0, s>>>(d_qoffsets, b->count,
+                                                                                                  scr);
+  rmq::launch_multi_query(b->d_views, b->d_variant, reinterpret_cast<const QueryTile*>(scr + rmq::kScrTiles),
+                          reinterpret_cast<const uint32_t*>(scr + rmq::kScrQstart),
+                          reinterpret_cast<uint32_t*>(scr + rmq::kScrCounters), static_cast<uint32_t>(max_tiles),
+                          reinterpret_cast<const uint2*>(d_lr), d_val, d_idx, s, b->lanes_only);
   return err();
 }
 

@@ -128,6 +128,7 @@ SIGNATURES = {
     "rmq_batch_build": (ctypes.c_int, [_vp, _vp, _u32, _vp, ctypes.POINTER(_vp)]),
     "rmq_batch_plan_queries": (ctypes.c_int, [_vp, _vp, _vp, ctypes.POINTER(_vp)]),
     "rmq_batch_query": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "rmq_batch_query_noalloc": (ctypes.c_int, [_vp, _vp, _u64, _vp, _vp, _vp, _vp]),
     "rmq_batch_capture": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
     "rmq_graph_launch": (ctypes.c_int, [_vp, _vp]),
     "rmq_graph_free": (None, [_vp]),

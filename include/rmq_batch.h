@@ -68,6 +68,26 @@ rmq_status rmq_batch_plan_queries(const rmq_batch* b, const uint64_t* qoffsets, 
 rmq_status rmq_batch_query(const rmq_batch* b, const rmq_batch_plan* p, const uint32_t* d_lr,
                            float* d_val, uint32_t* d_idx, void* stream);
 
+/* rmq_batch_query without a plan, for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the
+ * reference. The query layout comes as d_qoffsets, a DEVICE array of count + 1 entries read like
+ * rmq_batch_plan_queries's qoffsets (array a's pairs start at pair d_qoffsets[a]; d_qoffsets[count]
+ * ends the last), and total_queries = d_qoffsets[count] - d_qoffsets[0]. The work queue (tiles,
+ * queue starts, counters) is built on the device, by two small kernels before the persistent
+ * query kernel, in a static array of the library: about 4.3 MiB per GPU, held until the process
+ * ends. Nothing is allocated and nothing is copied; every step is a kernel launch on `stream`, so
+ * the call can be captured in a HIP graph. Answers equal rmq_batch_query's.
+ * Limits: count <= RMQ_BATCH_NOALLOC_MAX_ARRAYS and total_queries / 256 + count <=
+ * RMQ_BATCH_NOALLOC_MAX_TILES (2^26 queries in 2^16 arrays, for example); larger calls return
+ * RMQ_EINVAL before anything runs. Offsets that do not add up to total_queries answer nothing.
+ * Every call on one GPU shares that GPU's static array and it keeps the last call's contents:
+ * calls that overlap on one GPU (other streams or threads) race and must be serialised by the
+ * caller. */
+#define RMQ_BATCH_NOALLOC_MAX_ARRAYS 65536u
+#define RMQ_BATCH_NOALLOC_MAX_TILES 262144u
+rmq_status rmq_batch_query_noalloc(const rmq_batch* b, const uint64_t* d_qoffsets, uint64_t total_queries,
+                                   const uint32_t* d_lr, float* d_val, uint32_t* d_idx, void* stream);
+
 /* Capture rmq_batch_query into a HIP graph on `stream` and instantiate it; replay with
  * rmq_graph_launch. Buffers are bound at capture time. */
 typedef struct rmq_graph rmq_graph;
