@@ -55,9 +55,29 @@ def measured_counters(key):
         return None
 
 
-def roofline_block(key, bytes_total, q, ev_ms):
+def issue_peaks():
+    """VALU / SALU issue peaks measured on this device by gpu-rmq_amd/issue_probe (independent
+    chains of one instruction kind on 8 wavefronts per SIMD), or None."""
+    import subprocess
+
+    probe = os.path.join(ROOT, "gpu-rmq_amd", "issue_probe")
+    if not os.path.exists(probe):
+        return None
+    p = subprocess.run([probe], capture_output=True, text=True, timeout=120)
+    if p.returncode != 0:
+        log(f"issue_probe failed ({p.returncode}): {p.stderr[-500:]}")
+        return None
+    lines = [json.loads(x) for x in p.stdout.splitlines() if x.startswith("{")]
+    out = lines[-1]
+    out["kinds"] = lines[:-1]
+    return out
+
+
+def roofline_block(key, bytes_total, q, ev_ms, peaks=None):
     """The roofline of the timed kernel: every resource with a measured per-launch figure, as a
-    fraction of its peak at the live kernel time; `bound` is the one closest to its peak."""
+    fraction of its peak at the live kernel time; `bound` is the one closest to its peak. Issue
+    peaks are the measured ones (issue_peaks) when given, else one wave-instruction per CU per
+    cycle."""
     from rmq_amd import roofline
 
     ks = ev_ms / 1e3
@@ -71,12 +91,14 @@ def roofline_block(key, bytes_total, q, ev_ms):
     if "bytes_per_launch" in c:
         g = c["bytes_per_launch"] / ks / 1e9
         res["hbm"] = (g, roofline.HBM_PEAK_GBPS, "GB/s")
+    vpeak = peaks["valu_peak_ginst_per_s"] if peaks else roofline.VALU_PEAK_GINST
+    speak = peaks["salu_peak_ginst_per_s"] if peaks else roofline.SALU_PEAK_GINST
     if "valu_insts" in c:
         g = c["valu_insts"] / ks / 1e9
-        res["valu_issue"] = (g, roofline.VALU_PEAK_GINST, "Ginst/s")
+        res["valu_issue"] = (g, vpeak, "Ginst/s")
     if "salu_insts" in c:
         g = c["salu_insts"] / ks / 1e9
-        res["salu_issue"] = (g, roofline.SALU_PEAK_GINST, "Ginst/s")
+        res["salu_issue"] = (g, speak, "Ginst/s")
     out = {"bound": "unmeasured", "achieved": None, "peak": None, "unit": None, "frac": None}
     if res:
         bound = max(res, key=lambda k: res[k][0] / res[k][1])
@@ -85,6 +107,8 @@ def roofline_block(key, bytes_total, q, ev_ms):
     out["traffic"] = c.get("bytes_per_launch")
     out["resources"] = {k: {"achieved": round(v[0], 2), "peak": v[1], "unit": v[2], "frac": round(v[0] / v[1], 4)}
                         for k, v in res.items()}
+    out["issue_peak_source"] = ("measured by gpu-rmq_amd/issue_probe: " + json.dumps(
+        {k: v for k, v in peaks.items() if k != "kinds"})) if peaks else "model: one per CU per cycle"
     out["counters_key"] = key
     out["counters_kernel_ns"] = c.get("avg_ns")
     out["kernel_ms"] = round(ev_ms, 4)
@@ -291,12 +315,15 @@ def serve_leg(rates, seconds: float):
         raise RuntimeError(f"serve_load failed ({p.returncode}): {p.stdout[-2000:]}")
     points = [json.loads(line) for line in p.stdout.splitlines() if line.startswith("{")]
     header = points.pop(0)
+    check = [x["check"] for x in points if "check" in x]
+    points = [x for x in points if "check" not in x]
     return {
         "workload": "64 resident arrays log-U[2^20,2^28], Poisson arrivals, Zipf(1.1) targets, "
                     "1..2^16 queries/request (log-uniform), small/medium/large lognormal lengths, "
                     "index mode; open-loop C++ client (gpu-rmq_amd/serve_load)",
         "resident_gb": header.get("resident_gb"),
         "curve": points,
+        "check": check[0] if check else None,
     }
 
 
@@ -425,6 +452,8 @@ def main():
               flush=True)
         return
 
+    peaks = issue_peaks()  # its own process, before this one touches the GPU
+
     import numpy as np
     import torch
 
@@ -521,7 +550,7 @@ def main():
                         f"q=2^{args.log_q}, lr={lr_mode} ({LR_NAMES.get(lr_mode, lr_mode)})",
             "alg": var, "t": t, "R": meta.R, "levels": meta.sizes, "n": n, "q": q,
         },
-        "roofline": roofline_block(counters_key, bytes_total, q, ev_ms),
+        "roofline": roofline_block(counters_key, bytes_total, q, ev_ms, peaks),
         "build_ms": round(build_ms, 3),
         "sweep": sweep,
     }

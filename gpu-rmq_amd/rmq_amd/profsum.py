@@ -127,15 +127,21 @@ def main():
             if any(a.kernel in k for k in vals):
                 insts[counter] = _match(vals, a.kernel)[1]
         waves = insts.get("SQ_WAVES")
-        for counter, peak, label in (("SQ_INSTS_VALU", roofline.VALU_PEAK_GINST, "VALU"),
-                                     ("SQ_INSTS_SALU", roofline.SALU_PEAK_GINST, "SALU")):
-            if counter in insts:
-                n = insts[counter]
-                per_wave = f", {n / waves:.0f} per wavefront" if waves else ""
-                out.append(f"| {label} wave-instructions per dispatch | {n:.4g}{per_wave} |")
-                out.append(f"| {label} issue at average duration | {n / avg_ns:.1f} G/s = "
-                           f"{n / avg_ns / peak:.3f} of the {peak:.1f} G/s peak (one per CU per cycle, "
-                           f"{roofline.CUS} CUs at {roofline.CLOCK_GHZ} GHz) |")
+        if "SQ_INSTS_VALU" in insts:
+            n = insts["SQ_INSTS_VALU"]
+            per_wave = f", {n / waves:.0f} per wavefront" if waves else ""
+            out.append(f"| VALU wave-instructions per dispatch | {n:.4g}{per_wave}, {n / q:.2f} per query |")
+            out.append(f"| VALU issue at average duration | {n / avg_ns:.1f} G/s = {n / avg_ns / roofline.VALU_PEAK_FASTEST:.3f}"
+                       f" of the fastest VALU op's measured issue rate ({roofline.VALU_PEAK_FASTEST} G/s, v_add_u32), "
+                       f"{n / avg_ns / roofline.VALU_PEAK_FP32:.3f} of the fp32 min / select rate "
+                       f"({roofline.VALU_PEAK_FP32} G/s) |")
+        if "SQ_INSTS_SALU" in insts:
+            n = insts["SQ_INSTS_SALU"]
+            per_wave = f", {n / waves:.0f} per wavefront" if waves else ""
+            out.append(f"| SALU wave-instructions per dispatch | {n:.4g}{per_wave} |")
+            out.append(f"| SALU issue at average duration | {n / avg_ns:.1f} G/s = "
+                       f"{n / avg_ns / roofline.SALU_PEAK_MEASURED:.3f} of the measured "
+                       f"{roofline.SALU_PEAK_MEASURED} G/s |")
         if waves:
             out.append(f"| wavefronts per dispatch | {waves:.0f} ({q / waves:.1f} queries each) |")
     out.append(f"| HBM fraction from measured traffic | {(fetch_b + write_b) / avg_ns / 8000:.3f} of 8 TB/s |")

@@ -27,6 +27,12 @@ CUS = 256
 CLOCK_GHZ = 2.4
 VALU_PEAK_GINST = CUS * CLOCK_GHZ  # 614.4 G wave-instructions/s
 SALU_PEAK_GINST = CUS * CLOCK_GHZ
+# What gpu-rmq_amd/issue_probe measured on an MI355X (G wave-instructions/s, 8 waves per SIMD,
+# independent chains): 32-bit integer adds issue faster than the one-per-CU-cycle model, float
+# min and selects at it. bench.py measures them live; these are for the profile summaries.
+VALU_PEAK_FASTEST = 983.1  # v_add_u32, 1.60 per CU per cycle
+VALU_PEAK_FP32 = 599.6     # v_min_f32 (v_cndmask_b32: 596.3), 0.98 per CU per cycle
+SALU_PEAK_MEASURED = 591.9  # s_mov_b32
 
 
 def _sec(a: np.ndarray, b: np.ndarray) -> np.ndarray:
