@@ -7,8 +7,9 @@
 //            [--cg_size_log G] [--cg_amount_log C] [--group_lanes K] [--pack_descents] [--long]
 //
 // alg: 16 (XXX / GPU-RMQ CL), 20 (hierarchical vector load / GPU-RMQ VL), 24 (XXX multi load),
-//      2 (GPU BASE); +100 returns leftmost indices instead of values (102, 116, 120, 124).
-// The CPU algorithms 0/1/100/101 and the OptiX ones are not part of this driver.
+//      2 (GPU BASE), 0 (CPU BASE, OpenMP over queries with --nt threads); +100 returns leftmost
+//      indices instead of values (100, 102, 116, 120, 124). HRMQ (1/101, an external library)
+//      and the OptiX algorithms are not part of this driver.
 // --check compares against GPU BASE as the reference does (src/main.cu:194-214), but
 // bit-exactly (value bit patterns, indices) instead of |a-b| < 1e-4 (src/tools.h:261-264).
 // The runtime flags --cg_size_log / --cg_amount_log replace the reference's compile-time
@@ -53,6 +54,8 @@ enum {
 
 const char* alg_name(int alg) {
   switch (alg) {
+    case 0: return "[CPU] BASE";
+    case 100: return "[CPU] BASE IDX";
     case 2: return "[GPU] BASE";
     case 16: return "[GPU] XXX";
     case 20: return "[GPU] BASIC VECTOR LOAD";
@@ -79,16 +82,18 @@ void print_help() {
           "  -3 -> lognormal distribution (small values)\n"
           "  -6 -> mixed distribution (large, medium and small values)\n"
           "alg = algorithm\n"
+          "   0 -> [CPU] BASE\n"
           "   2 -> [GPU] BASE\n"
           "  16 -> [GPU] XXX\n"
           "  20 -> [GPU] BASIC VECTOR LOAD\n"
           "  24 -> [GPU] XXX multi load\n"
-          "  102, 116, 120, 124 -> algs 2 16 20 24 returning indices\n\n"
+          "  100, 102, 116, 120, 124 -> algs 0 2 16 20 24 returning indices\n\n"
           "Options:\n"
           "   --log_bs <t>              scan threshold t of the hierarchical algorithms as log2 (default: 2^15)\n"
           "   --reps <repetitions>      RMQ repeats for the avg time (default: 10)\n"
           "   --dev <device ID>         device ID (default: 0)\n"
-          "   --nt  <thread num>        number of CPU threads\n"
+          "   --nb <blocks>             scan threshold as n / blocks, rounded up to a power of two\n"
+          "   --nt  <thread num>        number of CPU threads (algs 0 / 100)\n"
           "   --seed <seed>             seed for PRNG\n"
           "   --check                   check correctness (bit-exact against GPU BASE)\n"
           "   --randTrivialCheck        check only 2^15 random queries\n"
@@ -103,6 +108,13 @@ void print_help() {
 uint32_t log2_floor(uint64_t x) {
   uint32_t r = 0;
   while (x >>= 1) ++r;
+  return r;
+}
+
+// log2 of the smallest power of two >= x (x >= 1): log2(bit_ceil(x))
+uint32_t log2_ceil(uint64_t x) {
+  uint32_t r = 0;
+  while ((1ull << r) < x) ++r;
   return r;
 }
 
@@ -166,7 +178,16 @@ CmdArgs get_args(int argc, char* argv[]) {
       case ARG_THRESHOLD:
         if (optarg) a.log_build_threshold = atoi(optarg);  // parsed but unused, as in the reference
         break;
-      case ARG_NB: a.nb = std::min<int64_t>(a.n, atol(optarg)); break;
+      case ARG_NB: {  // src/tools.h:168-175: t = bit_ceil(n / nb)
+        const long v = atol(optarg);
+        if (v <= 0) {
+          fprintf(stderr, "Error: --nb %s must be a positive number of blocks\n", optarg);
+          exit(EXIT_FAILURE);
+        }
+        a.nb = std::min<int64_t>(a.n, v);
+        a.log_bs = log2_ceil(static_cast<uint64_t>(a.n / a.nb));
+        break;
+      }
       case ARG_REPS: a.reps = atoi(optarg); break;
       case ARG_DEV: a.dev = atoi(optarg); break;
       case ARG_NT: a.nt = atoi(optarg); break;
@@ -192,7 +213,7 @@ CmdArgs get_args(int argc, char* argv[]) {
   }
   if (!alg_name(a.alg)[0]) {
     fprintf(stderr, "Error: alg %d is not provided by this driver (hierarchical algs 16/20/24, "
-                    "GPU BASE 2, and their +100 index forms)\n", a.alg);
+                    "GPU BASE 2, CPU BASE 0, and their +100 index forms)\n", a.alg);
     exit(EXIT_FAILURE);
   }
   printf("Params:\n   reps = %i\n   seed = %i\n   dev  = %i\n   n    = %ld (~%f GB, float)\n"
@@ -217,6 +238,11 @@ std::string make_filename(const std::string& prefix, int64_t n, uint32_t q, int 
   s << prefix << "_n-" << n << "_q-" << q << "_lr-" << lr << "_seed-" << seed << "_trivCheck-"
     << a.trivialCheck << "_randTrivCheck-" << a.randTrivialCheck << ".bin";
   return s.str();
+}
+
+bool file_exists(const std::string& f) {
+  std::ifstream in(f, std::ios::binary);
+  return in.good();
 }
 
 bool load_file(const std::string& f, void* dst, size_t bytes) {
@@ -293,13 +319,20 @@ void write_params(const CmdArgs& a) {
   if (!a.save_time) return;
   hipDeviceProp_t prop;
   char device[256] = "unknown";
-  if (hipGetDeviceProperties(&prop, a.dev) == hipSuccess) snprintf(device, sizeof device, "%s", prop.name);
+  const bool cpu = a.alg % 100 == 0;
+  if (cpu) {  // the CPU algorithms name the host (src/tools.h:359-365)
+    char host[64] = "";
+    (void)gethostname(host, sizeof host - 1);
+    snprintf(device, sizeof device, "CPU %s", host);
+  } else if (hipGetDeviceProperties(&prop, a.dev) == hipSuccess) {
+    snprintf(device, sizeof device, "%s", prop.name);
+  }
   char ts[32];
   std::time_t now = std::time(nullptr);
   std::strftime(ts, sizeof ts, "%Y-%m-%dT%H:%M:%SZ", std::localtime(&now));
   FILE* fp = fopen(rtxrmq::csv_path(a), "a");
   if (!fp) return;
-  const int block = 256;  // workgroup size of this library's kernels (the reference's BSIZE)
+  const int block = cpu ? 0 : 256;  // workgroup size of this library's kernels (the reference's BSIZE)
   if (is_hier(a.alg)) {
     fprintf(fp, "%s,%s,%i,%i,%ld,%i,%ld,%u,%d,%i,%i,%i,%s,%lld,%u,%u,", device, alg_name(a.alg),
             a.nt, a.reps, (long)a.n, 1, 0L, a.q, a.lr, block, 0, 0, ts, 1ll << a.log_bs,
@@ -430,6 +463,10 @@ int run(CmdArgs a) {
     case 2:
       out = rtxrmq::gpu_rmq_basic(a.n, a.q, dA, dQ, a, idx_mode);
       break;
+    case 0:
+      out = idx_mode ? static_cast<void*>(rtxrmq::cpu_rmq_idx<P>(a.q, A.data(), Q.data(), a))
+                     : rtxrmq::cpu_rmq<P>(a.q, A.data(), Q.data(), a);
+      break;
   }
 
   const int save_time = a.save_time;
@@ -437,34 +474,65 @@ int run(CmdArgs a) {
     printf(a.check ? "\nCHECKING RESULT:\n"
                    : (a.trivialCheck ? "\nTRIVIALLY CHECKING RESULT:\n"
                                      : "\nRANDOMLY TRIVIALLY CHECKING RESULT:\n"));
+    // the queries whose expected answers GPU BASE computes (src/main.cu:194-283): all of them
+    // (--check), the first 2^10 (--trivialCheck: the queries repeat with that period,
+    // src/rand.cuh:172-179), or a sample of 2^15 (--randTrivialCheck); pos[i] is the query
+    // expected answer i is compared with (every query for --trivialCheck, expected answer
+    // i & (2^10 - 1), as fill_repeatedly does)
     std::vector<uint32_t> pos;
     std::vector<P> qs;
-    if (a.check) {
-      pos.resize(a.q);
-      for (uint32_t i = 0; i < a.q; ++i) pos[i] = i;
-    } else if (a.trivialCheck) {
-      // queries repeat with period 2^10 (src/rand.cuh:172-179): check every position
-      pos.resize(a.q);
-      for (uint32_t i = 0; i < a.q; ++i) pos[i] = i;
-    } else {
-      const uint64_t k = std::min<uint64_t>(a.q, 1ull << kRandTrivLog);
+    uint64_t k = a.q;
+    if (a.trivialCheck) {
+      k = std::min<uint64_t>(a.q, 1ull << kTrivLog);
+    } else if (a.randTrivialCheck) {
+      k = std::min<uint64_t>(a.q, 1ull << kRandTrivLog);
       pos.resize(k);
       rmq_gen_sample_indices(a.q, k, a.seed, pos.data());
     }
-    qs.resize(pos.size() * 2);
-    for (size_t i = 0; i < pos.size(); ++i) {
+    if (pos.empty()) {
+      pos.resize(k);
+      for (uint32_t i = 0; i < k; ++i) pos[i] = i;
+    }
+    qs.resize(k * 2);
+    for (size_t i = 0; i < k; ++i) {
       qs[2 * i] = Q[2ull * pos[i]];
       qs[2 * i + 1] = Q[2ull * pos[i] + 1];
     }
-    (void)kTrivLog;
     std::vector<float> ev;
     std::vector<P> ei;
+    // expected answers from the $RMQ_AUX_DIR cache when present (values need both files), else
+    // from GPU BASE, cached under --save-input-data; the names carry the checked count
+    // (get_or_generate_expected_val / _index, src/cuda_methods.cuh:175-210)
+    const std::string fv = make_filename(aux_dir() + "results_value", a.n, static_cast<uint32_t>(k), a.lr, a.seed, a);
+    const std::string fi = make_filename(aux_dir() + "results_index", a.n, static_cast<uint32_t>(k), a.lr, a.seed, a);
+    ev.resize(k);
+    ei.resize(k);
+    const bool cached = idx_mode ? load_file(fi, ei.data(), k * sizeof(P))
+                                 : (file_exists(fv) && file_exists(fi) && load_file(fi, ei.data(), k * sizeof(P)) &&
+                                    load_file(fv, ev.data(), k * 4));
     printf("Checking result...........................");
     fflush(stdout);
-    expected(dA, a.n, qs, ev, ei);
-    if (a.save_input_data && a.check) {
-      save_file(make_filename(aux_dir() + "results_value", a.n, a.q, a.lr, a.seed, a), ev.data(), ev.size() * 4);
-      save_file(make_filename(aux_dir() + "results_index", a.n, a.q, a.lr, a.seed, a), ei.data(), ei.size() * sizeof(P));
+    if (!cached) {
+      expected(dA, a.n, qs, ev, ei);
+      if (a.save_input_data) {
+        if (!idx_mode) save_file(fv, ev.data(), ev.size() * 4);
+        save_file(fi, ei.data(), ei.size() * sizeof(P));
+      }
+    } else if (idx_mode) {
+      for (size_t i = 0; i < k; ++i) ev[i] = A[ei[i]];  // for the messages only
+    }
+    if (a.trivialCheck) {  // every query against expected answer i & (2^10 - 1)
+      const uint64_t period = k;
+      std::vector<float> rv(a.q);
+      std::vector<P> ri(a.q);
+      pos.resize(a.q);
+      for (uint32_t i = 0; i < a.q; ++i) {
+        pos[i] = i;
+        rv[i] = ev[i % period];
+        ri[i] = ei[i % period];
+      }
+      ev.swap(rv);
+      ei.swap(ri);
     }
     const bool pass = compare(A, Q, out, idx_mode, ev, ei, pos);
     a.save_time = save_time;

@@ -11,14 +11,19 @@
 // CPU BASE IDX, src/cpu_methods.h:42-53). Every entry point is a template on the position type
 // P, the reference's nType (src/main.cu:21-27): uint32_t, or uint64_t for its IS_LONG build
 // (64-bit {l, r} pairs and indices, rmq_query64 / rmq_gpu_base64).
+// cpu_rmq / cpu_rmq_idx (algs 0 / 100) are the reference's CPU BASE (src/cpu_methods.h:18-39,
+// 55-77): a scan per query on the host, OpenMP over queries with --nt threads.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
+#include <omp.h>
+#include <unistd.h>
 
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <ctime>
 #include <string>
 
@@ -228,6 +233,64 @@ inline void* gpu_rmq_basic(int64_t n, uint32_t q, const float* d_array, const P*
   (void)hipMemcpy(out, d_out, out_bytes, hipMemcpyDeviceToHost);
   (void)hipFree(d_out);
   return out;
+}
+
+// ---- CPU BASE (algs 0 / 100) --------------------------------------------------------------
+// Leftmost minimum of A[l..r]: the first element seeds it and a later one replaces it only if
+// strictly smaller (cpu_min / cpu_min_idx, src/cpu_methods.h:7-16, 42-53), so the value is
+// A[index] bit for bit and a NaN at A[l] is kept.
+template <typename P>
+inline P cpu_scan_leftmost(const float* A, P l, P r) {
+  float m = A[l];
+  P at = l;
+  for (P i = l + 1; i <= r; ++i) {
+    const float x = A[i];
+    if (x < m) {
+      m = x;
+      at = i;
+    }
+  }
+  return at;
+}
+
+// reps passes over the q queries (P pairs {l, r}) on the host with a.nt OpenMP threads, timed on
+// the host wall clock; returns a malloc'd float[q] of values, or P[q] of indices.
+template <typename P>
+inline void* cpu_rmq_base(uint32_t q, const float* A, const P* Q, const CmdArgs& a, bool want_idx) {
+  omp_set_num_threads(a.nt > 0 ? a.nt : 1);
+  if (want_idx)
+    printf("Computing RMQs index (%-11s, nt=%2i,r=%-3i)..", "[CPU] BASE IDX", a.nt, a.reps);
+  else
+    printf("Computing RMQs (%-11s, nt=%2i,r=%-3i)..", "[CPU] BASE", a.nt, a.reps);
+  fflush(stdout);
+  void* out = malloc(static_cast<size_t>(q) * (want_idx ? sizeof(P) : sizeof(float)) + 8);
+  P* oi = static_cast<P*>(out);
+  float* ov = static_cast<float*>(out);
+  Timer timer;
+  for (int rep = 0; rep < a.reps; ++rep) {
+#pragma omp parallel for
+    for (int64_t k = 0; k < static_cast<int64_t>(q); ++k) {
+      const P at = cpu_scan_leftmost<P>(A, Q[2 * k], Q[2 * k + 1]);
+      if (want_idx) oi[k] = at;
+      else ov[k] = A[at];
+    }
+  }
+  const double ms = timer.ms();
+  const double it = ms / a.reps;
+  printf("done: %f secs (avg %f secs): [%.2f RMQs/sec, %f nsec/RMQ]\n", ms / 1000.0, ms / (1000.0 * a.reps),
+         q / (it / 1000.0), it * 1e6 / q);
+  write_results(ms, q, 0, a.reps, a, VBHMem{});
+  return out;
+}
+
+template <typename P>
+inline float* cpu_rmq(uint32_t q, const float* A, const P* Q, const CmdArgs& a) {
+  return static_cast<float*>(cpu_rmq_base<P>(q, A, Q, a, false));
+}
+
+template <typename P>
+inline P* cpu_rmq_idx(uint32_t q, const float* A, const P* Q, const CmdArgs& a) {
+  return static_cast<P*>(cpu_rmq_base<P>(q, A, Q, a, true));
 }
 
 }  // namespace rtxrmq
