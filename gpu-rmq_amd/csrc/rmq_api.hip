@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_top_row(const float* __restrict__ to
     a = prev[i];
     b = prev[i + h];
   }
-  tt[static_cast<size_t>(j - 1u) * stride + i] = dev::u2f(b.x) < dev::u2f(a.x) ? b : a;
+  tt[static_cast<size_t>(j - 1u) * stride + i] = dev::top_take_right(dev::u2f(a.x), dev::u2f(b.x)) ? b : a;
 }
 
 // The whole table of a top level of at most kTopSmall entries in one workgroup.
@@ -110,7 +110,7 @@ struct rmq_hierarchy {
   float* lv;
   uint8_t* lr;      // offset entries of levels 1.. (HierView::lr)
   uint64_t rel_bytes = 0;
-  uint32_t* zeros;  // device word: kZeroPos | kZeroNeg of level 0
+  uint32_t* zeros;  // device word: kZeroPos | kZeroNeg | kHasNaN of level 0
   uint2* tt = nullptr;  // sparse table over the top level (null when not built)
   HierView view;
   // 64-bit form: chunks of 2^chunk_log elements, each its own hierarchy (chunk_log = 0: one
@@ -602,8 +602,8 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
     H.off[k] = k < P.num_levels ? static_cast<uint32_t>(P.off[k]) : 0;
   }
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  // the level-0 pass records the zero signs; with no such pass both are assumed
-  rmq::k_init_words<<<1, 1, 0, s>>>(h->zeros, P.num_levels > 1 ? 0u : (rmq::kZeroPos | rmq::kZeroNeg), d_array,
+  // the level-0 pass records the zero signs and NaN; with no such pass all are assumed
+  rmq::k_init_words<<<1, 1, 0, s>>>(h->zeros, P.num_levels > 1 ? 0u : rmq::kLevel0Unknown, d_array,
                                     static_cast<uint32_t>(n));
   if (last_error() != RMQ_OK) {
     rmq_free(h);

@@ -44,7 +44,7 @@ struct rmq_batch {
   uint32_t* d_variant;
   float* lv;
   uint8_t* lr;      // offset entries of every array's levels 1..
-  uint32_t* zeros;  // per array: kZeroPos | kZeroNeg of its level 0
+  uint32_t* zeros;  // per array: kZeroPos | kZeroNeg | kHasNaN of its level 0
   float4* tails;    // per array: the copy of level 0's last 16-byte vector, +inf padded
   uint64_t level_floats;
   uint64_t rel_bytes;
@@ -111,6 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
               dev::zero_bits(x.z, pos + 2 < ssize) | dev::zero_bits(x.w, pos + 3 < ssize);
     }
     dev::group_argmin<W>(bv, bi);
+    if (k == 0 && bi == dev::kNoIdx) bi = j * kR;  // a block of NaNs: its first position, value +inf
     if (gl == 0) {
       dv[j] = bv;
       dev::rel_store(drel, j, RL * (k + 1), bi);
@@ -119,13 +120,13 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
   if (k == 0) dev::publish_zeros(const_cast<uint32_t*>(H.zeros), zb);
 }
 
-// Every array's zero-sign word (cleared for the level-0 pass, or both signs when there is none)
+// Every array's level-0 word (cleared for the level-0 pass, or every bit when there is none)
 // and its copy of level 0's last vector.
 __global__ void k_init_zeros(const HierView* __restrict__ views, uint32_t count) {
   const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a < count) {
     const HierView& H = views[a];
-    *const_cast<uint32_t*>(H.zeros) = H.num_levels > 1 ? 0u : (kZeroPos | kZeroNeg);
+    *const_cast<uint32_t*>(H.zeros) = H.num_levels > 1 ? 0u : kLevel0Unknown;
     dev::fill_tail(H.A, H.size[0], const_cast<float*>(H.tail));
   }
 }

@@ -39,8 +39,10 @@ __host__ __device__ constexpr uint32_t rel_bytes(uint32_t span_log) {
 
 // A value-only answer must carry the bit pattern of A[leftmost argmin]; a plain minimum can only
 // miss it when the minimum is zero and the array holds both +0 and -0. The level-0 build pass
-// records which signs occur; a hierarchy with no such pass (one level) records both.
-constexpr uint32_t kZeroPos = 1u, kZeroNeg = 2u;
+// records which signs occur, and whether the array holds a NaN (kHasNaN: queries then apply
+// CPU BASE's NaN rule, dev::nan_at_l); a hierarchy with no such pass (one level) records all.
+constexpr uint32_t kZeroPos = 1u, kZeroNeg = 2u, kHasNaN = 4u;
+constexpr uint32_t kLevel0Unknown = kZeroPos | kZeroNeg | kHasNaN;
 
 namespace dev {
 
@@ -177,27 +179,44 @@ __device__ __forceinline__ void fill_tail(const float* __restrict__ A, uint32_t 
   for (uint32_t j = 0; j < 4; ++j) tail[j] = p + j < n ? A[p + j] : RMQ_INF;
 }
 
-// The sign bits of the zeros among x (kZeroPos / kZeroNeg), if `ok`.
+// What x tells about level 0, if `ok`: the sign bit of a zero (kZeroPos / kZeroNeg), kHasNaN
+// for a NaN.
 __device__ __forceinline__ uint32_t zero_bits(float x, bool ok) {
   const uint32_t b = __float_as_uint(x);
-  return ok && (b & 0x7FFFFFFFu) == 0 ? (1u << (b >> 31)) : 0u;
+  const uint32_t z = (b & 0x7FFFFFFFu) == 0 ? (1u << (b >> 31)) : 0u;
+  return ok ? (z | (x != x ? kHasNaN : 0u)) : 0u;
 }
 
 // A zero minimum v of a value-only query, given the array's zero signs: true (and v set to the
 // right zero) unless both signs occur, when the query must be answered with index tracking.
 __device__ __forceinline__ bool zero_known(uint32_t zeros, float& v) {
-  if (zeros == (kZeroPos | kZeroNeg)) return false;
+  if ((zeros & (kZeroPos | kZeroNeg)) == (kZeroPos | kZeroNeg)) return false;
   v = (zeros & kZeroNeg) ? -0.0f : 0.0f;
   return true;
 }
 
-// OR of the wave's `bits` (kZeroPos | kZeroNeg) into *zeros, one atomic per wave that saw any.
-// All lanes of the wave must call it.
+// OR of the wave's `bits` (kZeroPos | kZeroNeg | kHasNaN) into *zeros, one atomic per wave that
+// saw any. All lanes of the wave must call it.
 __device__ __forceinline__ void publish_zeros(uint32_t* zeros, uint32_t bits) {
   const uint64_t pos = __ballot((bits & kZeroPos) != 0u);
   const uint64_t neg = __ballot((bits & kZeroNeg) != 0u);
-  const uint32_t w = (pos ? kZeroPos : 0u) | (neg ? kZeroNeg : 0u);
+  const uint64_t nan = __ballot((bits & kHasNaN) != 0u);
+  const uint32_t w = (pos ? kZeroPos : 0u) | (neg ? kZeroNeg : 0u) | (nan ? kHasNaN : 0u);
   if (w && (threadIdx.x & 63u) == 0) atomicOr(zeros, w);
+}
+
+// CPU BASE's NaN rule (src/cpu_methods.h:7-16, 42-53): the scan starts from A[l] and only a
+// strictly smaller element replaces it, so a NaN at A[l] is the answer and a NaN anywhere else
+// never is. The solvers never take a NaN (every comparison with one is false), so they give the
+// answer in every other case; this supplies the first, on arrays whose level-0 pass saw a NaN
+// (flags: the hierarchy's level-0 word) and only there loads A[l]. True (and v = A[l]) when it
+// applies.
+__device__ __forceinline__ bool nan_at_l(uint32_t flags, const float* __restrict__ A, uint64_t l, float& v) {
+  if (!(flags & kHasNaN)) return false;
+  const float x = A[l];
+  if (x == x) return false;
+  v = x;
+  return true;
 }
 
 // Final ranges on the top level longer than this are answered from the sparse table.
@@ -208,6 +227,10 @@ constexpr uint32_t kTopScan = 8;
 // [i, i + 2^j). With j = floor(log2(b - a)) the entries at a and b - 2^j cover [a, b); on equal
 // values the left one holds the leftmost position, since any position of that value in the
 // right entry's span left of it lies in the left entry's span too.
+// Entry b replaces entry a (a left of b) in a sparse-table row: strictly smaller, or a holds a NaN
+// and b does not (a NaN never wins; ties keep the left entry, which holds the leftmost position).
+__device__ __forceinline__ bool top_take_right(float a, float b) { return b < a || (a != a && b == b); }
+
 __device__ __forceinline__ void top_lookup(const uint2* __restrict__ tt, uint32_t stride, uint32_t a,
                                            uint32_t b, float& v, uint32_t& pos) {
   const uint32_t j = 31u - static_cast<uint32_t>(__clz(b - a));

@@ -53,6 +53,10 @@ __global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict_
       }
     }
     dev::group_argmin<W>(bv, bi);
+    // a level-0 block of NaNs (no element is ever taken): its first position, value +inf, which
+    // never wins over a real element left of it and is never asked for on its own (a query
+    // whose range holds no other element starts at a NaN, dev::nan_at_l)
+    if (!SRC_IDX && bi == dev::kNoIdx) bi = base;
     if (gl == 0) {
       dv[j] = bv;
       dev::rel_store(drel, static_cast<uint32_t>(j), dsk, bi);
@@ -82,7 +86,7 @@ __device__ __forceinline__ void top_table_block(const float* __restrict__ top, u
       const uint32_t i = threadIdx.x + r * kTopThreads;
       if (i < cnt) {
         const uint2 a = cur[i], b = cur[i + h];
-        nv[r] = dev::u2f(b.x) < dev::u2f(a.x) ? b : a;
+        nv[r] = dev::top_take_right(dev::u2f(a.x), dev::u2f(b.x)) ? b : a;
       }
     }
     __syncthreads();
@@ -281,7 +285,9 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
         const float xs[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) {
-          const bool t = dev::in_range(p + j, lk, ek) && (xs[j] < bv || bpos == dev::kNoIdx);
+          // the first element seeds the lane (an all-+inf range answers with its leftmost), but
+          // never a NaN: a lane holding one would disagree with its group in the reduction
+          const bool t = dev::in_range(p + j, lk, ek) && (xs[j] < bv || (bpos == dev::kNoIdx && xs[j] == xs[j]));
           bv = t ? xs[j] : bv;
           bpos = t ? p + j : bpos;
         }
@@ -367,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ 
   const uint32_t gl = lane & (G - 1);
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   const bool valid = qi < q;
-  const uint32_t zeros = IDX ? 0u : *H.zeros;
+  const uint32_t zeros = *H.zeros;  // zero signs and the NaN bit of level 0
   uint32_t l = 0, e = 0;
   if (valid) {
     const uint2 p = dev::load_query(Q, qi);
@@ -399,6 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ 
     }
   }
   if (valid) {
+    if (dev::nan_at_l(zeros, H.A, l, my_v)) my_i = l;
     if (out_v) out_v[qi] = my_v;
     if (IDX && out_i) out_i[qi] = my_i;
   }
@@ -419,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
   const uint32_t gl = threadIdx.x & (G - 1);
   const uint64_t grp = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / G;
   const uint64_t ngrp = static_cast<uint64_t>(gridDim.x) * kBlock / G;
-  const uint32_t zeros = IDX ? 0u : *H.zeros;
+  const uint32_t zeros = *H.zeros;
   for (uint64_t qi = grp; qi < q; qi += ngrp) {
     const uint2 p = dev::load_query(Q, qi);
     float v;
@@ -432,6 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
       v = value_exact<G, RL, P>(p.x, p.y + 1, gl, H, T, zeros);
     }
     if (gl == 0) {
+      if (dev::nan_at_l(zeros, H.A, p.x, v)) i = p.x;
       if (out_v) out_v[qi] = v;
       if (IDX && out_i) out_i[qi] = i;
     }
@@ -763,6 +771,18 @@ __device__ __forceinline__ GroupAnswer solve_lane(uint32_t l, uint32_t e, const 
   else return solve_vl<RL, IDX>(l, e, H);
 }
 
+// CPU BASE's NaN rule for a lane's query (dev::nan_at_l): a NaN at A[l] is stored as the answer
+// (value A[l], index l) and the query needs no solver. A uniform branch on NaN-free arrays.
+template <bool IDX>
+__device__ __forceinline__ bool emit_nan_at_l(uint32_t flags, const float* __restrict__ A, uint32_t l, uint64_t qi,
+                                              float* __restrict__ out_v, uint32_t* __restrict__ out_i) {
+  float v;
+  if (!dev::nan_at_l(flags, A, l, v)) return false;
+  if (out_v) out_v[qi] = v;
+  if (IDX && out_i) out_i[qi] = l;
+  return true;
+}
+
 template <int RL, bool P, bool IDX>
 __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q, uint32_t q,
                                                      float* __restrict__ out_v,
@@ -772,10 +792,11 @@ __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q
   if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
   const LevelsLds T{s_meta};
-  const uint32_t zeros = IDX ? 0u : *H.zeros;
+  const uint32_t zeros = *H.zeros;
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (qi >= q) return;
   const uint2 p = dev::load_query(Q, qi);
+  if (emit_nan_at_l<IDX>(zeros, H.A, p.x, qi, out_v, out_i)) return;
   GroupAnswer a = solve_lane<RL, P, IDX>(p.x, p.y + 1, H, T);
   if constexpr (!IDX) {
     if (a.v == 0.0f && !dev::zero_known(zeros, a.v)) a = solve_lane<RL, P, true>(p.x, p.y + 1, H, T);
@@ -819,17 +840,19 @@ __global__ __launch_bounds__(kBlock) void k_query_vlc(const uint2* __restrict__ 
   if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
   const LevelsLds T{s_meta};
-  const uint32_t zeros = IDX ? 0u : *H.zeros;
+  const uint32_t zeros = *H.zeros;
   const uint32_t w = threadIdx.x >> 6;
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   bool need = false;
   VlState st{};
   if (qi < q) {
     const uint2 p = dev::load_query(Q, qi);
-    st = vl_start<RL, IDX, true>(p.x, p.y + 1, H, T);
-    if (st.la || st.ra) vl_step<RL, IDX>(st, H, T);
-    need = st.la || st.ra;  // k >= 0 whenever a tail is open
-    if (!need) vl_emit<RL, IDX>(st, qi, H, T, zeros, out_v, out_i);
+    if (!emit_nan_at_l<IDX>(zeros, H.A, p.x, qi, out_v, out_i)) {
+      st = vl_start<RL, IDX, true>(p.x, p.y + 1, H, T);
+      if (st.la || st.ra) vl_step<RL, IDX>(st, H, T);
+      need = st.la || st.ra;  // k >= 0 whenever a tail is open
+      if (!need) vl_emit<RL, IDX>(st, qi, H, T, zeros, out_v, out_i);
+    }
   }
   const uint64_t m = __ballot(need);
   if (need) {

@@ -68,7 +68,11 @@ __global__ __launch_bounds__(kBlock) void k_query64(const ulonglong2* __restrict
   const GroupAnswer a = solve_chunk<RL>(HL, table(cl), static_cast<uint32_t>(l - bl), ea);
   float bv = a.v;
   uint64_t bi = bl + a.idx;
-  if (cl != cr) {
+  // CPU BASE's NaN rule, for the query's own first element only: the chunks' minima and the
+  // last chunk's prefix are answered without it (a NaN there never wins)
+  if (dev::nan_at_l(*HL.zeros, HL.A, l - bl, bv)) {
+    bi = l;
+  } else if (cl != cr) {
     for (uint32_t c = cl + 1; c < cr; ++c) {  // chunks covered whole, left to right
       const float v = cv[c];
       if (v < bv) {

@@ -123,6 +123,7 @@ __device__ __forceinline__ void unit_groups(const HierView& H, const LT& T, cons
     }
   }
   if (valid) {
+    if (dev::nan_at_l(*H.zeros, H.A, l, my_v)) my_i = l;
     if (ov) ov[q0 + local] = my_v;
     if (IDX && oi) oi[q0 + local] = my_i;
   }
@@ -135,6 +136,7 @@ __device__ __forceinline__ void unit_lanes(const HierView& H, const LT& T, const
   const uint32_t local = threadIdx.x & 63;
   if (local >= count) return;
   const uint2 p = dev::load_query(Q, q0 + local);
+  if (emit_nan_at_l<IDX>(*H.zeros, H.A, p.x, q0 + local, ov, oi)) return;
   GroupAnswer a = solve_vl_pruned<RL, IDX, false>(p.x, p.y + 1, H, T);
   if constexpr (!IDX) {
     if (a.v == 0.0f && !dev::zero_known(*H.zeros, a.v)) a = solve_vl_pruned<RL, true, false>(p.x, p.y + 1, H, T);

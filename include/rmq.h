@@ -19,7 +19,11 @@
  *  - Queries are `uint32_t[2*q]` = q pairs {l, r}, inclusive, 0 <= l <= r < n, exactly the
  *    reference's uint2 layout (src/main.cu:25-26, src/rand.cuh:79-80).
  *  - Results are exact: value = A[leftmost argmin], index = leftmost argmin under a strict
- *    `<` scan from l to r, identical to CPU BASE (src/cpu_methods.h:7-16, 42-53).
+ *    `<` scan from l to r, identical to CPU BASE (src/cpu_methods.h:7-16, 42-53). That includes
+ *    NaN input: the scan starts from A[l] and a later element replaces the minimum only if
+ *    strictly smaller, so a NaN at A[l] answers (A[l], l), and a NaN anywhere else never wins
+ *    (a range whose other elements are all NaN answers with A[l]). The build records whether the
+ *    array holds a NaN, so queries on NaN-free arrays skip the A[l] test.
  *  - Errors are status codes instead of the reference's exit() (src/xxx.h:211-217).
  *  - A built hierarchy is immutable; concurrent rmq_query calls on different streams are
  *    allowed. The hierarchy borrows d_array: it must stay valid and unchanged until rmq_free.

@@ -46,7 +46,11 @@ constexpr u32 kNoIdx = 0xFFFFFFFFu;
 
 // Strict `<` scan from l to r (inclusive), src/cpu_methods.h:8-15 and 43-52.
 inline u32 cpu_min_idx(const float* A, u32 l, u32 r) {
+#if ORACLE_MUTANT == 7
+  float m = INFINITY;  // planted: the scan is seeded with +inf instead of A[l]
+#else
   float m = A[l];
+#endif
   u32 mi = l;
   for (u64 i = l; i <= r; ++i) {
 #if ORACLE_MUTANT == 1
@@ -62,7 +66,11 @@ inline u32 cpu_min_idx(const float* A, u32 l, u32 r) {
 }
 
 inline float cpu_min(const float* A, u32 l, u32 r) {
+#if ORACLE_MUTANT == 7
+  float m = INFINITY;
+#else
   float m = A[l];
+#endif
 #if ORACLE_MUTANT == 2
   for (u64 i = l; i < r; ++i) {  // planted: drops the last element
 #else

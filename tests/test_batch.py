@@ -255,3 +255,52 @@ def test_config4_full_workload(device):
             assert np.array_equal(got, oracle.CpuHierarchy(Aa, 64, 5, 16).query(lr)[1]), (a, var[a], ts[a])
     del dA, dQ, bi
     torch.cuda.empty_cache()
+
+
+@pytest.mark.gpu
+def test_batch_nan_arrays(device):
+    """Arrays holding NaN in the batched form: a NaN at A[l] answers l, elsewhere a NaN never wins
+    (CPU BASE, the oracle), for every variant, value and index."""
+    import torch
+
+    w, dA, dQ = _device_workload(64, (4, 16), (2, 11), seed=71, kind="mixed")
+    A_host = dA.cpu().numpy()
+    rng = np.random.default_rng(71)
+    for a in range(64):
+        o, n = int(w.offsets[a]), int(w.sizes[a])
+        k = a % 4
+        if k == 0:
+            A_host[o + rng.choice(n, max(1, n // 20), replace=False)] = np.nan
+        elif k == 1:
+            s = int(rng.integers(0, n))
+            A_host[o + s:o + min(n, s + 40 * (a + 1))] = np.nan
+        elif k == 2:
+            A_host[o:o + n] = np.nan
+        q0, q1 = int(w.qoffsets[a]), int(w.qoffsets[a + 1])
+        nanpos = np.nonzero(np.isnan(A_host[o:o + n]))[0]
+        if nanpos.size:  # some queries start on a NaN
+            sel = w.queries[q0:q1:3]
+            sel[:, 0] = np.minimum(nanpos[rng.integers(0, nanpos.size, sel.shape[0])], sel[:, 1])
+            w.queries[q0:q1:3] = sel
+    dA.copy_(torch.from_numpy(A_host))
+    dQ = torch.from_numpy(w.queries.view(np.int32)).cuda()
+    variants = ["lane", "lane8", "lane4", "group8", "group4", "group16", "wave"]
+    var = [variants[a % 7] for a in range(64)]
+    t = [64 << (a % 3) for a in range(64)]
+    for lanes_only in (False, True):
+        vv = [("lane", "lane8", "lane4")[a % 3] for a in range(64)] if lanes_only else var
+        with rmq_amd.Batch(dA, w.offsets, w.sizes, t, vv) as b:
+            plan = b.plan(w.qoffsets)
+            bv, bi = plan.query(dQ, True, True)
+            ov, _ = plan.query(dQ, True, False)
+            torch.cuda.synchronize()
+        bv, ov = bv.cpu().numpy().view(np.uint32), ov.cpu().numpy().view(np.uint32)
+        bi = bi.cpu().numpy().view(np.uint32)
+        for a in range(64):
+            o, n = int(w.offsets[a]), int(w.sizes[a])
+            q0, q1 = int(w.qoffsets[a]), int(w.qoffsets[a + 1])
+            Aa = A_host[o:o + n]
+            ref = oracle.cpu_base_index(Aa, w.queries[q0:q1])
+            assert np.array_equal(bi[q0:q1], ref), (a, vv[a])
+            assert np.array_equal(bv[q0:q1], Aa[ref].view(np.uint32)), (a, vv[a])
+            assert np.array_equal(ov[q0:q1], Aa[ref].view(np.uint32)), (a, vv[a], "value")

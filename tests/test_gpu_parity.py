@@ -544,3 +544,74 @@ def test_query_count_above_one_launch(device):
         del v, i
     del dQ, bi
     torch.cuda.empty_cache()
+
+
+def _nan_cases():
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nan_vectors.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def test_nan_vectors(device):
+    """Arrays holding NaN follow CPU BASE's rule (src/cpu_methods.h:7-16, 42-53): a NaN at A[l]
+    answers l, anywhere else a NaN never wins. Every variant, a hierarchy of several levels and
+    one of one level (the sparse table over the array itself), value + index, value only and
+    index only, against the fixture's known answers (value bits name the position: every NaN
+    has its own payload)."""
+    for case in _nan_cases():
+        A = np.array(case["array_bits"], dtype=np.uint32).view(np.float32)
+        lr = np.array(case["lr"], dtype=np.uint32)
+        want_i = np.array(case["index"], dtype=np.uint32)
+        want_v = np.array(case["value_bits"], dtype=np.uint32)
+        for alg, opts in VARIANTS:
+            for t in (_min_t(alg, opts), 1 << 15):
+                _say(f"nan {case['name']} alg={alg} opts={opts} t={t}")
+                v, i = run_gpu(A, lr, alg, t, True, True, **opts)
+                assert np.array_equal(i, want_i), (case["name"], alg, opts, t, np.nonzero(i != want_i)[0][:5])
+                assert np.array_equal(v.view(np.uint32), want_v), (case["name"], alg, opts, t)
+                v, _ = run_gpu(A, lr, alg, t, True, False, **opts)
+                assert np.array_equal(v.view(np.uint32), want_v), (case["name"], alg, opts, t, "value")
+                _, i = run_gpu(A, lr, alg, t, False, True, **opts)
+                assert np.array_equal(i, want_i), (case["name"], alg, opts, t, "index")
+
+
+@pytest.mark.parametrize("pattern", ["scattered", "runs"])
+def test_nan_random_against_oracle(device, pattern):
+    """n = 2^20 + 3 with NaN planted (1% scattered, or runs of 1..5000 covering whole blocks of
+    every level), mixed lengths plus queries that start, end or sit next to a NaN: every variant,
+    value and index, against the oracle's CPU BASE; GPU BASE agrees too."""
+    import torch
+
+    rng = np.random.default_rng({"scattered": 1, "runs": 2}[pattern])
+    n = (1 << 20) + 3
+    A = rmq_amd.gen_array(n, 5)
+    if pattern == "scattered":
+        pos = rng.choice(n, n // 100, replace=False)
+    else:
+        pos = np.concatenate([np.arange(s, min(n, s + int(L))) for s, L in
+                              zip(rng.integers(0, n, 300), np.exp2(rng.uniform(0, 12.3, 300)))])
+    A[pos] = np.uint32(0x7FC00000 | 77).view(np.float32)
+    near = pos[rng.choice(pos.size, 3000)].astype(np.int64)
+    lens = np.exp2(rng.uniform(0, 20, near.size)).astype(np.int64)
+    starts = np.clip(near - rng.integers(0, 2, near.size) * lens, 0, n - 1)
+    ends = np.clip(starts + lens, 0, n - 1)
+    lr = np.concatenate([np.stack([starts, ends], 1), np.stack([near, near], 1),
+                         rmq_amd.gen_queries(1 << 13, -6, n, 9).astype(np.int64)]).astype(np.uint32)
+    lr = np.sort(lr, axis=1)
+    ref_i = oracle.cpu_base_index(A, lr)
+    for alg, opts in VARIANTS:
+        t = 1 << 13 if alg == 20 else 1 << 15
+        _say(f"nan random {pattern} alg={alg} opts={opts}")
+        v, i = run_gpu(A, lr, alg, t, True, True, **opts)
+        assert np.array_equal(i, ref_i), (alg, opts, np.nonzero(i != ref_i)[0][:5])
+        assert np.array_equal(v.view(np.uint32), A[ref_i].view(np.uint32)), (alg, opts)
+        v, _ = run_gpu(A, lr, alg, t, True, False, **opts)
+        assert np.array_equal(v.view(np.uint32), A[ref_i].view(np.uint32)), (alg, opts, "value")
+    short = np.nonzero((lr[:, 1].astype(np.int64) - lr[:, 0]) < (1 << 16))[0]
+    sq = torch.from_numpy(np.ascontiguousarray(lr[short]).view(np.int32)).cuda()
+    _, bi = rmq_amd.gpu_base(torch.from_numpy(A).cuda(), sq, False, True)
+    torch.cuda.synchronize()
+    assert np.array_equal(bi.cpu().numpy().view(np.uint32), ref_i[short])

@@ -44,11 +44,15 @@ def test_chunked_small_against_cpu_base(device, alg, opts):
 
     rng = np.random.default_rng(alg * 10 + len(opts))
     n, clog = 100003, 12
-    for kind in ("uniform", "ties"):
+    for kind in ("uniform", "ties", "nan"):
         A = rng.random(n, dtype=np.float32)
         if kind == "ties":
             A = rng.integers(0, 3, size=n).astype(np.float32)
             A[rng.random(n) < 0.5] *= np.float32(-1.0)  # +0 and -0 in many chunks
+        elif kind == "nan":  # NaN at every chunk start, one chunk all NaN, 1% scattered
+            A[:: 1 << clog] = np.nan
+            A[3 << clog:4 << clog] = np.nan
+            A[rng.random(n) < 0.01] = np.nan
         lr = _chunk_queries(rng, n, 1 << clog)
         ref = _cpu_ref(A, lr)
         dA = torch.from_numpy(A).cuda()

@@ -42,7 +42,26 @@ def _numpy_answers(A, lr):
     return idx, A[idx]
 
 
+def _nan_vectors():
+    with open(os.path.join(GOLDEN, "nan_vectors.json")) as f:
+        return json.load(f)
+
+
 # ---- checks (return True when the oracle L passes) ------------------------------------------
+
+
+def check_nan_vectors(L) -> bool:
+    """CPU BASE on arrays holding NaN: a NaN at A[l] answers l, elsewhere it never wins (the
+    fixture states the rule directly, tests/golden/make_nan_vectors.py)."""
+    for case in _nan_vectors():
+        A = np.array(case["array_bits"], dtype=np.uint32).view(np.float32)
+        lr = np.array(case["lr"], dtype=np.uint32)
+        if oracle.cpu_base_index(A, lr, L=L).tolist() != case["index"]:
+            return False
+        if oracle.cpu_base_value(A, lr, L=L).view(np.uint32).tolist() != case["value_bits"]:
+            return False
+    return True
+
 
 def check_cpu_base_vs_numpy(L) -> bool:
     rng = np.random.default_rng(7)
@@ -138,6 +157,7 @@ CHECKS = {
     "level_contents": check_level_contents,
     "coverage": check_coverage,
     "hier_vs_bruteforce": check_hier_vs_bruteforce,
+    "nan_vectors": check_nan_vectors,
 }
 
 
@@ -154,6 +174,7 @@ MUTANTS = {
     4: "level_contents",       # build block one element short
     5: "coverage",             # right partial block dropped by the traversal
     6: "tie_vectors",          # last equal candidate wins in the hierarchy
+    7: "nan_vectors",          # CPU BASE seeded with +inf instead of A[l]
 }
 
 
