@@ -137,6 +137,7 @@ def parse():
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched build and its query launch (no per-array baseline, no "
                          "graph, no other leg): the form rocprofv3 counter runs use")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving curve")
     ap.add_argument("--batch-arrays", type=int, default=4096)
     ap.add_argument("--serve-rates", default="20000,50000,100000,200000,400000",
                     help="offered requests/s of the serving curve (comma list)")
@@ -446,6 +447,10 @@ def main():
     os.environ.setdefault("OMP_PROC_BIND", "spread")
     if args.copy_probe:
         copy_probe()
+        return
+    if args.serve_only:
+        print(json.dumps({"serve": serve_leg([float(r) for r in args.serve_rates.split(",") if r],
+                                             args.serve_seconds)}), flush=True)
         return
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(args.batch_arrays, baseline=False, graph_too=False)}),

@@ -780,6 +780,7 @@ const char* rmq_status_str(rmq_status s) {
     case RMQ_ENOMEM: return "out of device memory";
     case RMQ_EDEVICE: return "HIP runtime error";
     case RMQ_EUNSUPPORTED: return "unsupported option";
+    case RMQ_EAGAIN: return "staging full, poll and submit again";
   }
   return "unknown status";
 }

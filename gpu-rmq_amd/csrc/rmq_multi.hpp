@@ -246,38 +246,48 @@ __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
   }
 }
 
-// Tiles of 256 queries per segment, in segment order sorted by array (segments of one array
-// adjacent), cut into kQueues contiguous queues of about equal query count. qoffsets[a] ..
-// qoffsets[a+1] are segment a's queries, on array ids[a] (ids == nullptr: array a). Returns the
-// tiles and the queue starts, or false when they would exceed kMaxTiles.
-inline bool make_query_tiles(uint32_t count, const uint64_t* qoffsets, std::vector<QueryTile>& tiles,
-                             std::vector<uint32_t>& qstart, const uint32_t* ids = nullptr) {
-  uint64_t need = 0;
-  for (uint32_t a = 0; a < count; ++a) need += (qoffsets[a + 1] - qoffsets[a] + kTileQueries - 1) / kTileQueries;
+// Tiles of 256 queries per segment: segment a holds cnt[a] queries starting at query index
+// start[a] of the launch's buffers, on array ids[a] (ids == nullptr: array a). Segments are taken
+// in order of array (segments of one array adjacent, otherwise in the given order), and the
+// tiles are cut into kQueues contiguous queues of about equal query count. Returns the tiles and
+// the queue starts, or false when they would exceed kMaxTiles.
+inline bool make_segment_tiles(uint32_t count, const uint64_t* start, const uint64_t* cnt, std::vector<QueryTile>& tiles,
+                               std::vector<uint32_t>& qstart, const uint32_t* ids = nullptr) {
+  uint64_t need = 0, total = 0;
+  for (uint32_t a = 0; a < count; ++a) {
+    need += (cnt[a] + kTileQueries - 1) / kTileQueries;
+    total += cnt[a];
+  }
   if (need > kMaxTiles) return false;
   std::vector<uint32_t> order(count);
   std::iota(order.begin(), order.end(), 0u);
   if (ids)
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return ids[x] < ids[y]; });
-  uint64_t total = 0;
-  for (uint32_t a = 0; a < count; ++a) total += qoffsets[a + 1] - qoffsets[a];
   tiles.clear();
   tiles.reserve(need);
   qstart.assign(kQueues + 1, 0);
   uint32_t queue = 0;
   uint64_t done = 0;
   for (uint32_t a : order) {
-    for (uint64_t q = qoffsets[a]; q < qoffsets[a + 1]; q += kTileQueries) {
+    for (uint64_t k = 0; k < cnt[a]; k += kTileQueries) {
       // queue k holds the tiles that start before k+1 eighths of all queries
       while (queue + 1 < kQueues && done * kQueues >= total * (queue + 1)) qstart[++queue] = static_cast<uint32_t>(tiles.size());
-      const uint64_t c = std::min<uint64_t>(kTileQueries, qoffsets[a + 1] - q);
-      tiles.push_back(QueryTile{ids ? ids[a] : a, static_cast<uint32_t>(c), q});
+      const uint64_t c = std::min<uint64_t>(kTileQueries, cnt[a] - k);
+      tiles.push_back(QueryTile{ids ? ids[a] : a, static_cast<uint32_t>(c), start[a] + k});
       done += c;
     }
   }
   while (queue + 1 < kQueues) qstart[++queue] = static_cast<uint32_t>(tiles.size());
   qstart[kQueues] = static_cast<uint32_t>(tiles.size());
   return true;
+}
+
+// The same for segments laid out back to back: segment a is [qoffsets[a], qoffsets[a + 1]).
+inline bool make_query_tiles(uint32_t count, const uint64_t* qoffsets, std::vector<QueryTile>& tiles,
+                             std::vector<uint32_t>& qstart, const uint32_t* ids = nullptr) {
+  std::vector<uint64_t> cnt(count);
+  for (uint32_t a = 0; a < count; ++a) cnt[a] = qoffsets[a + 1] - qoffsets[a];
+  return make_segment_tiles(count, qoffsets, cnt.data(), tiles, qstart, ids);
 }
 
 inline uint32_t persistent_grid(const void* kernel) {

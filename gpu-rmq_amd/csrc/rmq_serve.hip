@@ -1,12 +1,13 @@
 // rmq_serve.hip — serving loop over device-resident hierarchies (include/rmq_serve.h).
 //
-// Resident arrays are ordinary hierarchies (rmq_build_ex; R = 8 or 32 by variant) whose level views sit in a
-// device table; a batch is a set of pending requests packed into one slot's staging buffers and
-// answered by the multi-array persistent kernel (rmq_multi.hpp), one request = one segment of
-// tiles on its array. Each slot owns a stream and pinned, device-coherent host buffers that the
-// kernel reads (queries, tiles) and writes (answers) in place: a batch costs two kernel
-// launches and an event, with no copy commands, so the host never waits on a copy engine.
-// Up to `slots` batches overlap on the device while the host packs the next one.
+// Resident arrays are ordinary hierarchies (rmq_build_ex; R = 4, 8 or 32 by variant) whose level
+// views sit in a device table; a batch is a run of pending requests, one request = one segment of
+// tiles on its array, answered by the multi-array persistent kernel (rmq_multi.hpp). Requests are
+// staged once: submit checks and copies their pairs into a ring of pinned, device-coherent host
+// memory, which the kernel reads in place, and the kernel writes the answers into a second ring
+// at the same positions, from which retiring copies them to the caller. A batch is therefore two
+// kernel launches and an event, with no copy commands and no per-request allocation; up to
+// `slots` batches overlap on the device while the host stages and packs the next ones.
 
 #include <hip/hip_runtime.h>
 
@@ -32,35 +33,29 @@ struct Resident {
   uint32_t users = 0;      // pending + in-flight requests
 };
 
+// A request from submit until it is reported done. Its pairs (and then its answers) live at ring
+// positions [pos, pos + q) (absolute; the ring index is pos % ring_cap, and a request never wraps).
 struct Request {
   uint64_t ticket = 0;
   int32_t array = -1;
-  uint64_t q = 0;
-  std::vector<uint32_t> lr;
+  uint32_t q = 0;
+  uint64_t pos = 0;
   float* h_val = nullptr;
   uint32_t* h_idx = nullptr;
+  bool done = false;
 };
 
 struct Slot {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   bool busy = false;
-  bool want_idx = false;
-  uint64_t total = 0;
+  uint64_t t0 = 0, t1 = 0;  // tickets [t0, t1) of the batch
   // pinned host memory, coherent with the device; d_* are the device's addresses of the same
-  uint32_t* h_q = nullptr;
-  float* h_val = nullptr;
-  uint32_t* h_idx = nullptr;
   rmq::QueryTile* h_tiles = nullptr;
   uint32_t* h_qstart = nullptr;
-  uint32_t* d_q = nullptr;
-  float* d_val = nullptr;
-  uint32_t* d_idx = nullptr;
   rmq::QueryTile* d_tiles = nullptr;
   uint32_t* d_qstart = nullptr;
   uint32_t* d_counter = nullptr;  // device memory
-  std::vector<Request> reqs;
-  std::vector<uint64_t> offs;
 };
 
 bool dalloc(void** p, size_t bytes) {
@@ -109,19 +104,32 @@ struct rmq_server {
   rmq::HierView* h_views = nullptr;  // pinned mirrors of the tables
   uint32_t* h_variant = nullptr;
   hipStream_t admin = nullptr;
-  std::deque<Request> pending;
+  // the staging rings: pairs, value answers, index answers (ring_cap entries each)
+  uint64_t ring_cap = 0;
+  uint32_t* h_ring_q = nullptr;
+  float* h_ring_v = nullptr;
+  uint32_t* h_ring_i = nullptr;
+  uint32_t* d_ring_q = nullptr;
+  float* d_ring_v = nullptr;
+  uint32_t* d_ring_i = nullptr;
+  uint64_t head = 0;  // next free ring position (absolute)
+  // every request not yet reported done, in ticket order: the first `launched` are in batches
+  std::deque<Request> live;
+  size_t launched = 0;
   std::vector<Slot> slots;
   uint64_t next_ticket = 1;
   rmq_server_stats stats{};
+  // per-batch scratch of the packer (kept to avoid reallocating)
+  std::vector<uint64_t> seg_start, seg_cnt;
+  std::vector<uint32_t> seg_ids;
+  std::vector<rmq::QueryTile> tiles;
+  std::vector<uint32_t> qstart;
 };
 
 namespace {
 
 void free_slot(Slot& sl) {
   if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-  (void)hipHostFree(sl.h_q);
-  (void)hipHostFree(sl.h_val);
-  (void)hipHostFree(sl.h_idx);
   (void)hipHostFree(sl.h_tiles);
   (void)hipHostFree(sl.h_qstart);
   (void)hipFree(sl.d_counter);
@@ -130,71 +138,73 @@ void free_slot(Slot& sl) {
   sl = Slot();
 }
 
-bool init_slot(Slot& sl, uint64_t max_q, uint32_t max_tiles) {
+bool init_slot(Slot& sl, uint32_t max_tiles) {
   return hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
-         zalloc(&sl.h_q, &sl.d_q, max_q * 8) && zalloc(&sl.h_val, &sl.d_val, max_q * 4) &&
-         zalloc(&sl.h_idx, &sl.d_idx, max_q * 4) &&
          zalloc(&sl.h_tiles, &sl.d_tiles, max_tiles * sizeof(rmq::QueryTile)) &&
          zalloc(&sl.h_qstart, &sl.d_qstart, (rmq::kQueues + 1) * 4) &&
          dalloc(reinterpret_cast<void**>(&sl.d_counter), rmq::kCounterWords * 4);
 }
 
-// Copy a finished slot's answers to its requests' buffers and free the slot.
+// The oldest ring position still held by a request (head when none is).
+uint64_t ring_tail(const rmq_server* s) { return s->live.empty() ? s->head : s->live.front().pos; }
+
+// Copy a finished slot's answers to its requests' buffers, mark them done and free the slot; the
+// requests at the front of the queue that are done release their ring space.
 void retire(rmq_server* s, Slot& sl, uint64_t* done, uint32_t max_done, uint32_t* n_done) {
-  for (size_t i = 0; i < sl.reqs.size(); ++i) {
-    Request& r = sl.reqs[i];
-    if (r.h_val) std::memcpy(r.h_val, sl.h_val + sl.offs[i], r.q * 4);
-    if (r.h_idx) std::memcpy(r.h_idx, sl.h_idx + sl.offs[i], r.q * 4);
+  const uint64_t first = s->live.front().ticket;
+  for (uint64_t t = sl.t0; t < sl.t1; ++t) {
+    Request& r = s->live[t - first];
+    const uint64_t at = r.pos % s->ring_cap;
+    if (r.h_val) std::memcpy(r.h_val, s->h_ring_v + at, r.q * 4ull);
+    if (r.h_idx) std::memcpy(r.h_idx, s->h_ring_i + at, r.q * 4ull);
+    r.done = true;
     s->arrays[r.array].users--;
     s->stats.completed++;
     if (done && *n_done < max_done) done[(*n_done)++] = r.ticket;
   }
-  sl.reqs.clear();
-  sl.offs.clear();
   sl.busy = false;
   s->stats.in_flight--;
+  while (!s->live.empty() && s->live.front().done) {
+    s->live.pop_front();
+    s->launched--;
+  }
 }
 
-// Pack pending requests (FIFO) into a free slot and launch the batch.
+// Pack the oldest pending requests into a free slot and launch the batch.
 rmq_status launch(rmq_server* s, Slot& sl) {
+  const size_t p0 = s->launched;
   uint64_t total = 0;
-  sl.want_idx = false;
-  std::vector<uint64_t> qoff;
-  std::vector<uint32_t> ids;
-  while (!s->pending.empty() && sl.reqs.size() < s->opt.max_batch_requests &&
-         total + s->pending.front().q <= s->opt.max_batch_queries) {
-    Request r = std::move(s->pending.front());
-    s->pending.pop_front();
-    std::memcpy(sl.h_q + 2 * total, r.lr.data(), r.q * 8);
-    sl.offs.push_back(total);
-    qoff.push_back(total);
-    ids.push_back(static_cast<uint32_t>(r.array));
-    sl.want_idx = sl.want_idx || r.h_idx != nullptr;
+  bool want_idx = false, lanes_only = true;
+  s->seg_start.clear();
+  s->seg_cnt.clear();
+  s->seg_ids.clear();
+  size_t p = p0;
+  while (p < s->live.size() && p - p0 < s->opt.max_batch_requests && total + s->live[p].q <= s->opt.max_batch_queries) {
+    const Request& r = s->live[p];
+    s->seg_start.push_back(r.pos % s->ring_cap);
+    s->seg_cnt.push_back(r.q);
+    s->seg_ids.push_back(static_cast<uint32_t>(r.array));
+    want_idx = want_idx || r.h_idx != nullptr;
+    lanes_only = lanes_only && rmq::variant_is_lane(s->h_variant[r.array]);
     total += r.q;
-    r.lr.clear();
-    r.lr.shrink_to_fit();
-    sl.reqs.push_back(std::move(r));
+    ++p;
   }
-  if (sl.reqs.empty()) return RMQ_OK;
-  qoff.push_back(total);
-  std::vector<rmq::QueryTile> tiles;
-  std::vector<uint32_t> qstart;
+  if (p == p0) return RMQ_OK;
   // within kMaxTiles: at most max_batch_queries / 256 + max_batch_requests tiles
-  (void)rmq::make_query_tiles(static_cast<uint32_t>(ids.size()), qoff.data(), tiles, qstart, ids.data());
-  std::memcpy(sl.h_tiles, tiles.data(), tiles.size() * sizeof(rmq::QueryTile));
-  std::memcpy(sl.h_qstart, qstart.data(), qstart.size() * 4);
-  const hipStream_t st = sl.stream;
-  // every request of the batch on a lane-variant array: the kernel without the group solvers
-  const bool lanes_only =
-      std::all_of(ids.begin(), ids.end(), [s](uint32_t id) { return rmq::variant_is_lane(s->h_variant[id]); });
+  (void)rmq::make_segment_tiles(static_cast<uint32_t>(s->seg_ids.size()), s->seg_start.data(), s->seg_cnt.data(),
+                                s->tiles, s->qstart, s->seg_ids.data());
+  std::memcpy(sl.h_tiles, s->tiles.data(), s->tiles.size() * sizeof(rmq::QueryTile));
+  std::memcpy(sl.h_qstart, s->qstart.data(), s->qstart.size() * 4);
   rmq::launch_multi_query(s->d_views, s->d_variant, sl.d_tiles, sl.d_qstart, sl.d_counter,
-                          static_cast<uint32_t>(tiles.size()), reinterpret_cast<const uint2*>(sl.d_q), sl.d_val,
-                          sl.want_idx ? sl.d_idx : nullptr, st, lanes_only);
+                          static_cast<uint32_t>(s->tiles.size()), reinterpret_cast<const uint2*>(s->d_ring_q),
+                          s->d_ring_v, want_idx ? s->d_ring_i : nullptr, sl.stream, lanes_only);
   bool ok = hipGetLastError() == hipSuccess;
-  ok = ok && hipEventRecord(sl.done, st) == hipSuccess;
-  sl.total = total;
+  ok = ok && hipEventRecord(sl.done, sl.stream) == hipSuccess;
+  sl.t0 = s->live[p0].ticket;
+  sl.t1 = sl.t0 + (p - p0);
   sl.busy = true;
+  s->launched = p;
   s->stats.in_flight++;
   s->stats.batches++;
   s->stats.queries += total;
@@ -217,17 +227,23 @@ rmq_status rmq_server_create(const rmq_server_options* opt, rmq_server** out) {
   if (!s->opt.max_arrays) s->opt.max_arrays = 1024;
   if (!s->opt.max_batch_requests) s->opt.max_batch_requests = 4096;
   s->max_tiles = s->opt.max_batch_queries / rmq::kTileQueries + s->opt.max_batch_requests;
-  if (s->opt.max_batch_requests > rmq::kMaxTiles / 2) {
+  if (!s->opt.ring_queries)
+    s->opt.ring_queries = std::max<uint64_t>(1ull << 22, 2ull * s->opt.slots * s->opt.max_batch_queries);
+  if (s->opt.max_batch_requests > rmq::kMaxTiles / 2 || s->opt.ring_queries < s->opt.max_batch_queries ||
+      s->opt.ring_queries > (1ull << 32)) {
     delete s;
     return RMQ_EINVAL;
   }
+  s->ring_cap = s->opt.ring_queries;
   bool ok = hipStreamCreateWithFlags(&s->admin, hipStreamNonBlocking) == hipSuccess &&
             dalloc(reinterpret_cast<void**>(&s->d_views), s->opt.max_arrays * sizeof(rmq::HierView)) &&
             dalloc(reinterpret_cast<void**>(&s->d_variant), s->opt.max_arrays * 4) &&
             halloc(reinterpret_cast<void**>(&s->h_views), s->opt.max_arrays * sizeof(rmq::HierView)) &&
-            halloc(reinterpret_cast<void**>(&s->h_variant), s->opt.max_arrays * 4);
+            halloc(reinterpret_cast<void**>(&s->h_variant), s->opt.max_arrays * 4) &&
+            zalloc(&s->h_ring_q, &s->d_ring_q, s->ring_cap * 8) && zalloc(&s->h_ring_v, &s->d_ring_v, s->ring_cap * 4) &&
+            zalloc(&s->h_ring_i, &s->d_ring_i, s->ring_cap * 4);
   s->slots.resize(s->opt.slots);
-  for (Slot& sl : s->slots) ok = ok && init_slot(sl, s->opt.max_batch_queries, s->max_tiles);
+  for (Slot& sl : s->slots) ok = ok && init_slot(sl, s->max_tiles);
   if (!ok) {
     rmq_server_destroy(s);
     return RMQ_ENOMEM;
@@ -247,6 +263,9 @@ void rmq_server_destroy(rmq_server* s) {
   (void)hipFree(s->d_variant);
   (void)hipHostFree(s->h_views);
   (void)hipHostFree(s->h_variant);
+  (void)hipHostFree(s->h_ring_q);
+  (void)hipHostFree(s->h_ring_v);
+  (void)hipHostFree(s->h_ring_i);
   if (s->admin) (void)hipStreamDestroy(s->admin);
   delete s;
 }
@@ -338,28 +357,36 @@ rmq_status rmq_server_submit(rmq_server* s, int32_t array_id, const uint32_t* h_
     return RMQ_EINVAL;
   if (array_id < 0 || static_cast<size_t>(array_id) >= s->arrays.size() || !s->arrays[array_id].live)
     return RMQ_EINVAL;
+  // ring space: q consecutive entries, starting over at the ring's beginning when they would
+  // not fit before its end. An empty ring always has room (q <= max_batch_queries <= ring_cap):
+  // the space skipped at its end then holds nothing, so it must not count against the request.
+  uint64_t pos = s->head;
+  if (pos % s->ring_cap + q > s->ring_cap) pos += s->ring_cap - pos % s->ring_cap;
+  if (!s->live.empty() && pos + q - ring_tail(s) > s->ring_cap) return RMQ_EAGAIN;
   // every pair is checked while it is copied: one bad pair refuses the whole request before
   // anything is queued, so no kernel ever reads outside a resident array
   const uint64_t n = s->arrays[array_id].n;
-  Request r;
-  r.lr.resize(2 * q);
+  uint32_t* dst = s->h_ring_q + 2 * (pos % s->ring_cap);
   uint32_t bad = 0;
   for (uint64_t i = 0; i < q; ++i) {
-    const uint32_t l = h_lr[2 * i], rr = h_lr[2 * i + 1];
-    bad |= static_cast<uint32_t>(l > rr) | static_cast<uint32_t>(rr >= n);
-    r.lr[2 * i] = l;
-    r.lr[2 * i + 1] = rr;
+    const uint32_t l = h_lr[2 * i], r = h_lr[2 * i + 1];
+    bad |= static_cast<uint32_t>(l > r) | static_cast<uint32_t>(r >= n);
+    dst[2 * i] = l;
+    dst[2 * i + 1] = r;
   }
   if (bad) return RMQ_EINVAL;
+  Request r;
   r.ticket = s->next_ticket++;
   r.array = array_id;
-  r.q = q;
+  r.q = static_cast<uint32_t>(q);
+  r.pos = pos;
   r.h_val = h_val;
   r.h_idx = h_idx;
+  s->head = pos + q;
   s->arrays[array_id].users++;
   s->stats.submitted++;
   *ticket = r.ticket;
-  s->pending.push_back(std::move(r));
+  s->live.push_back(r);
   return RMQ_OK;
 }
 
@@ -367,6 +394,7 @@ rmq_status rmq_server_poll(rmq_server* s, uint64_t* done, uint32_t max_done, uin
   if (!s) return RMQ_EINVAL;
   uint32_t nd = 0;
   rmq_status st = RMQ_OK;
+  // batches finish in any order; ring space is released in ticket order (retire)
   for (Slot& sl : s->slots) {
     if (!sl.busy) continue;
     const hipError_t e = hipEventQuery(sl.done);
@@ -377,7 +405,7 @@ rmq_status rmq_server_poll(rmq_server* s, uint64_t* done, uint32_t max_done, uin
     }
   }
   for (Slot& sl : s->slots) {
-    if (s->pending.empty() || st != RMQ_OK) break;
+    if (s->launched == s->live.size() || st != RMQ_OK) break;
     if (!sl.busy) st = launch(s, sl);
   }
   if (n_done) *n_done = nd;
@@ -402,7 +430,7 @@ rmq_status rmq_server_drain(rmq_server* s, uint64_t* done, uint32_t max_done, ui
         wait = &sl;
         break;
       }
-    if (!wait && s->pending.empty()) break;
+    if (!wait && s->launched == s->live.size()) break;
     if (wait && hipEventSynchronize(wait->done) != hipSuccess) {
       if (n_done) *n_done = total;
       return RMQ_EDEVICE;

@@ -12,8 +12,10 @@
 // of host memory. The client is one thread that submits every request whose scheduled arrival
 // has passed and polls the server in between; latency = the time poll reports a request done -
 // its scheduled arrival, so a client that falls behind shows up in the latency and in
-// `client_lag`. One JSON line per rate on stdout (after a header line); a final line checks 32
-// requests answered by the server against GPU BASE (rmq_gpu_base) on the same device arrays.
+// `client_lag` (a submit refused with RMQ_EAGAIN, the staging ring being full, is retried after
+// the next poll). The client's time is split into submit, poll and the rest (the loop itself).
+// One JSON line per rate on stdout (after a header line); a final line checks 32 requests
+// answered by the server against GPU BASE (rmq_gpu_base) on the same device arrays.
 
 #include <hip/hip_runtime_api.h>
 
@@ -206,7 +208,8 @@ int main(int argc, char** argv) {
     for (uint64_t i = 0; i < N; ++i) total_q += pool[i % kPool].q;
     rmq_server_get_stats(srv, &ss);
     const uint64_t batches0 = ss.batches, first_ticket = ss.submitted + 1;
-    uint64_t next = 0, oldest = 0, alloc = 0, completed = 0;
+    uint64_t next = 0, oldest = 0, alloc = 0, completed = 0, refused = 0;
+    double t_submit = 0, t_poll = 0;
     const Clock::time_point t0 = Clock::now();
     auto now = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
     while (completed < N) {
@@ -218,7 +221,13 @@ int main(int argc, char** argv) {
         while (oldest < next && done[oldest]) ++oldest;
         if (oldest < next && s0 + r.q - start[oldest] > kRing) break;  // ring full: poll first
         uint64_t tk = 0;
+        const double ts = now();
         st = rmq_server_submit(srv, ids[r.array], pairs.data() + r.off, r.q, nullptr, ring.data() + s0 % kRing, &tk);
+        t_submit += now() - ts;
+        if (st == RMQ_EAGAIN) {  // the server's staging is full: poll first
+          ++refused;
+          break;
+        }
         if (st != RMQ_OK) die("rmq_server_submit", st);
         start[next] = s0;
         alloc = s0 + r.q;
@@ -226,7 +235,9 @@ int main(int argc, char** argv) {
         ++next;
       }
       uint32_t nd = 0;
+      const double tp = now();
       st = rmq_server_poll(srv, done_buf.data(), static_cast<uint32_t>(done_buf.size()), &nd);
+      t_poll += now() - tp;
       if (st != RMQ_OK) die("rmq_server_poll", st);
       if (nd) {
         const double tf = now();
@@ -253,10 +264,12 @@ int main(int argc, char** argv) {
     printf("{\"offered_rps\": %.0f, \"requests\": %llu, \"queries\": %llu, \"offered_gqs\": %.4f, "
            "\"gqs\": %.4f, \"p50_ms\": %.3f, \"p90_ms\": %.3f, \"p99_ms\": %.3f, \"max_ms\": %.3f, "
            "\"batches\": %llu, \"requests_per_batch\": %.1f, \"client_lag_p99_ms\": %.3f, "
+           "\"client_submit_s\": %.3f, \"client_poll_s\": %.3f, \"wall_s\": %.3f, \"refused\": %llu, "
            "\"saturated\": %s}\n",
            rate, (unsigned long long)N, (unsigned long long)total_q, offered, achieved, pct(latency, 50),
            pct(latency, 90), pct(latency, 99), pct(latency, 100), (unsigned long long)batches,
-           batches ? static_cast<double>(N) / batches : 0.0, lag99,
+           batches ? static_cast<double>(N) / batches : 0.0, lag99, t_submit, t_poll, last,
+           (unsigned long long)refused,
            (achieved < 0.9 * offered || lag99 > 1.0) ? "true" : "false");
     fflush(stdout);
   }

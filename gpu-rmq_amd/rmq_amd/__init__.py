@@ -30,6 +30,7 @@ from ._lib import (  # noqa: F401  (re-exported)
     RMQ_ENOMEM,
     RMQ_ETHRESHOLD,
     RMQ_EUNSUPPORTED,
+    RMQ_EAGAIN,
     RMQ_OK,
     RMQError,
     BuildOptions,
@@ -465,9 +466,9 @@ class Server:
     """
 
     def __init__(self, slots=0, max_batch_queries=0, reserve_bytes=0, max_arrays=0,
-                 max_batch_requests=0):
+                 max_batch_requests=0, ring_queries=0):
         o = _lib.ServerOptions(slots, max_batch_queries, reserve_bytes, max_arrays,
-                               max_batch_requests)
+                               max_batch_requests, ring_queries)
         self._s = ctypes.c_void_p()
         _lib.check(_lib.lib().rmq_server_create(ctypes.byref(o), ctypes.byref(self._s)),
                    "rmq_server_create")

@@ -18,6 +18,7 @@ RMQ_ETHRESHOLD = -2
 RMQ_ENOMEM = -3
 RMQ_EDEVICE = -4
 RMQ_EUNSUPPORTED = -5
+RMQ_EAGAIN = -6
 
 RMQ_ALG_GPU_BASE = 2
 RMQ_ALG_XXX = 16
@@ -58,6 +59,7 @@ class ServerOptions(ctypes.Structure):
         ("reserve_bytes", ctypes.c_uint64),
         ("max_arrays", ctypes.c_uint32),
         ("max_batch_requests", ctypes.c_uint32),
+        ("ring_queries", ctypes.c_uint64),
     ]
 
 

@@ -44,7 +44,8 @@ typedef enum {
   RMQ_ETHRESHOLD = -2,    /* t < 2R (algs 16/24) or t < R (alg 20): src/xxx.h:211-217,938-944 */
   RMQ_ENOMEM = -3,        /* device allocation failed */
   RMQ_EDEVICE = -4,       /* HIP runtime error (launch or API) */
-  RMQ_EUNSUPPORTED = -5   /* option combination not compiled in */
+  RMQ_EUNSUPPORTED = -5,  /* option combination not compiled in */
+  RMQ_EAGAIN = -6         /* a server's staging ring is full: poll (or drain), then submit again */
 } rmq_status;
 
 /* Algorithm ids, the reference's own numbering (src/main.cu:36-53). */

@@ -5,10 +5,15 @@
  * A server keeps built hierarchies resident in HBM, admits new arrays only while they fit in
  * free device memory (minus a reserve), and answers requests by continuous batching: pending
  * requests (each: queries on one resident array) are packed into the next free "slot" — a
- * stream with its own staging buffers — and answered by one persistent multi-array launch
+ * stream with its own tile buffers — and answered by one persistent multi-array launch
  * (the batched form's kernel, rmq_batch.h); up to `slots` batches are in flight at once, so
  * requests that arrive while others are on the GPU overlap with them. Answers are identical to
  * rmq_query on the same hierarchy, whatever else is in the batch.
+ *
+ * Staging: submit copies (and checks) a request's pairs once, into a ring of pinned host memory
+ * that the kernels read in place; they write the answers into a ring beside it, from which poll
+ * copies them to the caller's buffers. A request holds its ring space until it is reported done;
+ * when the ring has no room, submit returns RMQ_EAGAIN.
  *
  * The server is driven by the caller (rmq_server_poll / rmq_server_drain) from one thread; it
  * starts no threads of its own.
@@ -32,6 +37,9 @@ typedef struct {
   uint64_t reserve_bytes;      /* free HBM admission keeps untouched; 0 -> 2 GiB */
   uint32_t max_arrays;         /* resident arrays; 0 -> 1024 */
   uint32_t max_batch_requests; /* requests one batch holds; 0 -> 4096 */
+  uint64_t ring_queries;       /* staging ring, in queries (12 bytes of pinned host memory each:
+                                  the pair and the answer); 0 -> max(2^22, 2 x slots x
+                                  max_batch_queries); at least max_batch_queries */
 } rmq_server_options;
 
 enum {
@@ -63,12 +71,13 @@ rmq_status rmq_server_admit(rmq_server* s, int source, const float* data, uint64
 /* Drop a resident array. RMQ_EINVAL if it is unknown or a pending/in-flight request uses it. */
 rmq_status rmq_server_evict(rmq_server* s, int32_t array_id);
 
-/* Queue q queries (host {l, r} pairs, copied now) on a resident array. Answers are written to
- * the caller's host buffers h_val / h_idx (either may be NULL, not both) before the ticket is
- * reported done. q must be 1..max_batch_queries. Every pair must satisfy 0 <= l <= r < n of the
- * array: the pairs are checked while they are copied, and a request holding any other pair is
- * refused with RMQ_EINVAL, with nothing queued (the batch kernel is shared by all requests in
- * flight, so a bad pair is never handed to it). */
+/* Queue q queries (host {l, r} pairs, copied now into the staging ring) on a resident array.
+ * Answers are written to the caller's host buffers h_val / h_idx (either may be NULL, not both)
+ * before the ticket is reported done. q must be 1..max_batch_queries. Every pair must satisfy
+ * 0 <= l <= r < n of the array: the pairs are checked while they are copied, and a request
+ * holding any other pair is refused with RMQ_EINVAL, with nothing queued (the batch kernel is
+ * shared by all requests in flight, so a bad pair is never handed to it). RMQ_EAGAIN, with
+ * nothing queued, when the ring has no room for q pairs until earlier requests are done. */
 rmq_status rmq_server_submit(rmq_server* s, int32_t array_id, const uint32_t* h_lr, uint64_t q,
                              float* h_val, uint32_t* h_idx, uint64_t* ticket);
 

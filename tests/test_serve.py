@@ -238,3 +238,57 @@ def test_config5_full_workload(device):
             assert np.array_equal(oracle.cpu_base_index(A, lr[short]), ref[short]), a
     del dev
     torch.cuda.empty_cache()
+
+
+@pytest.mark.gpu
+def test_staging_ring_full_and_wrapping(device):
+    """The staging ring: a request that does not fit is refused with RMQ_EAGAIN and nothing is
+    queued; once earlier requests are done it is accepted. Then 400 requests of 1..4096 queries
+    go around a ring of 5000 queries many times (starting over at its beginning whenever a
+    request would not fit before its end), with polls in between and retries after RMQ_EAGAIN;
+    every answer equals the oracle's."""
+    rng = np.random.default_rng(12)
+    arrays = _host_arrays(rng, 4, 2000, 200000)
+    with rmq_amd.Server(slots=2, max_batch_queries=4096, ring_queries=5000) as srv:
+        ids = [srv.admit_host(A, t=64, variant=("lane", "lane8", "lane4", "group8")[i]) for i, A in enumerate(arrays)]
+        lr = workloads.make_queries(rng, arrays[0].size, 3000, "medium")
+        t1, v1, _ = srv.submit(ids[0], lr, True, False)
+        with pytest.raises(rmq_amd.RMQError) as e:
+            srv.submit(ids[0], lr, True, False)
+        assert e.value.status == rmq_amd.RMQ_EAGAIN
+        assert srv.stats()["submitted"] == 1
+        assert srv.drain() == [t1]
+        t2, v2, _ = srv.submit(ids[0], lr, True, False)
+        assert srv.drain() == [t2]
+        ref = oracle.cpu_base_index(arrays[0], lr)
+        assert np.array_equal(v1.view(np.uint32), arrays[0][ref].view(np.uint32))
+        assert np.array_equal(v2.view(np.uint32), arrays[0][ref].view(np.uint32))
+        reqs, out = [], []
+        refused = 0
+        for k in range(400):
+            a = int(rng.integers(0, 4))
+            q = int(np.exp2(rng.uniform(0, 12)))
+            lr = workloads.make_queries(rng, arrays[a].size, q, ("small", "medium", "large")[k % 3])
+            for attempt in range(1000):  # bounded: every request fits once earlier ones are done
+                try:
+                    out.append(srv.submit(ids[a], lr, True, k % 2 == 0))
+                    break
+                except rmq_amd.RMQError as err:
+                    assert err.status == rmq_amd.RMQ_EAGAIN
+                    refused += 1
+                    if attempt % 2:
+                        srv.drain()
+                    else:
+                        srv.poll()
+            else:
+                raise AssertionError(f"request {k} of {q} queries never fitted the ring")
+            reqs.append((a, lr))
+            if k % 3 == 0:
+                srv.poll()
+        srv.drain()
+        assert refused > 0 and srv.stats()["completed"] == 402
+        for (a, lr), (_, v, x) in zip(reqs, out):
+            ref = oracle.cpu_base_index(arrays[a], lr)
+            assert np.array_equal(v.view(np.uint32), arrays[a][ref].view(np.uint32))
+            if x is not None:
+                assert np.array_equal(x, ref)
