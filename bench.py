@@ -139,7 +139,7 @@ def parse():
                          "graph, no other leg): the form rocprofv3 counter runs use")
     ap.add_argument("--serve-only", action="store_true", help="only the serving curve")
     ap.add_argument("--batch-arrays", type=int, default=4096)
-    ap.add_argument("--serve-rates", default="20000,50000,100000,200000,400000",
+    ap.add_argument("--serve-rates", default="20000,50000,100000,200000,300000,400000",
                     help="offered requests/s of the serving curve (comma list)")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival span of each point")
     ap.add_argument("--copy-probe", action="store_true",
