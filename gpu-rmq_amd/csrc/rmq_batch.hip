@@ -46,6 +46,8 @@ struct rmq_batch {
   uint8_t* lr;      // offset entries of every array's levels 1..
   uint32_t* zeros;  // per array: kZeroPos | kZeroNeg | kHasNaN of its level 0
   float4* tails;    // per array: the copy of level 0's last 16-byte vector, +inf padded
+  uint2* tt = nullptr;  // sparse tables over the arrays' top levels (those of 64..kTopSmall entries)
+  uint64_t tt_entries = 0;
   uint64_t level_floats;
   uint64_t rel_bytes;
   std::vector<uint32_t> variant;  // host copy
@@ -118,6 +120,19 @@ __global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restri
     }
   }
   if (k == 0) dev::publish_zeros(const_cast<uint32_t*>(H.zeros), zb);
+}
+
+// The sparse table over the top level of array list[blockIdx.x] (64 <= m <= kTopSmall entries),
+// built in LDS by one workgroup, as the plain build's k_top_table_small does.
+__global__ __launch_bounds__(kTopThreads) void k_top_table_batch(const HierView* __restrict__ views,
+                                                                 const uint32_t* __restrict__ list) {
+  __shared__ uint2 cur[kTopSmall];
+  const HierView& H = views[list[blockIdx.x]];
+  const uint32_t top = H.num_levels - 1;
+  const uint32_t m = H.size[top];
+  uint32_t rows = 0;
+  while ((2u << rows) <= m) ++rows;  // floor(log2 m)
+  top_table_block(top == 0 ? H.A : H.lv + H.off[top], m, const_cast<uint2*>(H.tt), H.tstride, rows, cur);
 }
 
 // Every array's level-0 word (cleared for the level-0 pass, or every bit when there is none)
@@ -318,9 +333,26 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     max_levels = std::max(max_levels, nl);
   }
 
+  // sparse tables over top levels of 64..kTopSmall entries: floor(log2 m) rows of a stride of m
+  // rounded up to 64 (the lanes-only query kernel reads them; the others ignore them)
+  std::vector<uint64_t> tt_off(count, ~0ull);
+  std::vector<uint32_t> tt_list;
+  uint64_t tt_entries = 0;
+  for (uint32_t a = 0; a < count; ++a) {
+    const uint32_t m = views[a].size[views[a].num_levels - 1];
+    if (m < 64 || m > rmq::kTopSmall) continue;
+    uint32_t rows = 0;
+    while ((2ull << rows) <= m) ++rows;
+    views[a].tstride = (m + 63) & ~63u;
+    tt_off[a] = tt_entries;
+    tt_entries += static_cast<uint64_t>(rows) * views[a].tstride;
+    tt_list.push_back(a);
+  }
+
   rmq_batch* b = new (std::nothrow) rmq_batch();
   if (!b) return RMQ_ENOMEM;
   b->count = count;
+  b->tt_entries = tt_entries;
   b->level_floats = arena;
   b->rel_bytes = rel_arena;
   (void)hipGetDevice(&b->device);
@@ -330,6 +362,7 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
   if (st == RMQ_OK) st = dmalloc(&b->tails, count);
   if (st == RMQ_OK) st = dmalloc(&b->d_views, count);
   if (st == RMQ_OK) st = dmalloc(&b->d_variant, count);
+  if (st == RMQ_OK && tt_entries) st = dmalloc(&b->tt, tt_entries);
   if (st != RMQ_OK) {
     rmq_batch_free(b);
     return st;
@@ -342,6 +375,8 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     H.lr = b->lr + lr_off[a];
     H.zeros = b->zeros + a;
     H.tail = reinterpret_cast<const float*>(b->tails + a);
+    H.tt = tt_off[a] == ~0ull ? nullptr : b->tt + tt_off[a];
+    if (!H.tt) H.tstride = 0;
     b->variant[a] = arrays[a].variant;
   }
   b->lanes_only = std::all_of(b->variant.begin(), b->variant.end(), [](uint32_t v) { return rmq::variant_is_lane(v); });
@@ -363,9 +398,18 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
   }
   start.push_back(static_cast<uint32_t>(tiles.size()));
   BuildTile* d_tiles = nullptr;
+  uint32_t* d_tt_list = nullptr;
+  if (!tt_list.empty()) {
+    st = dmalloc(&d_tt_list, tt_list.size());
+    if (st != RMQ_OK) {
+      rmq_batch_free(b);
+      return st;
+    }
+  }
   if (!tiles.empty()) {
     st = dmalloc(&d_tiles, tiles.size());
     if (st != RMQ_OK) {
+      if (d_tt_list) (void)hipFree(d_tt_list);
       rmq_batch_free(b);
       return st;
     }
@@ -374,6 +418,8 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
             hipMemcpyAsync(b->d_variant, b->variant.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice, s) == hipSuccess;
   if (ok && !tiles.empty())
     ok = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(BuildTile), hipMemcpyHostToDevice, s) == hipSuccess;
+  if (ok && !tt_list.empty())
+    ok = hipMemcpyAsync(d_tt_list, tt_list.data(), tt_list.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess;
   if (ok) {
     rmq::k_init_zeros<<<(count + 255) / 256, 256, 0, s>>>(b->d_views, count);
     ok = hipGetLastError() == hipSuccess;
@@ -388,9 +434,14 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     }
     ok = hipGetLastError() == hipSuccess;
   }
+  if (ok && !tt_list.empty()) {  // after every level: the tables read the top levels
+    rmq::k_top_table_batch<<<static_cast<uint32_t>(tt_list.size()), rmq::kTopThreads, 0, s>>>(b->d_views, d_tt_list);
+    ok = hipGetLastError() == hipSuccess;
+  }
   // the host vectors die here: wait for the copies (and the build) before returning
   ok = ok && hipStreamSynchronize(s) == hipSuccess;
   if (d_tiles) (void)hipFree(d_tiles);
+  if (d_tt_list) (void)hipFree(d_tt_list);
   if (!ok) {
     rmq_batch_free(b);
     return RMQ_EDEVICE;
@@ -523,9 +574,12 @@ void rmq_batch_free(rmq_batch* b) {
   if (b->lr) (void)hipFree(b->lr);
   if (b->d_views) (void)hipFree(b->d_views);
   if (b->d_variant) (void)hipFree(b->d_variant);
+  if (b->tt) (void)hipFree(b->tt);
   delete b;
 }
 
-uint64_t rmq_batch_level_bytes(const rmq_batch* b) { return b ? b->level_floats * 4 + b->rel_bytes : 0; }
+uint64_t rmq_batch_level_bytes(const rmq_batch* b) {
+  return b ? b->level_floats * 4 + b->rel_bytes + b->tt_entries * sizeof(uint2) : 0;
+}
 
 }  // extern "C"

@@ -556,7 +556,9 @@ struct VlState {
   bool la, ra;    // a tail's next piece can still win
 };
 
-template <int RL, bool IDX, bool TOP, class LT>
+// TOP: long final ranges on the top level come from its sparse table when the hierarchy has
+// one; EARLY (R <= 8): a short final range's vectors are all loaded before any is folded.
+template <int RL, bool IDX, bool TOP, class LT, bool EARLY = TOP>
 __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
   const uint32_t top = H.num_levels - 1;
   uint32_t K = min((31u - static_cast<uint32_t>(__clz(e - l))) / RL, top);
@@ -593,7 +595,7 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
     // (Not in kernels without the table path: the batched kernel has no registers to spare.)
     constexpr uint32_t kScanMax = 2 * (1u << RL) - 2 > dev::kTopScan ? 2 * (1u << RL) - 2 : dev::kTopScan;
     constexpr uint32_t NV = (kScanMax + 3 + 3) / 4;
-    if (TOP && RL <= 3 && ek - lk <= kScanMax) {
+    if (EARLY && RL <= 3 && ek - lk <= kScanMax) {
       const float ga = left_tail && (lk & 3u) == 0 ? vK[lk - 1u] : RMQ_INF;
       const float gb = right_tail && (ek & 3u) == 0 ? vK[ek] : RMQ_INF;
       float4 xv[NV];
@@ -758,9 +760,9 @@ __device__ __forceinline__ GroupAnswer vl_finish(const VlState& st, const HierVi
   else return GroupAnswer{st.bv, dev::kNoIdx};
 }
 
-template <int RL, bool IDX, bool TOP = true, class LT>
+template <int RL, bool IDX, bool TOP = true, class LT, bool EARLY = TOP>
 __device__ __forceinline__ GroupAnswer solve_vl_pruned(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
-  VlState st = vl_start<RL, IDX, TOP>(l, e, H, T);
+  VlState st = vl_start<RL, IDX, TOP, LT, EARLY>(l, e, H, T);
   while (st.la || st.ra) vl_step<RL, IDX>(st, H, T);
   return vl_finish<RL, IDX>(st, H, T);
 }

@@ -129,7 +129,9 @@ __device__ __forceinline__ void unit_groups(const HierView& H, const LT& T, cons
   }
 }
 
-template <int RL, bool IDX, class LT>
+// TOP: long final ranges on the top level come from the array's sparse table when it has one
+// (the lanes-only kernel, which has the registers for it).
+template <int RL, bool IDX, bool TOP, class LT>
 __device__ __forceinline__ void unit_lanes(const HierView& H, const LT& T, const uint2* __restrict__ Q,
                                            uint64_t q0, uint32_t count, float* __restrict__ ov,
                                            uint32_t* __restrict__ oi) {
@@ -137,7 +139,7 @@ __device__ __forceinline__ void unit_lanes(const HierView& H, const LT& T, const
   if (local >= count) return;
   const uint2 p = dev::load_query(Q, q0 + local);
   if (emit_nan_at_l<IDX>(*H.zeros, H.A, p.x, q0 + local, ov, oi)) return;
-  GroupAnswer a = solve_vl_pruned<RL, IDX, false>(p.x, p.y + 1, H, T);
+  GroupAnswer a = solve_vl_pruned<RL, IDX, TOP, LT, false>(p.x, p.y + 1, H, T);
   if constexpr (!IDX) {
     if (a.v == 0.0f && !dev::zero_known(*H.zeros, a.v)) a = solve_vl_pruned<RL, true, false>(p.x, p.y + 1, H, T);
   }
@@ -166,9 +168,9 @@ __device__ __forceinline__ void run_unit(uint32_t* meta, const HierView* __restr
   const LevelsLds L{meta};
   if constexpr (LANES) {  // every array of the launch runs a lane variant
     const uint32_t v = variant[a];
-    if (v == RMQ_VARIANT_LANE4) unit_lanes<kLane4RLog, IDX>(H, L, Q, q0, cnt, ov, oi);
-    else if (v == RMQ_VARIANT_LANE8) unit_lanes<kLane8RLog, IDX>(H, L, Q, q0, cnt, ov, oi);
-    else unit_lanes<kMultiRLog, IDX>(H, L, Q, q0, cnt, ov, oi);
+    if (v == RMQ_VARIANT_LANE4) unit_lanes<kLane4RLog, IDX, true>(H, L, Q, q0, cnt, ov, oi);
+    else if (v == RMQ_VARIANT_LANE8) unit_lanes<kLane8RLog, IDX, true>(H, L, Q, q0, cnt, ov, oi);
+    else unit_lanes<kMultiRLog, IDX, true>(H, L, Q, q0, cnt, ov, oi);
     return;
   }
   switch (variant[a]) {
@@ -176,9 +178,9 @@ __device__ __forceinline__ void run_unit(uint32_t* meta, const HierView* __restr
     case RMQ_VARIANT_GROUP16: unit_groups<16, IDX>(H, L, Q, q0, cnt, ov, oi); break;
     case RMQ_VARIANT_GROUP32: unit_groups<32, IDX>(H, L, Q, q0, cnt, ov, oi); break;
     case RMQ_VARIANT_WAVE: unit_groups<64, IDX>(H, L, Q, q0, cnt, ov, oi); break;
-    case RMQ_VARIANT_LANE: unit_lanes<kMultiRLog, IDX>(H, L, Q, q0, cnt, ov, oi); break;
-    case RMQ_VARIANT_LANE8: unit_lanes<kLane8RLog, IDX>(H, L, Q, q0, cnt, ov, oi); break;
-    case RMQ_VARIANT_LANE4: unit_lanes<kLane4RLog, IDX>(H, L, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_LANE: unit_lanes<kMultiRLog, IDX, false>(H, L, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_LANE8: unit_lanes<kLane8RLog, IDX, false>(H, L, Q, q0, cnt, ov, oi); break;
+    case RMQ_VARIANT_LANE4: unit_lanes<kLane4RLog, IDX, false>(H, L, Q, q0, cnt, ov, oi); break;
     default: unit_groups<8, IDX>(H, L, Q, q0, cnt, ov, oi); break;
   }
 }
@@ -201,9 +203,9 @@ __device__ __forceinline__ void queue_shape(const uint32_t* __restrict__ qstart,
 // Eight wavefronts per SIMD (64 VGPRs): with every variant's solver (LANES = false) the
 // occupancy is worth more than the few registers the compiler then spills (measured); LANES =
 // true, for launches whose arrays all run lane variants, leaves the group solvers out and fits
-// in 48 registers without spills. For the same reason its solvers scan the top level instead of
-// reading a sparse table over it: the batched build makes none, and the autotuned thresholds
-// keep top-level scans short.
+// in 60 registers without spills, with the sparse table over each array's top level (the
+// batched build makes one for top levels of 64..8192 entries). The kernel with every variant
+// scans the top level instead: the table path would spill there.
 template <bool IDX, bool LANES>
 __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
     const HierView* __restrict__ views, const uint32_t* __restrict__ variant,
