@@ -50,6 +50,17 @@ __global__ void k_init_words(uint32_t* __restrict__ w, uint32_t zeros, const flo
   dev::fill_tail(A, n, reinterpret_cast<float*>(w + 4));
 }
 
+// Level read-back (rmq_read_levels): entry j of level k >= 1 to out_v[off[k] + j], its leftmost
+// level-0 index (j << R_log k plus the offset entry) to out_i[off[k] + j].
+__global__ __launch_bounds__(kBlock) void k_read_level(HierView H, uint32_t k, uint32_t span_log,
+                                                       float* __restrict__ out_v, uint32_t* __restrict__ out_i) {
+  for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < H.size[k]; j += gridDim.x * kBlock) {
+    const uint32_t o = H.off[k] + j;
+    if (out_v) out_v[o] = H.lv[o];
+    if (out_i) out_i[o] = (span_log >= 32u ? 0u : j << span_log) + dev::rel_load(H.lr, H.roff[k], j, span_log);
+  }
+}
+
 // Query validation (RMQ_BUILD_CHECK_QUERIES): *bad becomes nonzero when a query is not
 // 0 <= l <= r < n. k_set_word clears it first (a kernel, so that the pair can be captured).
 __global__ void k_set_word(uint32_t* __restrict__ w, uint32_t v) { *w = v; }
@@ -687,6 +698,12 @@ rmq_status rmq_get_metadata(const rmq_hierarchy* h, uint32_t* num_levels, uint64
   return RMQ_OK;
 }
 
+rmq_status rmq_read_levels(const rmq_hierarchy* h, float* d_val, uint32_t* d_idx, void* stream) {
+  if (!h || (!d_val && !d_idx)) return RMQ_EINVAL;
+  if (h->chunk_log) return RMQ_EUNSUPPORTED;  // the 64-bit form keeps its levels per chunk
+  return rmq_internal_read_levels(h->view, h->R_log, d_val, d_idx, stream);
+}
+
 rmq_status rmq_build_each(const float* const* d_arrays, const uint64_t* n, const uint64_t* t,
                           int alg, const rmq_build_options* opt, uint32_t count, void* stream,
                           rmq_hierarchy** out) {
@@ -792,6 +809,17 @@ const char* rmq_version(void) { return "rmq-mi355x 0.1 gfx950"; }
 const rmq::HierView* rmq_internal_view(const rmq_hierarchy* h) { return h ? &h->view : nullptr; }
 
 uint64_t rmq_internal_level_bytes(const rmq_hierarchy* h) { return h ? h->mem_bytes : 0; }
+
+rmq_status rmq_internal_read_levels(const HierView& H, uint32_t R_log, float* d_val, uint32_t* d_idx,
+                                    void* stream) {
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  for (uint32_t k = 1; k < H.num_levels; ++k) {
+    rmq::k_read_level<<<grid_for(H.size[k]), kBlock, 0, s>>>(H, k, R_log * k, d_val, d_idx);
+    if (last_error() != RMQ_OK) return RMQ_EDEVICE;
+  }
+  return RMQ_OK;
+}
+
 uint64_t rmq_internal_rel_layout(uint32_t R_log, uint32_t num_levels, const uint64_t* size, uint32_t* roff) {
   return rel_layout(R_log, num_levels, size, roff);
 }

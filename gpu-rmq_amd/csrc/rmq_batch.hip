@@ -51,7 +51,8 @@ struct rmq_batch {
   uint64_t level_floats;
   uint64_t rel_bytes;
   std::vector<uint32_t> variant;  // host copy
-  bool lanes_only = false;        // every array runs RMQ_VARIANT_LANE or RMQ_VARIANT_LANE8
+  std::vector<HierView> views;    // host copy of d_views
+  bool lanes_only = false;        // every array runs a lane variant
 };
 
 struct rmq_batch_plan {
@@ -379,6 +380,7 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
     if (!H.tt) H.tstride = 0;
     b->variant[a] = arrays[a].variant;
   }
+  b->views = views;
   b->lanes_only = std::all_of(b->variant.begin(), b->variant.end(), [](uint32_t v) { return rmq::variant_is_lane(v); });
 
   // build tiles, grouped by source level and reduction factor (R = 32, then R = 8, then R = 4)
@@ -580,6 +582,12 @@ void rmq_batch_free(rmq_batch* b) {
 
 uint64_t rmq_batch_level_bytes(const rmq_batch* b) {
   return b ? b->level_floats * 4 + b->rel_bytes + b->tt_entries * sizeof(uint2) : 0;
+}
+
+rmq_status rmq_batch_read_levels(const rmq_batch* b, uint32_t array, float* d_val, uint32_t* d_idx,
+                                 void* stream) {
+  if (!b || array >= b->count || (!d_val && !d_idx)) return RMQ_EINVAL;
+  return rmq_internal_read_levels(b->views[array], rmq::variant_rlog(b->variant[array]), d_val, d_idx, stream);
 }
 
 }  // extern "C"

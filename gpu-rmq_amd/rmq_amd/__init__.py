@@ -227,6 +227,21 @@ class Hierarchy:
         k = nl.value
         return Plan(k, list(sizes[:k]), list(offs[:k]), bf.value, R.value)
 
+    def levels(self, stream=None):
+        """Levels 1.. read back (rmq_read_levels): (values float32[buffer_floats], leftmost
+        level-0 indices int32[buffer_floats]), entry j of level k at metadata().offsets[k] + j.
+        Alignment gaps between levels hold zeros here."""
+        torch = _torch()
+        if self._h.value is None:
+            raise RuntimeError("hierarchy already freed")
+        bf = self.metadata().buffer_floats
+        val = torch.zeros(max(bf, 1), dtype=torch.float32, device=self.array.device)
+        idx = torch.zeros(max(bf, 1), dtype=torch.int32, device=self.array.device)
+        if bf:
+            _lib.check(_lib.lib().rmq_read_levels(self._h, val.data_ptr(), idx.data_ptr(),
+                                                  _stream_handle(stream)), "rmq_read_levels")
+        return val[:bf], idx[:bf]
+
     def close(self) -> None:
         if self._h.value is not None:
             _lib.lib().rmq_free(self._h)
@@ -378,6 +393,18 @@ class Batch:
 
     def level_bytes(self) -> int:
         return int(_lib.lib().rmq_batch_level_bytes(self._b))
+
+    def levels(self, array: int, buffer_floats: int, stream=None):
+        """Array `array`'s levels read back (rmq_batch_read_levels) into new tensors of
+        buffer_floats entries (its rmq_plan buffer length): (values float32, indices int32)."""
+        torch = _torch()
+        dev = self.arrays.device
+        val = torch.zeros(max(buffer_floats, 1), dtype=torch.float32, device=dev)
+        idx = torch.zeros(max(buffer_floats, 1), dtype=torch.int32, device=dev)
+        if buffer_floats:
+            _lib.check(_lib.lib().rmq_batch_read_levels(self._b, array, val.data_ptr(), idx.data_ptr(),
+                                                        _stream_handle(stream)), "rmq_batch_read_levels")
+        return val[:buffer_floats], idx[:buffer_floats]
 
     def close(self) -> None:
         for p in self._plans:

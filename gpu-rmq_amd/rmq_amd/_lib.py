@@ -121,6 +121,7 @@ SIGNATURES = {
         ctypes.c_int,
         [_u64, _u64, ctypes.c_int, ctypes.POINTER(BuildOptions), _pu32, _pu64, _pu64, _pu64, _pu32],
     ),
+    "rmq_read_levels": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "rmq_gpu_base": (ctypes.c_int, [_vp, _u64, _vp, _u64, _vp, _vp, _vp]),
     "rmq_build_each": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(BuildOptions),
                                       _u32, _vp, _vp]),
@@ -137,6 +138,7 @@ SIGNATURES = {
     "rmq_batch_plan_free": (None, [_vp]),
     "rmq_batch_free": (None, [_vp]),
     "rmq_batch_level_bytes": (_u64, [_vp]),
+    "rmq_batch_read_levels": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp]),
     "rmq_fill_uniform": (ctypes.c_int, [_vp, _u64, _u64, _u64, _vp]),
     "rmq_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOptions), ctypes.POINTER(_vp)]),
     "rmq_server_destroy": (None, [_vp]),

@@ -146,6 +146,14 @@ void rmq_free(rmq_hierarchy* h);
 rmq_status rmq_get_metadata(const rmq_hierarchy* h, uint32_t* num_levels, uint64_t* sizes,
                             uint64_t* offsets, uint64_t* buffer_floats, uint32_t* R);
 
+/* Read back the levels of a built hierarchy into caller device buffers of buffer_floats entries,
+ * laid out as the reference's level buffer (d_level_buffer, src/xxx.h:244-249): entry j of level
+ * k >= 1 at offsets[k] + j. d_val receives the block minima (xxx_build_step, src/xxx.h:91-120);
+ * d_idx the leftmost level-0 index of each minimum (no reference counterpart: the reference keeps
+ * values only). Either may be NULL, not both. Entries in the alignment gaps between levels are
+ * left as they were. Asynchronous on `stream`; RMQ_EUNSUPPORTED for the 64-bit (chunked) form. */
+rmq_status rmq_read_levels(const rmq_hierarchy* h, float* d_val, uint32_t* d_idx, void* stream);
+
 /* Same metadata without building anything (host only, no device use). */
 rmq_status rmq_plan(uint64_t n, uint64_t t, int alg, const rmq_build_options* opt,
                     uint32_t* num_levels, uint64_t* sizes, uint64_t* offsets,

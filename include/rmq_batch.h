@@ -102,6 +102,13 @@ void rmq_batch_free(rmq_batch* b);
 /* Bytes of device memory the batch's level buffers occupy. */
 uint64_t rmq_batch_level_bytes(const rmq_batch* b);
 
+/* Read back array `array`'s levels as rmq_read_levels does for a plain hierarchy: entry j of level
+ * k >= 1 at offsets[k] + j of that array's own metadata (rmq_plan with its variant's alg and
+ * options), into device buffers of its buffer_floats entries. Reference counterpart: the level
+ * buffer xxx_build_step writes (src/xxx.h:91-120, 244-271), here one array's part of the batch. */
+rmq_status rmq_batch_read_levels(const rmq_batch* b, uint32_t array, float* d_val, uint32_t* d_idx,
+                                 void* stream);
+
 /* Fill d_out[0..n) with uniform floats in [0, 1): SplitMix64 of seed + (start + i) * golden
  * gamma, top 24 bits scaled by 2^-24 (exact in float32). Used for the synthetic batched and
  * serving workloads. */

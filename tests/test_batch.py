@@ -228,6 +228,15 @@ def test_config4_full_workload(device):
             torch.cuda.synchronize()
             assert float(junk) > 0
         del gi
+        # the batched build's levels of every 64th array against the oracle's level build
+        rlog = {"lane": 5, "lane8": 3, "lane4": 2}
+        for a in range(0, 4096, 64):
+            o, n = int(w.offsets[a]), int(w.sizes[a])
+            ref = oracle.CpuHierarchy(dA[o:o + n].cpu().numpy(), ts[a], rlog.get(var[a], 5))
+            lv, li = b.levels(a, ref.floats)
+            torch.cuda.synchronize()
+            assert np.array_equal(lv.cpu().numpy().view(np.uint32), ref.lv[:ref.floats].view(np.uint32)), a
+            assert np.array_equal(li.cpu().numpy().view(np.uint32), ref.li[:ref.floats]), a
     sys.stderr.write("[gpu-case] config 4: per-array plain launches\n")
     with rmq_amd.PlainMany(dA, w.offsets, w.sizes, ts) as pm:
         out_p = torch.empty(Q, dtype=torch.int32, device="cuda")
