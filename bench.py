@@ -216,11 +216,17 @@ def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True):
     log(f"batched workload: {arrays} arrays, {w.total_elements / 1e9:.2f} G elements, {Q} queries "
         f"({time.perf_counter() - t0:.1f} s to generate)")
 
-    def timed(fn, reps=3):
+    def timed(fn, reps=3, pre=None):
+        """Best wall time of fn() (device synchronised); pre() runs before each call, untimed."""
+        if pre:
+            pre()
         fn()
         torch.cuda.synchronize()
         best = 1e30
         for _ in range(reps):
+            if pre:
+                pre()
+                torch.cuda.synchronize()
             t = time.perf_counter()
             fn()
             torch.cuda.synchronize()
@@ -230,12 +236,14 @@ def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True):
     out_b = torch.empty(Q, dtype=torch.int32, device="cuda")
     state = {}
 
-    def batched_build():
+    def batched_free():  # the previous batch is freed outside the timed build
         if "b" in state:
-            state["b"].close()
+            state.pop("b").close()
+
+    def batched_build():
         state["b"] = rmq_amd.Batch(dA, w.offsets, w.sizes, ts, var)
 
-    build_b = timed(batched_build)
+    build_b = timed(batched_build, pre=batched_free)
     plan = state["b"].plan(w.qoffsets)
     query_b = timed(lambda: plan.query(dQ, False, True, None, out_b))
     res = {
@@ -259,12 +267,14 @@ def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True):
         out_p = torch.empty(Q, dtype=torch.int32, device="cuda")
         pstate = {}
 
-        def plain_build():
+        def plain_free():
             if "p" in pstate:
-                pstate["p"].close()
+                pstate.pop("p").close()
+
+        def plain_build():
             pstate["p"] = rmq_amd.PlainMany(dA, w.offsets, w.sizes, ts)
 
-        build_p = timed(plain_build, reps=1)
+        build_p = timed(plain_build, reps=1, pre=plain_free)
         query_p = timed(lambda: pstate["p"].query(dQ, w.qoffsets, out_p))
         if graph_too:
             graph.launch()

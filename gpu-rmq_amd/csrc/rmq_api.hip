@@ -231,34 +231,30 @@ uint32_t grid_for(uint64_t threads) {
 rmq_status last_error() { return hipGetLastError() == hipSuccess ? RMQ_OK : RMQ_EDEVICE; }
 
 // ---- build dispatch -----------------------------------------------------------------------
+// Levels 1.. of H (sizes, offsets and pointers set; level 0's word cleared), several levels per
+// pass (rmq::k_build_pass): a pass from level k makes levels k+1 .. k+M.
 template <int RL>
-void launch_build_level(const float* sv, const uint8_t* srel, uint32_t ssk, uint32_t ssize, float* dv,
-                        uint8_t* drel, uint32_t dsk, uint32_t dsize, uint32_t* zeros, hipStream_t s) {
-  constexpr uint32_t V = (1u << RL) / 4;
-  constexpr uint32_t W = V < 64 ? V : 64;
-  const uint32_t grid = grid_for(static_cast<uint64_t>(dsize) * W);
-  if (srel) {
-    rmq::k_build_level<RL, true><<<grid, kBlock, 0, s>>>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros);
-  } else {
-    rmq::k_build_level<RL, false><<<grid, kBlock, 0, s>>>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros);
+rmq_status launch_passes(const HierView& H, hipStream_t s) {
+  using P = rmq::PassShape<RL>;
+  for (uint32_t k = 0; k + 1 < H.num_levels; k += P::M) {
+    const uint32_t m = std::min<uint32_t>(P::M, H.num_levels - 1 - k);
+    const uint32_t tiles = static_cast<uint32_t>((static_cast<uint64_t>(H.size[k]) + (1u << P::SLOG) - 1) >> P::SLOG);
+    rmq::k_build_pass<RL><<<tiles, kBlock, 0, s>>>(H, k, m);
+    if (last_error() != RMQ_OK) return RMQ_EDEVICE;
   }
+  return RMQ_OK;
 }
 
-// Level k -> k + 1 of a hierarchy: values into dv, offset entries into drel; srel is null on
-// level 0 (positions are indices there).
-rmq_status build_level(uint32_t R_log, uint32_t k, const float* sv, const uint8_t* srel, uint32_t ssize,
-                       float* dv, uint8_t* drel, uint32_t dsize, uint32_t* zeros, hipStream_t s) {
-  const uint32_t ssk = R_log * k, dsk = R_log * (k + 1);
+rmq_status build_levels(uint32_t R_log, const HierView& H, hipStream_t s) {
   switch (R_log) {
-    case 2: launch_build_level<2>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
-    case 3: launch_build_level<3>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
-    case 4: launch_build_level<4>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
-    case 5: launch_build_level<5>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
-    case 6: launch_build_level<6>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
-    case 7: launch_build_level<7>(sv, srel, ssk, ssize, dv, drel, dsk, dsize, zeros, s); break;
+    case 2: return launch_passes<2>(H, s);
+    case 3: return launch_passes<3>(H, s);
+    case 4: return launch_passes<4>(H, s);
+    case 5: return launch_passes<5>(H, s);
+    case 6: return launch_passes<6>(H, s);
+    case 7: return launch_passes<7>(H, s);
     default: return RMQ_EUNSUPPORTED;
   }
-  return last_error();
 }
 
 // Byte offsets of the levels' offset entries (rel_bytes per level, 16-byte aligned, four entries
@@ -620,16 +616,10 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
     rmq_free(h);
     return RMQ_EDEVICE;
   }
-  for (uint32_t k = 0; k + 1 < P.num_levels; ++k) {
-    const float* sv = k == 0 ? d_array : h->lv + P.off[k];
-    const uint8_t* srel = k == 0 ? nullptr : h->lr + H.roff[k];
-    const rmq_status bs =
-        build_level(P.R_log, k, sv, srel, static_cast<uint32_t>(P.size[k]), h->lv + P.off[k + 1],
-                    h->lr + H.roff[k + 1], static_cast<uint32_t>(P.size[k + 1]), h->zeros, s);
-    if (bs != RMQ_OK) {
-      rmq_free(h);
-      return bs;
-    }
+  const rmq_status bs = build_levels(P.R_log, H, s);
+  if (bs != RMQ_OK) {
+    rmq_free(h);
+    return bs;
   }
   const rmq_status tst = build_top_table(h, P, ts, d_array, s);
   if (tst != RMQ_OK) {

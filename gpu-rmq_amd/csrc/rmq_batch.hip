@@ -24,14 +24,6 @@ using rmq::kBlock;
 
 namespace {
 
-constexpr uint32_t kBuildTile = 4096;    // level-(k+1) entries per build tile
-
-struct BuildTile {
-  uint32_t array;
-  uint32_t level;  // source level k
-  uint32_t j0, j1; // destination entries [j0, j1) of level k+1
-};
-
 using rmq::QueryTile;
 using rmq::kQueues;
 
@@ -86,41 +78,26 @@ __global__ __launch_bounds__(kBlock) void k_fill_uniform(float* __restrict__ out
   }
 }
 
-// ---- build: one launch per (source level, R), one workgroup per tile -------------------------
+// ---- build: one launch per (pass, R) over every array with that R ------------------------------
+// Workgroup b builds tile b - first[a] of array list[a], a the last with first[a] <= b
+// (first: each listed array's first tile, ascending; first[count] = the launch's tiles).
 template <int RL>
-__global__ __launch_bounds__(kBlock) void k_build_batch(const HierView* __restrict__ views,
-                                                        const BuildTile* __restrict__ tiles) {
-  constexpr uint32_t kR = 1u << RL;
-  constexpr uint32_t W = kR / 4;  // lanes per block: 8 at R = 32, 2 at R = 8, 1 at R = 4
-  const BuildTile T = tiles[blockIdx.x];
-  const HierView& H = views[T.array];
-  const uint32_t k = T.level;
-  const float* sv = k == 0 ? H.A : H.lv + H.off[k];
-  const uint8_t* srel = k == 0 ? nullptr : H.lr + H.roff[k];
-  const uint32_t ssize = H.size[k];
-  float* dv = const_cast<float*>(H.lv) + H.off[k + 1];
-  uint8_t* drel = const_cast<uint8_t*>(H.lr) + H.roff[k + 1];
-  const uint32_t gl = threadIdx.x & (W - 1);
-  uint32_t zb = 0;
-  for (uint32_t j = T.j0 + threadIdx.x / W; j < T.j1; j += kBlock / W) {
-    float bv = RMQ_INF;
-    uint32_t bi = dev::kNoIdx;
-    const uint32_t pos = j * kR + 4 * gl;
-    if (pos < ssize) {
-      const float4 x = dev::load4(sv, pos, ssize);
-      dev::fold4_idx(bv, bi, x, srel, RL * k, pos, pos, ssize);
-      if (k == 0)
-        zb |= dev::zero_bits(x.x, true) | dev::zero_bits(x.y, pos + 1 < ssize) |
-              dev::zero_bits(x.z, pos + 2 < ssize) | dev::zero_bits(x.w, pos + 3 < ssize);
-    }
-    dev::group_argmin<W>(bv, bi);
-    if (k == 0 && bi == dev::kNoIdx) bi = j * kR;  // a block of NaNs: its first position, value +inf
-    if (gl == 0) {
-      dv[j] = bv;
-      dev::rel_store(drel, j, RL * (k + 1), bi);
-    }
+__global__ __launch_bounds__(kBlock) void k_build_pass_batch(const HierView* __restrict__ views,
+                                                             const uint32_t* __restrict__ list,
+                                                             const uint32_t* __restrict__ first,
+                                                             uint32_t count, uint32_t k) {
+  __shared__ float s_v[PassShape<RL>::LDS];
+  __shared__ uint32_t s_i[PassShape<RL>::LDS];
+  const uint32_t b = blockIdx.x;
+  uint32_t lo = 0, hi = count;  // first[lo] <= b < first[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid;
   }
-  if (k == 0) dev::publish_zeros(const_cast<uint32_t*>(H.zeros), zb);
+  const HierView& H = views[list[lo]];
+  const uint32_t m = min(PassShape<RL>::M, H.num_levels - 1 - k);
+  build_pass_tile<RL>(H, k, m, b - first[lo], s_v, s_i);
 }
 
 // The sparse table over the top level of array list[blockIdx.x] (64 <= m <= kTopSmall entries),
@@ -383,67 +360,81 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
   b->views = views;
   b->lanes_only = std::all_of(b->variant.begin(), b->variant.end(), [](uint32_t v) { return rmq::variant_is_lane(v); });
 
-  // build tiles, grouped by source level and reduction factor (R = 32, then R = 8, then R = 4)
+  // build passes, grouped by reduction factor and pass (a pass from level k makes levels
+  // k+1 .. k+M of every array of that R with levels above k); one device word array holds the
+  // top-table list, then each group's array list and first tiles
+  struct PassGroup {
+    uint32_t rl, k, cnt, list, first, tiles;
+  };
   constexpr uint32_t kRLogs[3] = {rmq::kMultiRLog, rmq::kLane8RLog, rmq::kLane4RLog};
-  std::vector<BuildTile> tiles;
-  std::vector<uint32_t> start;  // [level][3] -> tile index
-  for (uint32_t k = 0; k + 1 < max_levels; ++k) {
-    for (uint32_t rl : kRLogs) {
-      start.push_back(static_cast<uint32_t>(tiles.size()));
+  std::vector<uint32_t> words(tt_list);
+  std::vector<PassGroup> groups;
+  for (uint32_t rl : kRLogs) {
+    const uint32_t M = rmq::kPassTileLog / rl;  // PassShape<rl>::M
+    for (uint32_t k = 0; k + 1 < max_levels; k += M) {
+      std::vector<uint32_t> list, first;
+      uint64_t tiles = 0;
       for (uint32_t a = 0; a < count; ++a) {
         if (rmq::variant_rlog(arrays[a].variant) != rl || views[a].num_levels <= k + 1) continue;
-        const uint32_t dsize = views[a].size[k + 1];
-        for (uint32_t j = 0; j < dsize; j += kBuildTile)
-          tiles.push_back(BuildTile{a, k, j, std::min(dsize, j + kBuildTile)});
+        list.push_back(a);
+        first.push_back(static_cast<uint32_t>(tiles));
+        tiles += (static_cast<uint64_t>(views[a].size[k]) + (1u << rmq::kPassTileLog) - 1) >> rmq::kPassTileLog;
       }
+      if (list.empty()) continue;
+      if (tiles > 0xFFFFFFFFull / kBlock) {  // one launch's threads must fit in 32 bits
+        rmq_batch_free(b);
+        return RMQ_EINVAL;
+      }
+      first.push_back(static_cast<uint32_t>(tiles));
+      const PassGroup g{rl, k, static_cast<uint32_t>(list.size()), static_cast<uint32_t>(words.size()),
+                        static_cast<uint32_t>(words.size() + list.size()), static_cast<uint32_t>(tiles)};
+      words.insert(words.end(), list.begin(), list.end());
+      words.insert(words.end(), first.begin(), first.end());
+      groups.push_back(g);
     }
   }
-  start.push_back(static_cast<uint32_t>(tiles.size()));
-  BuildTile* d_tiles = nullptr;
-  uint32_t* d_tt_list = nullptr;
-  if (!tt_list.empty()) {
-    st = dmalloc(&d_tt_list, tt_list.size());
+  uint32_t* d_words = nullptr;
+  if (!words.empty()) {
+    st = dmalloc(&d_words, words.size());
     if (st != RMQ_OK) {
-      rmq_batch_free(b);
-      return st;
-    }
-  }
-  if (!tiles.empty()) {
-    st = dmalloc(&d_tiles, tiles.size());
-    if (st != RMQ_OK) {
-      if (d_tt_list) (void)hipFree(d_tt_list);
       rmq_batch_free(b);
       return st;
     }
   }
   bool ok = hipMemcpyAsync(b->d_views, views.data(), count * sizeof(HierView), hipMemcpyHostToDevice, s) == hipSuccess &&
             hipMemcpyAsync(b->d_variant, b->variant.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice, s) == hipSuccess;
-  if (ok && !tiles.empty())
-    ok = hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(BuildTile), hipMemcpyHostToDevice, s) == hipSuccess;
-  if (ok && !tt_list.empty())
-    ok = hipMemcpyAsync(d_tt_list, tt_list.data(), tt_list.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess;
+  if (ok && !words.empty())
+    ok = hipMemcpyAsync(d_words, words.data(), words.size() * 4, hipMemcpyHostToDevice, s) == hipSuccess;
   if (ok) {
     rmq::k_init_zeros<<<(count + 255) / 256, 256, 0, s>>>(b->d_views, count);
     ok = hipGetLastError() == hipSuccess;
   }
-  for (size_t g = 0; ok && g + 1 < start.size(); ++g) {
-    const uint32_t t0 = start[g], t1 = start[g + 1];
-    if (t1 == t0) continue;
-    switch (kRLogs[g % 3]) {
-      case rmq::kMultiRLog: rmq::k_build_batch<rmq::kMultiRLog><<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0); break;
-      case rmq::kLane8RLog: rmq::k_build_batch<rmq::kLane8RLog><<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0); break;
-      default: rmq::k_build_batch<rmq::kLane4RLog><<<t1 - t0, kBlock, 0, s>>>(b->d_views, d_tiles + t0); break;
+  // passes in order of k within each R (a pass reads the level the one before it made)
+  for (const PassGroup& g : groups) {
+    if (!ok) break;
+    const uint32_t* list = d_words + g.list;
+    const uint32_t* first = d_words + g.first;
+    switch (g.rl) {
+      case rmq::kMultiRLog:
+        rmq::k_build_pass_batch<rmq::kMultiRLog><<<g.tiles, kBlock, 0, s>>>(b->d_views, list, first, g.cnt, g.k);
+        break;
+      case rmq::kLane8RLog:
+        rmq::k_build_pass_batch<rmq::kLane8RLog><<<g.tiles, kBlock, 0, s>>>(b->d_views, list, first, g.cnt, g.k);
+        break;
+      default:
+        rmq::k_build_pass_batch<rmq::kLane4RLog><<<g.tiles, kBlock, 0, s>>>(b->d_views, list, first, g.cnt, g.k);
+        break;
     }
     ok = hipGetLastError() == hipSuccess;
   }
+  const uint32_t* d_tt_list = d_words;
   if (ok && !tt_list.empty()) {  // after every level: the tables read the top levels
     rmq::k_top_table_batch<<<static_cast<uint32_t>(tt_list.size()), rmq::kTopThreads, 0, s>>>(b->d_views, d_tt_list);
     ok = hipGetLastError() == hipSuccess;
   }
   // the host vectors die here: wait for the copies (and the build) before returning
   ok = ok && hipStreamSynchronize(s) == hipSuccess;
-  if (d_tiles) (void)hipFree(d_tiles);
-  if (d_tt_list) (void)hipFree(d_tt_list);
+  if (d_words) (void)hipFree(d_words);
   if (!ok) {
     rmq_batch_free(b);
     return RMQ_EDEVICE;

@@ -340,27 +340,6 @@ __device__ __forceinline__ uint4 rel_load4(const uint8_t* __restrict__ r, uint32
   return *reinterpret_cast<const uint4*>(r + 4u * pos);
 }
 
-// Build fold over the vector x at level-k position i, elements in [lo, hi). The candidates'
-// level-0 indices: the positions themselves on level 0 (srel null), else (position << span_log)
-// plus the level's offset entries.
-__device__ __forceinline__ void fold4_idx(float& bv, uint32_t& bi, float4 x, const uint8_t* srel,
-                                          uint32_t span_log, uint32_t i, uint32_t lo, uint32_t hi) {
-  uint4 id;
-  if (srel) {
-    id = rel_load4(srel, i, rel_bytes(span_log));
-    id.x += (i + 0) << span_log;
-    id.y += (i + 1) << span_log;
-    id.z += (i + 2) << span_log;
-    id.w += (i + 3) << span_log;
-  } else {
-    id = make_uint4(i, i + 1, i + 2, i + 3);
-  }
-  take(bv, bi, x.x, id.x, in_range(i + 0, lo, hi));
-  take(bv, bi, x.y, id.y, in_range(i + 1, lo, hi));
-  take(bv, bi, x.z, id.z, in_range(i + 2, lo, hi));
-  take(bv, bi, x.w, id.w, in_range(i + 3, lo, hi));
-}
-
 // Stores the offset entry of level position j (blocks of 2^span_log elements) for winner bi.
 __device__ __forceinline__ void rel_store(uint8_t* __restrict__ drel, uint32_t j, uint32_t span_log, uint32_t bi) {
   const uint32_t w = rel_bytes(span_log);

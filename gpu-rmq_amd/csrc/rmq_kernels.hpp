@@ -16,53 +16,122 @@ constexpr int kBlock = 256;  // 4 waves per workgroup
 
 // ---------------------------------------------------------------------------------------
 // Build: level k+1 entry j = min of level k over [j*R, min(j*R+R, size_k)) with the leftmost
-// level-0 index of that minimum (reference: xxx_build_step, src/xxx.h:91-120, which keeps
-// the value only), stored as an offset entry (rel_store). W = R/4 lanes share one block, each
-// lane owns 16-byte vectors, so one wave instruction streams 1 KiB of contiguous level-k data;
-// the W lanes meet in a DPP all-reduce. Grid-stride over output entries. ssk / dsk: log2 of
-// the level-0 span of a source / destination entry. The level-0 pass (SRC_IDX false) also
-// records which signs of zero the array holds in *zeros.
-template <int RL, bool SRC_IDX>
-__global__ __launch_bounds__(kBlock) void k_build_level(const float* __restrict__ sv,
-                                                        const uint8_t* __restrict__ srel, uint32_t ssk,
-                                                        uint32_t ssize, float* __restrict__ dv,
-                                                        uint8_t* __restrict__ drel, uint32_t dsk,
-                                                        uint32_t dsize, uint32_t* __restrict__ zeros) {
+// level-0 index of that minimum (reference: xxx_build_step, src/xxx.h:91-120, one launch per
+// level, which keeps the value only), stored as an offset entry (rel_store).
+//
+// Here several levels are made per pass. One workgroup reduces a tile of 2^SLOG = 4096 level-k
+// entries (whole blocks of level k + M): sweeps of 1024 entries, each lane a 16-byte vector (one wave
+// instruction streams 1 KiB of contiguous level-k data), W = R/4 lanes meeting in the DPP
+// argmin, give the tile's level-(k+1) entries, which are written out and kept in LDS; levels
+// k+2 .. k+m are then reduced in LDS, one thread per entry scanning its R sources left to
+// right, and written as they are made. Level k is read once for m levels, where one pass per
+// level would read every level once more for the next.
+constexpr uint32_t kPassTileLog = 12;  // level-k entries per build tile (log2): four sweeps
+
+template <int RL>
+struct PassShape {
+  static constexpr uint32_t SLOG = kPassTileLog;
+  static constexpr uint32_t M = SLOG / RL;    // levels per pass: 6 at R = 4, 2 at R = 32
+  static constexpr uint32_t LDS = 1u << (SLOG - RL);  // level-(k+1) entries of one tile
+};
+
+// Tile `tile` of the pass from level k of H making levels k+1 .. k+m (m <= M). s_v / s_i hold
+// PassShape<RL>::LDS entries. Level 0 (k = 0) also ORs its zero-sign / NaN bits into *H.zeros.
+template <int RL>
+__device__ __forceinline__ void build_pass_tile(const HierView& H, uint32_t k, uint32_t m, uint32_t tile,
+                                                float* s_v, uint32_t* s_i) {
+  using P = PassShape<RL>;
   constexpr uint32_t R = 1u << RL;
-  constexpr uint32_t V = R / 4;                 // 16-byte vectors per block
-  constexpr uint32_t W = V < 64 ? V : 64;       // lanes per block
-  constexpr uint32_t PER = V / W;               // vectors per lane
+  constexpr uint32_t W = R / 4;                        // lanes per block of R entries (<= 32)
+  constexpr uint32_t kSweep = kBlock * 4;              // level-k entries per sweep
+  constexpr uint32_t kSweeps = (1u << P::SLOG) / kSweep;
+  constexpr uint32_t kPerSweep = kSweep / R;           // level-(k+1) entries per sweep
+  const bool l0 = k == 0;
+  const float* sv = l0 ? H.A : H.lv + H.off[k];
+  const uint8_t* srel = l0 ? nullptr : H.lr + H.roff[k];
+  const uint32_t ssize = H.size[k];
+  const uint32_t t0 = tile << P::SLOG;
+  float* dv = const_cast<float*>(H.lv) + H.off[k + 1];
+  uint8_t* drel = const_cast<uint8_t*>(H.lr) + H.roff[k + 1];
+  const uint32_t dsize = H.size[k + 1];
   const uint32_t gl = threadIdx.x & (W - 1);
-  const uint64_t grp = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / W;
-  const uint64_t ngrp = static_cast<uint64_t>(gridDim.x) * kBlock / W;
+  const uint32_t sw = rel_bytes(RL * k);
   uint32_t zb = 0;
-  for (uint64_t j = grp; j < dsize; j += ngrp) {
-    float bv = RMQ_INF;
-    uint32_t bi = dev::kNoIdx;
-    const uint32_t base = static_cast<uint32_t>(j) * R;
+  // kBatch sweeps' loads are issued before any of them is reduced
+  constexpr uint32_t kBatch = kSweeps < 4 ? kSweeps : 4;
+  for (uint32_t w0 = 0; w0 < kSweeps; w0 += kBatch) {
+    float4 x[kBatch];
+    uint4 id[kBatch];
 #pragma unroll
-    for (uint32_t p = 0; p < PER; ++p) {
-      const uint32_t pos = base + 4 * (p * W + gl);
+    for (uint32_t b = 0; b < kBatch; ++b) {
+      const uint32_t pos = t0 + (w0 + b) * kSweep + 4 * threadIdx.x;
+      x[b] = make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+      id[b] = make_uint4(0, 0, 0, 0);
       if (pos < ssize) {
-        const float4 x = dev::load4(sv, pos, ssize);
-        dev::fold4_idx(bv, bi, x, SRC_IDX ? srel : nullptr, ssk, pos, pos, ssize);
-        if constexpr (!SRC_IDX) {
-          zb |= dev::zero_bits(x.x, true) | dev::zero_bits(x.y, pos + 1 < ssize) |
-                dev::zero_bits(x.z, pos + 2 < ssize) | dev::zero_bits(x.w, pos + 3 < ssize);
-        }
+        x[b] = dev::load4(sv, pos, ssize);
+        if (!l0) id[b] = dev::rel_load4(srel, pos, sw);
       }
     }
-    dev::group_argmin<W>(bv, bi);
-    // a level-0 block of NaNs (no element is ever taken): its first position, value +inf, which
-    // never wins over a real element left of it and is never asked for on its own (a query
-    // whose range holds no other element starts at a NaN, dev::nan_at_l)
-    if (!SRC_IDX && bi == dev::kNoIdx) bi = base;
-    if (gl == 0) {
-      dv[j] = bv;
-      dev::rel_store(drel, static_cast<uint32_t>(j), dsk, bi);
+#pragma unroll
+    for (uint32_t b = 0; b < kBatch; ++b) {
+      const uint32_t pos = t0 + (w0 + b) * kSweep + 4 * threadIdx.x;
+      const uint32_t sh = RL * k;  // level-0 span of a level-k entry (log2)
+      float bv = RMQ_INF;
+      uint32_t bi = dev::kNoIdx;
+      dev::take(bv, bi, x[b].x, id[b].x + ((pos + 0) << sh), pos + 0 < ssize);
+      dev::take(bv, bi, x[b].y, id[b].y + ((pos + 1) << sh), pos + 1 < ssize);
+      dev::take(bv, bi, x[b].z, id[b].z + ((pos + 2) << sh), pos + 2 < ssize);
+      dev::take(bv, bi, x[b].w, id[b].w + ((pos + 3) << sh), pos + 3 < ssize);
+      if (l0)
+        zb |= dev::zero_bits(x[b].x, pos < ssize) | dev::zero_bits(x[b].y, pos + 1 < ssize) |
+              dev::zero_bits(x[b].z, pos + 2 < ssize) | dev::zero_bits(x[b].w, pos + 3 < ssize);
+      dev::group_argmin<W>(bv, bi);
+      const uint32_t lj = (w0 + b) * kPerSweep + threadIdx.x / W;
+      const uint32_t j = (t0 >> RL) + lj;
+      if (gl == 0) {
+        if (j < dsize) {
+          // a level-0 block of NaNs (no element is ever taken): its first position, value +inf,
+          // which never wins over a real element left of it and is never asked for on its own
+          // (a query whose range holds no other element starts at a NaN, dev::nan_at_l)
+          if (l0 && bi == dev::kNoIdx) bi = j * R;
+          dv[j] = bv;
+          dev::rel_store(drel, j, RL * (k + 1), bi);
+        }
+        s_v[lj] = bv;
+        s_i[lj] = bi;
+      }
     }
   }
-  if constexpr (!SRC_IDX) dev::publish_zeros(zeros, zb);
+  if (l0) dev::publish_zeros(const_cast<uint32_t*>(H.zeros), zb);
+  uint32_t cnt = P::LDS;
+  for (uint32_t L = 2; L <= m; ++L) {
+    __syncthreads();
+    cnt >>= RL;
+    float v = RMQ_INF;
+    uint32_t i = dev::kNoIdx;
+    if (threadIdx.x < cnt) {
+#pragma unroll 8
+      for (uint32_t r = 0; r < R; ++r) dev::take(v, i, s_v[threadIdx.x * R + r], s_i[threadIdx.x * R + r], true);
+    }
+    __syncthreads();
+    if (threadIdx.x < cnt) {
+      s_v[threadIdx.x] = v;
+      s_i[threadIdx.x] = i;
+      const uint32_t j = (tile << (P::SLOG - RL * L)) + threadIdx.x;
+      if (j < H.size[k + L]) {
+        const_cast<float*>(H.lv)[H.off[k + L] + j] = v;
+        dev::rel_store(const_cast<uint8_t*>(H.lr) + H.roff[k + L], j, RL * (k + L), i);
+      }
+    }
+  }
+}
+
+// One pass of a plain build: tiles 0 .. gridDim.x - 1 of level k of H.
+template <int RL>
+__global__ __launch_bounds__(kBlock) void k_build_pass(HierView H, uint32_t k, uint32_t m) {
+  __shared__ float s_v[PassShape<RL>::LDS];
+  __shared__ uint32_t s_i[PassShape<RL>::LDS];
+  build_pass_tile<RL>(H, k, m, blockIdx.x, s_v, s_i);
 }
 
 // The sparse table over a top level top[0, m), m <= kTopSmall, built by one workgroup of

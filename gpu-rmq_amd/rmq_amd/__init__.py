@@ -268,25 +268,25 @@ class PlainMany:
         torch = _torch()
         _check_device_tensor(arrays, "arrays", (torch.float32,))
         self.count = count = len(sizes)
-        base = arrays.data_ptr()
-        ptrs = (ctypes.c_void_p * count)(*[base + 4 * int(o) for o in offsets])
-        ns = (ctypes.c_uint64 * count)(*[int(x) for x in sizes])
-        ts = (ctypes.c_uint64 * count)(*[int(x) for x in t])
+        # pointer and size tables as numpy arrays (no per-array Python work on the host path)
+        ptrs = np.uint64(arrays.data_ptr()) + 4 * np.asarray(offsets, dtype=np.uint64)
+        ns = np.asarray(sizes, dtype=np.uint64)
+        ts = np.asarray(t, dtype=np.uint64)
         self._hs = (ctypes.c_void_p * count)()
         self.arrays = arrays
-        _lib.check(_lib.lib().rmq_build_each(ptrs, ns, ts, alg, None, count, _stream_handle(stream),
-                                             self._hs), "rmq_build_each")
+        _lib.check(_lib.lib().rmq_build_each(ptrs.ctypes.data, ns.ctypes.data, ts.ctypes.data, alg, None,
+                                             count, _stream_handle(stream), self._hs), "rmq_build_each")
 
     def query(self, lr, qoffsets, out_idx, stream=None) -> None:
         """Index answers of every array's queries (local pairs in lr, array a's pairs at
         qoffsets[a]..qoffsets[a+1]) into out_idx at the same positions."""
         c = self.count
-        qb, ob = lr.data_ptr(), out_idx.data_ptr()
-        qs = (ctypes.c_void_p * c)(*[qb + 8 * int(qoffsets[a]) for a in range(c)])
-        os_ = (ctypes.c_void_p * c)(*[ob + 4 * int(qoffsets[a]) for a in range(c)])
-        qn = (ctypes.c_uint64 * c)(*[int(qoffsets[a + 1] - qoffsets[a]) for a in range(c)])
-        _lib.check(_lib.lib().rmq_query_each(self._hs, qs, qn, None, os_, c, _stream_handle(stream)),
-                   "rmq_query_each")
+        qo = np.asarray(qoffsets, dtype=np.uint64)
+        qs = np.uint64(lr.data_ptr()) + 8 * qo[:c]
+        os_ = np.uint64(out_idx.data_ptr()) + 4 * qo[:c]
+        qn = qo[1:c + 1] - qo[:c]
+        _lib.check(_lib.lib().rmq_query_each(self._hs, qs.ctypes.data, qn.ctypes.data, None, os_.ctypes.data,
+                                             c, _stream_handle(stream)), "rmq_query_each")
 
     def close(self) -> None:
         if getattr(self, "_hs", None) is not None:
@@ -354,6 +354,12 @@ def fill_uniform(out, seed: int, start: int = 0, stream=None) -> None:
                                            _stream_handle(stream)), "rmq_fill_uniform")
 
 
+# numpy layout of rmq_batch_array (include/rmq_batch.h), checked against the ctypes record
+_BATCH_ARRAY_DTYPE = np.dtype([("offset", np.uint64), ("n", np.uint32), ("t", np.uint32),
+                               ("variant", np.uint32), ("reserved", np.uint32)])
+assert _BATCH_ARRAY_DTYPE.itemsize == ctypes.sizeof(_lib.BatchArray)
+
+
 class Batch:
     """Many independent arrays, one build launch per level, one persistent query launch.
 
@@ -365,18 +371,18 @@ class Batch:
         torch = _torch()
         _check_device_tensor(arrays, "arrays", (torch.float32,))
         count = len(sizes)
-        descs = (_lib.BatchArray * count)()
-        for a in range(count):
-            v = variant[a]
-            descs[a].offset = int(offsets[a])
-            descs[a].n = int(sizes[a])
-            descs[a].t = int(t[a])
-            descs[a].variant = VARIANTS[v] if isinstance(v, str) else int(v)
+        # rmq_batch_array records, filled column-wise (a Python loop over ctypes fields costs
+        # milliseconds at thousands of arrays)
+        descs = np.zeros(count, dtype=_BATCH_ARRAY_DTYPE)
+        descs["offset"] = np.asarray(offsets, dtype=np.uint64)
+        descs["n"] = np.asarray(sizes, dtype=np.uint32)
+        descs["t"] = np.asarray(t, dtype=np.uint32)
+        descs["variant"] = [VARIANTS[v] if isinstance(v, str) else int(v) for v in variant]
         self.arrays = arrays
         self.count = count
         self._b = ctypes.c_void_p()
         self._plans = []
-        _lib.check(_lib.lib().rmq_batch_build(arrays.data_ptr(), descs, count,
+        _lib.check(_lib.lib().rmq_batch_build(arrays.data_ptr(), descs.ctypes.data, count,
                                               _stream_handle(stream), ctypes.byref(self._b)),
                    "rmq_batch_build")
 
