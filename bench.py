@@ -186,7 +186,7 @@ def time_queries(h, dQ, want_val, want_idx, out_v, out_i, steps, warmup):
     return wall, ev0.elapsed_time(ev1) / steps
 
 
-def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True):
+def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True, peaks=None):
     """4096 arrays (sizes log-U[2^10, 2^24], mixed lengths, queries log-U[2^6, 2^16] per array):
     one batched build + one persistent query launch vs per-array plain launches (rmq_build +
     rmq_query per array); identical answers required. baseline / graph_too = False leave the
@@ -196,7 +196,7 @@ def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True):
     import torch
 
     import rmq_amd
-    from rmq_amd import autotune, workloads
+    from rmq_amd import autotune, roofline, workloads
 
     t0 = time.perf_counter()
     w = workloads.batch_workload(arrays=arrays, log_n=(10, 24), log_q=(6, 16), kind="mixed",
@@ -256,6 +256,14 @@ def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True):
         "batched_build_ms": round(build_b, 3),
         "batched_gqs": round(Q / query_b / 1e6, 3),
     }
+    # the build's algorithmic bytes (level 0 read once, levels and offset entries written once)
+    # at its wall time, allocations included
+    rlog = {"lane4": 2, "lane8": 3}
+    bb = roofline.build_bytes(w.sizes, ts, [rlog.get(v, 5) for v in var])
+    res["batched_build_gb"] = round(bb / 1e9, 3)
+    res["batched_build_gbps_wall"] = round(bb / (build_b / 1e3) / 1e9, 1)
+    res["roofline"] = roofline_block(f"batched {arrays} arrays log-U[2^10,2^24] mixed index",
+                                     roofline.total_bytes(w.queries, 1), Q, query_b, peaks)
     if graph_too:
         graph = plan.capture(dQ, None, out_b)
         query_g = timed(lambda: graph.launch())
@@ -463,8 +471,8 @@ def main():
                                              args.serve_seconds)}), flush=True)
         return
     if args.batched_only:
-        print(json.dumps({"batched": batched_leg(args.batch_arrays, baseline=False, graph_too=False)}),
-              flush=True)
+        print(json.dumps({"batched": batched_leg(args.batch_arrays, baseline=False, graph_too=False,
+                                                 peaks=issue_peaks())}), flush=True)
         return
 
     peaks = issue_peaks()  # its own process, before this one touches the GPU
@@ -567,6 +575,7 @@ def main():
         },
         "roofline": roofline_block(counters_key, bytes_total, q, ev_ms, peaks),
         "build_ms": round(build_ms, 3),
+        "build_gbps_wall": round(roofline.build_bytes(n, t, int(meta.R).bit_length() - 1) / (build_ms / 1e3) / 1e9, 1),
         "sweep": sweep,
     }
     del out_v
@@ -578,7 +587,7 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_batched:
         log("batched leg")
-        result["batched"] = batched_leg(args.batch_arrays)
+        result["batched"] = batched_leg(args.batch_arrays, peaks=peaks)
         torch.cuda.empty_cache()
     if not args.no_serve:
         log("serving leg")

@@ -62,8 +62,26 @@ def total_bytes(lr: np.ndarray, k_out: int, chunk: int = 1 << 22) -> int:
     return tot
 
 
-def build_bytes(sizes) -> int:
-    """Each non-top level read once, each upper level written once (value + index arrays)."""
-    read = 4 * sum(sizes[:-1])
-    write = 8 * sum(sizes[1:])
-    return read + write
+def _rel_bytes(span_log: np.ndarray) -> np.ndarray:
+    """Bytes of one offset entry whose block spans 2^span_log level-0 elements."""
+    return np.where(span_log <= 8, 1, np.where(span_log <= 16, 2, 4))
+
+
+def build_bytes(n, t, r_log) -> int:
+    """Algorithmic HBM bytes of building hierarchies over arrays of n elements (scalars or
+    arrays, one per hierarchy) with scan threshold t and R = 2^r_log: level 0 read once (4 B per
+    element), every upper-level entry written once (4 B value + its offset entry). Levels follow
+    the metadata recurrence (src/xxx.h:225-243): while a level holds more than t entries and
+    fewer than 20 levels exist, the next holds ceil(size / R)."""
+    n = np.atleast_1d(np.asarray(n, dtype=np.int64))
+    t = np.broadcast_to(np.asarray(t, dtype=np.int64), n.shape)
+    rl = np.broadcast_to(np.asarray(r_log, dtype=np.int64), n.shape)
+    total = 4 * int(n.sum())
+    size = n.copy()
+    for k in range(1, 20):
+        live = size > t
+        if not live.any():
+            break
+        size = np.where(live, (size + (1 << rl) - 1) >> rl, size)
+        total += int(np.where(live, size * (4 + _rel_bytes(rl * k)), 0).sum())
+    return total
