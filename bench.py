@@ -73,11 +73,29 @@ def issue_peaks():
     return out
 
 
-def roofline_block(key, bytes_total, q, ev_ms, peaks=None):
+def hbm_ceilings():
+    """What plain kernels get from HBM on this device, measured by gpu-rmq_amd/hbm_probe (best
+    stream read with and without non-temporal loads, copy, the build's read/write mix, random
+    128-byte segments per second), or None."""
+    import subprocess
+
+    probe = os.path.join(ROOT, "gpu-rmq_amd", "hbm_probe")
+    if not os.path.exists(probe):
+        return None
+    p = subprocess.run([probe], capture_output=True, text=True, timeout=120)
+    if p.returncode != 0:
+        log(f"hbm_probe failed ({p.returncode}): {p.stderr[-500:]}")
+        return None
+    return [json.loads(x) for x in p.stdout.splitlines() if x.startswith("{")][-1]
+
+
+def roofline_block(key, bytes_total, q, ev_ms, peaks=None, ceilings=None):
     """The roofline of the timed kernel: every resource with a measured per-launch figure, as a
     fraction of its peak at the live kernel time; `bound` is the one closest to its peak. Issue
     peaks are the measured ones (issue_peaks) when given, else one wave-instruction per CU per
-    cycle."""
+    cycle. With measured HBM ceilings (hbm_ceilings), the traffic is also set against the best
+    stream rate of a plain kernel (hbm_achievable), and the fetched bytes, as 128-byte lines,
+    against the rate of independent random 128-byte fetches (hbm_random_lines)."""
     from rmq_amd import roofline
 
     ks = ev_ms / 1e3
@@ -91,6 +109,11 @@ def roofline_block(key, bytes_total, q, ev_ms, peaks=None):
     if "bytes_per_launch" in c:
         g = c["bytes_per_launch"] / ks / 1e9
         res["hbm"] = (g, roofline.HBM_PEAK_GBPS, "GB/s")
+        if ceilings:
+            res["hbm_achievable"] = (g, max(ceilings["read_gbps"], ceilings["read_nt_gbps"]), "GB/s")
+    if ceilings and "fetch_bytes" in c:
+        lines = c["fetch_bytes"] / 128 / ks / 1e9
+        res["hbm_random_lines"] = (lines, ceilings["random128_gsegments_per_s"], "G lines/s")
     vpeak = peaks["valu_peak_ginst_per_s"] if peaks else roofline.VALU_PEAK_GINST
     speak = peaks["salu_peak_ginst_per_s"] if peaks else roofline.SALU_PEAK_GINST
     if "valu_insts" in c:
@@ -107,6 +130,8 @@ def roofline_block(key, bytes_total, q, ev_ms, peaks=None):
     out["traffic"] = c.get("bytes_per_launch")
     out["resources"] = {k: {"achieved": round(v[0], 2), "peak": v[1], "unit": v[2], "frac": round(v[0] / v[1], 4)}
                         for k, v in res.items()}
+    if ceilings:
+        out["hbm_ceilings"] = dict(ceilings, source="measured by gpu-rmq_amd/hbm_probe")
     out["issue_peak_source"] = ("measured by gpu-rmq_amd/issue_probe: " + json.dumps(
         {k: v for k, v in peaks.items() if k != "kinds"})) if peaks else "model: one per CU per cycle"
     out["counters_key"] = key
@@ -186,7 +211,7 @@ def time_queries(h, dQ, want_val, want_idx, out_v, out_i, steps, warmup):
     return wall, ev0.elapsed_time(ev1) / steps
 
 
-def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True, peaks=None):
+def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True, peaks=None, ceilings=None):
     """4096 arrays (sizes log-U[2^10, 2^24], mixed lengths, queries log-U[2^6, 2^16] per array):
     one batched build + one persistent query launch vs per-array plain launches (rmq_build +
     rmq_query per array); identical answers required. baseline / graph_too = False leave the
@@ -262,8 +287,10 @@ def batched_leg(arrays: int, baseline: bool = True, graph_too: bool = True, peak
     bb = roofline.build_bytes(w.sizes, ts, [rlog.get(v, 5) for v in var])
     res["batched_build_gb"] = round(bb / 1e9, 3)
     res["batched_build_gbps_wall"] = round(bb / (build_b / 1e3) / 1e9, 1)
+    if ceilings:
+        res["batched_build_frac_of_read4w1"] = round(bb / (build_b / 1e3) / 1e9 / ceilings["read4w1_gbps"], 3)
     res["roofline"] = roofline_block(f"batched {arrays} arrays log-U[2^10,2^24] mixed index",
-                                     roofline.total_bytes(w.queries, 1), Q, query_b, peaks)
+                                     roofline.total_bytes(w.queries, 1), Q, query_b, peaks, ceilings)
     if graph_too:
         graph = plan.capture(dQ, None, out_b)
         query_g = timed(lambda: graph.launch())
@@ -472,10 +499,11 @@ def main():
         return
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(args.batch_arrays, baseline=False, graph_too=False,
-                                                 peaks=issue_peaks())}), flush=True)
+                                                 peaks=issue_peaks(), ceilings=hbm_ceilings())}), flush=True)
         return
 
-    peaks = issue_peaks()  # its own process, before this one touches the GPU
+    peaks = issue_peaks()  # its own processes, before this one touches the GPU
+    ceilings = hbm_ceilings()
 
     import numpy as np
     import torch
@@ -533,11 +561,13 @@ def main():
     # --- timed region --------------------------------------------------------------------------
     with rmq_amd.Hierarchy(dA, t, alg, **OPTS_OF[var]) as h:
         torch.cuda.synchronize()
-        tb0 = time.perf_counter()
-        with rmq_amd.Hierarchy(dA, t, alg, **OPTS_OF[var]) as h2:
-            torch.cuda.synchronize()
-            build_ms = (time.perf_counter() - tb0) * 1e3
-            meta = h2.metadata()
+        build_ms = 1e30  # best of five builds (allocation, level passes, top-level table)
+        for _ in range(5):
+            tb0 = time.perf_counter()
+            with rmq_amd.Hierarchy(dA, t, alg, **OPTS_OF[var]) as h2:
+                torch.cuda.synchronize()
+                build_ms = min(build_ms, (time.perf_counter() - tb0) * 1e3)
+                meta = h2.metadata()
         log(f"timing alg={var} t={t}: {args.warmup} warmup + {args.steps} steps")
         wall_ms, ev_ms = time_queries(h, dQ, want_val, want_idx, out_v, out_i,
                                       args.steps, args.warmup)
@@ -573,7 +603,7 @@ def main():
                         f"q=2^{args.log_q}, lr={lr_mode} ({LR_NAMES.get(lr_mode, lr_mode)})",
             "alg": var, "t": t, "R": meta.R, "levels": meta.sizes, "n": n, "q": q,
         },
-        "roofline": roofline_block(counters_key, bytes_total, q, ev_ms, peaks),
+        "roofline": roofline_block(counters_key, bytes_total, q, ev_ms, peaks, ceilings),
         "build_ms": round(build_ms, 3),
         "build_gbps_wall": round(roofline.build_bytes(n, t, int(meta.R).bit_length() - 1) / (build_ms / 1e3) / 1e9, 1),
         "sweep": sweep,
@@ -587,7 +617,7 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_batched:
         log("batched leg")
-        result["batched"] = batched_leg(args.batch_arrays, peaks=peaks)
+        result["batched"] = batched_leg(args.batch_arrays, peaks=peaks, ceilings=ceilings)
         torch.cuda.empty_cache()
     if not args.no_serve:
         log("serving leg")
