@@ -361,13 +361,15 @@ def serve_leg(rates, seconds: float):
         raise RuntimeError(f"serve_load failed ({p.returncode}): {p.stdout[-2000:]}")
     points = [json.loads(line) for line in p.stdout.splitlines() if line.startswith("{")]
     header = points.pop(0)
+    warm = [x["warmup_requests"] for x in points if "warmup_requests" in x]
     check = [x["check"] for x in points if "check" in x]
-    points = [x for x in points if "check" not in x]
+    points = [x for x in points if "check" not in x and "warmup_requests" not in x]
     return {
         "workload": "64 resident arrays log-U[2^20,2^28], Poisson arrivals, Zipf(1.1) targets, "
                     "1..2^16 queries/request (log-uniform), small/medium/large lognormal lengths, "
                     "index mode; open-loop C++ client (gpu-rmq_amd/serve_load)",
         "resident_gb": header.get("resident_gb"),
+        "warmup_requests": warm[0] if warm else 0,
         "curve": points,
         "check": check[0] if check else None,
     }

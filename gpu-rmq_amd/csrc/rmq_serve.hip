@@ -10,6 +10,7 @@
 // `slots` batches overlap on the device while the host stages and packs the next ones.
 
 #include <hip/hip_runtime.h>
+#include <emmintrin.h>
 
 #include <algorithm>
 #include <cstdint>
@@ -211,6 +212,39 @@ rmq_status launch(rmq_server* s, Slot& sl) {
   return ok ? RMQ_OK : RMQ_EDEVICE;
 }
 
+// Copies q {l, r} pairs from src into the staging ring at dst and returns nonzero when any pair
+// is outside 0 <= l <= r < n (n < 2^31). Two pairs per 16-byte step, checked with signed
+// compares of the values offset by 2^31 (which keeps their unsigned order), and written with
+// streaming stores: the ring is written once here and read by the device, so its lines are not
+// read into the cache first. The fence orders those stores before the launch that reads them.
+uint32_t copy_checked(uint32_t* dst, const uint32_t* src, uint64_t q, uint64_t n) {
+  uint32_t bad = 0;
+  uint64_t i = 0;
+  auto one = [&](uint64_t k) {
+    const uint32_t l = src[2 * k], r = src[2 * k + 1];
+    bad |= static_cast<uint32_t>(l > r) | static_cast<uint32_t>(r >= n);
+    dst[2 * k] = l;
+    dst[2 * k + 1] = r;
+  };
+  if ((reinterpret_cast<uintptr_t>(dst) & 15u) != 0 && q > 0) one(i++);  // pairs are 8-byte aligned
+  const __m128i bias = _mm_set1_epi32(static_cast<int32_t>(0x80000000u));
+  const __m128i nb = _mm_set1_epi32(static_cast<int32_t>(static_cast<uint32_t>(n) ^ 0x80000000u));
+  const __m128i lanes_l = _mm_set_epi32(0, -1, 0, -1);  // elements 0 and 2: l; 1 and 3: r
+  __m128i acc = _mm_setzero_si128();
+  for (; i + 2 <= q; i += 2) {
+    const __m128i v = _mm_loadu_si128(reinterpret_cast<const __m128i*>(src + 2 * i));
+    const __m128i b = _mm_xor_si128(v, bias);
+    const __m128i sw = _mm_shuffle_epi32(b, _MM_SHUFFLE(2, 3, 0, 1));  // r0 l0 r1 l1
+    const __m128i l_gt_r = _mm_and_si128(_mm_cmpgt_epi32(b, sw), lanes_l);
+    const __m128i r_ge_n = _mm_andnot_si128(_mm_or_si128(_mm_cmpgt_epi32(nb, b), lanes_l), _mm_set1_epi32(-1));
+    acc = _mm_or_si128(acc, _mm_or_si128(l_gt_r, r_ge_n));
+    _mm_stream_si128(reinterpret_cast<__m128i*>(dst + 2 * i), v);
+  }
+  if (i < q) one(i);
+  _mm_sfence();
+  return bad | static_cast<uint32_t>(_mm_movemask_epi8(acc) != 0);
+}
+
 }  // namespace
 
 extern "C" {
@@ -366,15 +400,7 @@ rmq_status rmq_server_submit(rmq_server* s, int32_t array_id, const uint32_t* h_
   // every pair is checked while it is copied: one bad pair refuses the whole request before
   // anything is queued, so no kernel ever reads outside a resident array
   const uint64_t n = s->arrays[array_id].n;
-  uint32_t* dst = s->h_ring_q + 2 * (pos % s->ring_cap);
-  uint32_t bad = 0;
-  for (uint64_t i = 0; i < q; ++i) {
-    const uint32_t l = h_lr[2 * i], r = h_lr[2 * i + 1];
-    bad |= static_cast<uint32_t>(l > r) | static_cast<uint32_t>(r >= n);
-    dst[2 * i] = l;
-    dst[2 * i + 1] = r;
-  }
-  if (bad) return RMQ_EINVAL;
+  if (copy_checked(s->h_ring_q + 2 * (pos % s->ring_cap), h_lr, q, n)) return RMQ_EINVAL;
   Request r;
   r.ticket = s->next_ticket++;
   r.array = array_id;
