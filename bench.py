@@ -375,7 +375,7 @@ def serve_leg(rates, seconds: float):
     }
 
 
-def uniform_leg(dA, n, steps, warmup):
+def uniform_leg(dA, n, steps, warmup, peaks=None, ceilings=None):
     """The uniform-length workload (lr = -1, L ~ U_int[1, n]) on the same array: the lane
     solver at R = 4, t = 2^10 (packed and not), R = 8, t = 2^13, and alg 16 at the reference
     default t = 2^15; the fastest is reported, alg 16 at t = 2^15 beside it. Index mode."""
@@ -383,6 +383,7 @@ def uniform_leg(dA, n, steps, warmup):
     import torch
 
     import rmq_amd
+    from rmq_amd import roofline
 
     q = n
     lr = rmq_amd.gen_queries(q, -1, n, SEED)
@@ -408,9 +409,12 @@ def uniform_leg(dA, n, steps, warmup):
     sel = torch.from_numpy(sidx.astype(np.int64)).cuda()
     assert torch.equal(out_i[sel], bi), "uniform-length output differs from GPU BASE"
     a16 = [r for r in runs if r["alg"] == "16"][0]
-    return {"workload": f"n=2^{n.bit_length() - 1}, q=2^{q.bit_length() - 1}, lr=-1 (uniform lengths), index mode",
+    ln, lq = n.bit_length() - 1, q.bit_length() - 1
+    key = f"alg={best['alg']} t={best['t']} n=2^{ln} q=2^{lq} lr=-1 index"
+    return {"workload": f"n=2^{ln}, q=2^{lq}, lr=-1 (uniform lengths), index mode",
             "gqs": best["gqs"], "alg": best["alg"], "t": best["t"], "ms": best["ms"],
             "alg16_t2e15_gqs": a16["gqs"], "runs": runs,
+            "roofline": roofline_block(key, roofline.total_bytes(lr, 1), q, best["ms"], peaks, ceilings),
             "spot_check": f"{sidx.size} sampled queries shorter than 2^22 equal GPU BASE"}
 
 
@@ -614,7 +618,8 @@ def main():
     if not args.no_uniform:
         log("uniform-length leg")
         del dQ
-        result["uniform"] = uniform_leg(dA, n, steps=max(3, args.steps // 2), warmup=1)
+        result["uniform"] = uniform_leg(dA, n, steps=max(3, args.steps // 2), warmup=1, peaks=peaks,
+                                        ceilings=ceilings)
     del dA
     torch.cuda.empty_cache()
     if not args.no_batched:
