@@ -81,14 +81,15 @@ __global__ __launch_bounds__(kBlock) void k_check_queries(const P* __restrict__ 
 // leftmost minimum of [i, i + 2^j), from two entries of row j - 1 (row 1 from the level itself).
 // Entries past m - 2^j are never read and stay unwritten.
 __global__ __launch_bounds__(kBlock) void k_top_row(const float* __restrict__ top, uint32_t m,
-                                                    uint2* __restrict__ tt, uint32_t stride, uint32_t j) {
+                                                    uint2* __restrict__ tt, uint32_t stride, uint32_t j,
+                                                    const uint8_t* __restrict__ lr, uint32_t roff, uint32_t sk) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t h = 1u << (j - 1u);
   if (static_cast<uint64_t>(i) + 2u * h > m) return;
   uint2 a, b;
   if (j == 1) {
-    a = make_uint2(dev::f2u(top[i]), i);
-    b = make_uint2(dev::f2u(top[i + 1]), i + 1);
+    a = make_uint2(dev::f2u(top[i]), level0_index(lr, roff, i, sk));
+    b = make_uint2(dev::f2u(top[i + 1]), level0_index(lr, roff, i + 1, sk));
   } else {
     const uint2* prev = tt + static_cast<size_t>(j - 2u) * stride;
     a = prev[i];
@@ -100,9 +101,10 @@ __global__ __launch_bounds__(kBlock) void k_top_row(const float* __restrict__ to
 // The whole table of a top level of at most kTopSmall entries in one workgroup.
 __global__ __launch_bounds__(kTopThreads) void k_top_table_small(const float* __restrict__ top, uint32_t m,
                                                                  uint2* __restrict__ tt, uint32_t stride,
-                                                                 uint32_t rows) {
+                                                                 uint32_t rows, const uint8_t* __restrict__ lr,
+                                                                 uint32_t roff, uint32_t sk) {
   __shared__ uint2 cur[kTopSmall];
-  top_table_block(top, m, tt, stride, rows, cur);
+  top_table_block(top, m, tt, stride, rows, cur, lr, roff, sk);
 }
 
 }  // namespace rmq
@@ -427,13 +429,16 @@ rmq_status build_top_table(rmq_hierarchy* h, const Plan& P, const TopShape& ts, 
   const uint32_t top = P.num_levels - 1;
   const uint64_t m = P.size[top];
   const float* tv = top == 0 ? d_array : h->lv + P.off[top];
+  const uint8_t* lr = top == 0 ? nullptr : h->lr;  // the top level's offset entries
+  const uint32_t roff = h->view.roff[top], sk = P.R_log * top;
   if (m <= rmq::kTopSmall) {
     rmq::k_top_table_small<<<1, rmq::kTopThreads, 0, s>>>(tv, static_cast<uint32_t>(m), h->tt,
-                                                          static_cast<uint32_t>(ts.stride), ts.rows);
+                                                          static_cast<uint32_t>(ts.stride), ts.rows, lr, roff, sk);
   } else {
     const uint32_t grid = static_cast<uint32_t>((m + kBlock - 1) / kBlock);
     for (uint32_t j = 1; j <= ts.rows; ++j)
-      rmq::k_top_row<<<grid, kBlock, 0, s>>>(tv, static_cast<uint32_t>(m), h->tt, static_cast<uint32_t>(ts.stride), j);
+      rmq::k_top_row<<<grid, kBlock, 0, s>>>(tv, static_cast<uint32_t>(m), h->tt, static_cast<uint32_t>(ts.stride), j,
+                                             lr, roff, sk);
   }
   h->view.tt = h->tt;
   h->view.tstride = static_cast<uint32_t>(ts.stride);

@@ -103,14 +103,17 @@ __global__ __launch_bounds__(kBlock) void k_build_pass_batch(const HierView* __r
 // The sparse table over the top level of array list[blockIdx.x] (64 <= m <= kTopSmall entries),
 // built in LDS by one workgroup, as the plain build's k_top_table_small does.
 __global__ __launch_bounds__(kTopThreads) void k_top_table_batch(const HierView* __restrict__ views,
+                                                                 const uint32_t* __restrict__ variant,
                                                                  const uint32_t* __restrict__ list) {
   __shared__ uint2 cur[kTopSmall];
-  const HierView& H = views[list[blockIdx.x]];
+  const uint32_t a = list[blockIdx.x];
+  const HierView& H = views[a];
   const uint32_t top = H.num_levels - 1;
   const uint32_t m = H.size[top];
   uint32_t rows = 0;
   while ((2u << rows) <= m) ++rows;  // floor(log2 m)
-  top_table_block(top == 0 ? H.A : H.lv + H.off[top], m, const_cast<uint2*>(H.tt), H.tstride, rows, cur);
+  top_table_block(top == 0 ? H.A : H.lv + H.off[top], m, const_cast<uint2*>(H.tt), H.tstride, rows, cur,
+                  top == 0 ? nullptr : H.lr, H.roff[top], variant_rlog(variant[a]) * top);
 }
 
 // Every array's level-0 word (cleared for the level-0 pass, or every bit when there is none)
@@ -429,7 +432,8 @@ rmq_status rmq_batch_build(const float* d_arrays, const rmq_batch_array* arrays,
   }
   const uint32_t* d_tt_list = d_words;
   if (ok && !tt_list.empty()) {  // after every level: the tables read the top levels
-    rmq::k_top_table_batch<<<static_cast<uint32_t>(tt_list.size()), rmq::kTopThreads, 0, s>>>(b->d_views, d_tt_list);
+    rmq::k_top_table_batch<<<static_cast<uint32_t>(tt_list.size()), rmq::kTopThreads, 0, s>>>(b->d_views, b->d_variant,
+                                                                                              d_tt_list);
     ok = hipGetLastError() == hipSuccess;
   }
   // the host vectors die here: wait for the copies (and the build) before returning

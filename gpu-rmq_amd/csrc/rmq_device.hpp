@@ -231,6 +231,9 @@ constexpr uint32_t kTopScan = 8;
 // and b does not (a NaN never wins; ties keep the left entry, which holds the leftmost position).
 __device__ __forceinline__ bool top_take_right(float a, float b) { return b < a || (a != a && b == b); }
 
+// The minimum of top-level entries [a, b) (b - a >= 2) from two table entries of row
+// floor(log2(b - a)): its value and the entry's payload (the level-0 index of the leftmost
+// minimum); on equal values the left entry holds the leftmost one.
 __device__ __forceinline__ void top_lookup(const uint2* __restrict__ tt, uint32_t stride, uint32_t a,
                                            uint32_t b, float& v, uint32_t& pos) {
   const uint32_t j = 31u - static_cast<uint32_t>(__clz(b - a));

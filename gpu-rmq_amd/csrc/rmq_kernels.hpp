@@ -141,10 +141,22 @@ __global__ __launch_bounds__(kBlock) void k_build_pass(HierView H, uint32_t k, u
 constexpr uint32_t kTopSmall = 8192;
 constexpr uint32_t kTopThreads = 1024;
 
+// The level-0 index of the minimum under entry i of a level whose blocks span 2^sk level-0
+// elements (its offset entries at lr + roff); entry i itself when the level is level 0 (lr null).
+__device__ __forceinline__ uint32_t level0_index(const uint8_t* __restrict__ lr, uint32_t roff, uint32_t i,
+                                                 uint32_t sk) {
+  return lr ? (i << sk) + dev::rel_load(lr, roff, i, sk) : i;
+}
+
+// Table entries are {value bits, level-0 index of the leftmost minimum}, so a query answered from
+// the table needs no offset entry afterwards (top: the level's values; lr / roff / sk: its
+// offset entries for level0_index).
 __device__ __forceinline__ void top_table_block(const float* __restrict__ top, uint32_t m, uint2* __restrict__ tt,
-                                                uint32_t stride, uint32_t rows, uint2* cur) {
+                                                uint32_t stride, uint32_t rows, uint2* cur,
+                                                const uint8_t* __restrict__ lr, uint32_t roff, uint32_t sk) {
   constexpr uint32_t kPer = kTopSmall / kTopThreads;
-  for (uint32_t i = threadIdx.x; i < m; i += kTopThreads) cur[i] = make_uint2(dev::f2u(top[i]), i);
+  for (uint32_t i = threadIdx.x; i < m; i += kTopThreads)
+    cur[i] = make_uint2(dev::f2u(top[i]), level0_index(lr, roff, i, sk));
   __syncthreads();
   for (uint32_t j = 1; j <= rows; ++j) {
     const uint32_t h = 1u << (j - 1u);
@@ -368,8 +380,10 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
   dev::Cand b;
   if constexpr (IDX) {
     b.v = bv;
-    b.key = bpos == dev::kNoIdx ? dev::kNoIdx : bpos << shK;
-    b.pay = bpos == dev::kNoIdx ? dev::kNoIdx : payK;
+    // a table answer is a level-0 index already (a key inside the entry's span, so candidates
+    // keep their order); a scanned one is a level-K position
+    b.key = bpos == dev::kNoIdx ? dev::kNoIdx : (table ? bpos : bpos << shK);
+    b.pay = bpos == dev::kNoIdx ? dev::kNoIdx : (table ? dev::kDirect : payK);
     dev::group_cand<G>(b);
     bv = b.v;
   } else {
@@ -650,9 +664,9 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
   float gL = RMQ_INF, gR = RMQ_INF;
   if (TOP && K == top && H.tt != nullptr && ek - lk > dev::kTopScan) {
     // a long final range on the top level: the sparse table's leftmost minimum
-    uint32_t pos;
-    dev::top_lookup(H.tt, H.tstride, lk, ek, bv, pos);
-    bpay = pb + (pos << shK);
+    uint32_t idx0;  // the level-0 index of the minimum: a direct payload
+    dev::top_lookup(H.tt, H.tstride, lk, ek, bv, idx0);
+    bpay = dev::kDirect | idx0;
     if (left_tail) gL = vK[lk - 1u];
     if (right_tail) gR = vK[ek];
   } else {
