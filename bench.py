@@ -150,7 +150,8 @@ def parse():
     ap.add_argument("--log-q", type=int, default=26)
     ap.add_argument("--lr", type=int, default=-2)
     ap.add_argument("--alg", default="", help="16, 16/g1, 16/g4, 16/g8, 16/g64, 24/g4, 24/g64, 20, 20/c0, 20/c1, "
-                                              "or 20/c0/p, 20/c1/p, 16/p (packed descents) (empty = sweep)")
+                                              "or 20/c0/p, 20/c1/p, 16/p (packed descents), 20/c0/2, 20/c1/2, "
+                                              "16/2 (two queries per lane) (empty = sweep)")
     ap.add_argument("--t", type=int, default=0, help="scan threshold (0 = sweep)")
     ap.add_argument("--mode", choices=["index", "value", "both"], default="index")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -390,7 +391,8 @@ def uniform_leg(dA, n, steps, warmup, peaks=None, ceilings=None):
     dQ = torch.from_numpy(lr.view(np.int32)).cuda()
     out_i = torch.empty(q, dtype=torch.int32, device="cuda")
     runs = []
-    for var, t in (("20/c0", 1 << 10), ("20/c0/p", 1 << 10), ("20/c1", 1 << 13), ("16", 1 << 15)):
+    for var, t in (("20/c0", 1 << 10), ("20/c0/p", 1 << 10), ("20/c0/2", 1 << 10), ("20/c1", 1 << 13),
+                   ("20/c1/2", 1 << 13), ("16", 1 << 15), ("16/2", 1 << 15)):
         alg = int(var.split("/")[0])
         with rmq_amd.Hierarchy(dA, t, alg, **OPTS_OF[var]) as h:
             torch.cuda.synchronize()
@@ -483,7 +485,8 @@ OPTS_OF = {"16": {}, "16/g4": dict(group_lanes=4), "24/g4": dict(group_lanes=4),
            "16/g64": dict(group_lanes=64), "24/g64": dict(group_lanes=64),
            "16/g1": dict(group_lanes=1), "16/g8": dict(group_lanes=8),
            "20/c0/p": dict(cg_amount_log=0, pack=True), "20/c1/p": dict(cg_amount_log=1, pack=True),
-           "16/p": dict(pack=True)}
+           "16/p": dict(pack=True), "20/c0/2": dict(cg_amount_log=0, paired=True),
+           "20/c1/2": dict(cg_amount_log=1, paired=True), "16/2": dict(paired=True)}
 
 
 def main():
@@ -546,6 +549,8 @@ def main():
         candidates += [(v, 1 << 18) for v in ("16/g64", "24/g64")]  # a wave per query, long scans
         # the lane solver with its descents packed onto full wavefronts
         candidates += [(v, 1 << tl) for v in ("20/c0/p", "20/c1/p", "16/p") for tl in (10, 13, 15)]
+        # two queries per lane
+        candidates += [(v, 1 << tl) for v in ("20/c0/2", "20/c1/2", "16/2") for tl in (10, 13, 15)]
     sweep = []
     best = None
     for var, t in candidates:

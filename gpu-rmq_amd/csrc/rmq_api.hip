@@ -310,7 +310,15 @@ rmq_status launch_cl_g(uint32_t G, bool full, int alg, const uint2* Q, uint32_t 
 
 template <int RL>
 rmq_status launch_vl(bool full, bool packed, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
-                     const HierView& H, hipStream_t s) {
+                     const HierView& H, hipStream_t s, bool paired = false) {
+  if (paired && !full) {  // two queries per lane
+    const uint64_t lanes = (static_cast<uint64_t>(q) + 1) / 2;
+    const uint32_t grid = grid_for(lanes);
+    if (static_cast<uint64_t>(grid) * kBlock < lanes) return RMQ_EINVAL;
+    if (oi) rmq::k_query_vl2<RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_vl2<RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    return last_error();
+  }
   const uint32_t grid = grid_for(q);
   if (static_cast<uint64_t>(grid) * kBlock < q) return RMQ_EINVAL;
   if (full) {
@@ -352,22 +360,24 @@ rmq_status query_dispatch(const rmq_hierarchy* h, const uint2* Q, uint32_t q, fl
                           uint32_t* oi, hipStream_t s) {
   const HierView& H = h->view;
   const bool packed = (h->flags & RMQ_BUILD_PACK_DESCENTS) != 0;
+  const bool paired = (h->flags & RMQ_BUILD_PAIRED) != 0;
   if (h->alg == RMQ_ALG_VECTOR_LOAD) {
+    const bool full = (h->flags & RMQ_BUILD_FULL) != 0;
     switch (h->R_log) {
-      case 2: return launch_vl<2>((h->flags & RMQ_BUILD_FULL) != 0, packed, Q, q, ov, oi, H, s);
-      case 3: return launch_vl<3>((h->flags & RMQ_BUILD_FULL) != 0, packed, Q, q, ov, oi, H, s);
-      case 4: return launch_vl<4>((h->flags & RMQ_BUILD_FULL) != 0, packed, Q, q, ov, oi, H, s);
-      case 5: return launch_vl<5>((h->flags & RMQ_BUILD_FULL) != 0, packed, Q, q, ov, oi, H, s);
-      case 6: return launch_vl<6>((h->flags & RMQ_BUILD_FULL) != 0, packed, Q, q, ov, oi, H, s);
+      case 2: return launch_vl<2>(full, packed, Q, q, ov, oi, H, s, paired);
+      case 3: return launch_vl<3>(full, packed, Q, q, ov, oi, H, s, paired);
+      case 4: return launch_vl<4>(full, packed, Q, q, ov, oi, H, s, paired);
+      case 5: return launch_vl<5>(full, packed, Q, q, ov, oi, H, s, paired);
+      case 6: return launch_vl<6>(full, packed, Q, q, ov, oi, H, s, paired);
       default: return RMQ_EUNSUPPORTED;
     }
   }
   if (h->group_lanes == 1) {  // algs 16 / 24 with a lane per query (R <= 64, pruned order)
     switch (h->R_log) {
-      case 3: return launch_vl<3>(false, packed, Q, q, ov, oi, H, s);
-      case 4: return launch_vl<4>(false, packed, Q, q, ov, oi, H, s);
-      case 5: return launch_vl<5>(false, packed, Q, q, ov, oi, H, s);
-      case 6: return launch_vl<6>(false, packed, Q, q, ov, oi, H, s);
+      case 3: return launch_vl<3>(false, packed, Q, q, ov, oi, H, s, paired);
+      case 4: return launch_vl<4>(false, packed, Q, q, ov, oi, H, s, paired);
+      case 5: return launch_vl<5>(false, packed, Q, q, ov, oi, H, s, paired);
+      case 6: return launch_vl<6>(false, packed, Q, q, ov, oi, H, s, paired);
       default: return RMQ_EUNSUPPORTED;
     }
   }

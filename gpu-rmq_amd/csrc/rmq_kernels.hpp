@@ -975,4 +975,106 @@ __global__ __launch_bounds__(kBlock) void k_query_vlc(const uint2* __restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Two queries per lane (RMQ_BUILD_PAIRED). A query answered from the top-level table is a
+// chain of two dependent reads (its pair, then the two table entries and its guards); a
+// wavefront at full occupancy waits on them. Here each lane takes two adjacent queries (one
+// 16-byte pair load) and issues both queries' table entries and guards before using either, so
+// twice the reads are in flight per wavefront. A query that is not answered from the table
+// (short final range, or below the top) takes the ordinary lane solver; descents and answers
+// follow per query.
+
+// The final range's level and bounds of [l, e) as vl_start finds them, and, when it lies on the
+// top level and is answered from the table, that answer's two entries and the guards, loaded
+// but not yet compared.
+struct VlTop {
+  uint32_t lk, ek, shK;
+  bool table, lt, rt;
+  uint2 x, y;
+  float gL, gR;
+};
+
+template <int RL, class LT>
+__device__ __forceinline__ VlTop vl_top_issue(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
+  VlTop r;
+  const uint32_t top = H.num_levels - 1;
+  uint32_t K = min((31u - static_cast<uint32_t>(__clz(e - l))) / RL, top);
+  uint32_t shK = RL * K;
+  uint32_t lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
+  uint32_t ek = e >> shK;
+  if (lk >= ek) {
+    --K;
+    shK -= RL;
+    lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
+    ek = e >> shK;
+  }
+  r.lk = lk;
+  r.ek = ek;
+  r.shK = shK;
+  r.table = K == top && H.tt != nullptr && ek - lk > dev::kTopScan;
+  r.lt = l < (lk << shK);
+  r.rt = (ek << shK) < e;
+  r.x = make_uint2(dev::f2u(RMQ_INF), 0u);
+  r.y = r.x;
+  r.gL = RMQ_INF;
+  r.gR = RMQ_INF;
+  if (r.table) {
+    const float* vK = K == 0 ? H.A : H.lv + T.offz(K);
+    const uint32_t j = 31u - static_cast<uint32_t>(__clz(ek - lk));
+    const uint2* row = H.tt + static_cast<size_t>(j - 1u) * H.tstride;
+    r.x = row[lk];
+    r.y = row[ek - (1u << j)];
+    if (r.lt) r.gL = vK[lk - 1u];
+    if (r.rt) r.gR = vK[ek];
+  }
+  return r;
+}
+
+// The lane solver's state after its final range, from vl_top_issue's loads (table case).
+__device__ __forceinline__ VlState vl_top_finish(const VlTop& r, uint32_t l, uint32_t e, uint32_t RL) {
+  const bool right = dev::u2f(r.y.x) < dev::u2f(r.x.x);
+  const float bv = dev::u2f(right ? r.y.x : r.x.x);
+  const uint32_t bpay = dev::kDirect | (right ? r.y.y : r.x.y);
+  const int K = static_cast<int>(r.shK / RL);
+  return VlState{l, e, bv, bpay, K - 1, r.lt && r.gL <= bv, r.rt && r.gR < bv};
+}
+
+template <int RL, bool IDX>
+__global__ __launch_bounds__(kBlock) void k_query_vl2(const uint2* __restrict__ Q, uint32_t q,
+                                                      float* __restrict__ out_v,
+                                                      uint32_t* __restrict__ out_i, HierView H) {
+  __shared__ uint32_t s_meta[kMetaWords];
+  if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
+  __syncthreads();
+  const LevelsLds T{s_meta};
+  const uint32_t zeros = *H.zeros;
+  const uint64_t q0 = 2 * (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x);
+  if (q0 >= q) return;
+  const bool two = q0 + 1 < q;
+  uint2 pa, pb;
+  if (two) {  // Q is 8-byte aligned; q0 is even, so the two pairs are one 16-byte load when Q is
+    const uint4 v = (reinterpret_cast<uintptr_t>(Q) & 15u) == 0
+                        ? *reinterpret_cast<const uint4*>(Q + q0)
+                        : make_uint4(Q[q0].x, Q[q0].y, Q[q0 + 1].x, Q[q0 + 1].y);
+    pa = make_uint2(v.x, v.y);
+    pb = make_uint2(v.z, v.w);
+  } else {
+    pa = Q[q0];
+    pb = pa;
+  }
+  const bool da = emit_nan_at_l<IDX>(zeros, H.A, pa.x, q0, out_v, out_i);
+  const bool db = !two || emit_nan_at_l<IDX>(zeros, H.A, pb.x, q0 + 1, out_v, out_i);
+  // both queries' table reads in flight before either is used
+  const VlTop ta = vl_top_issue<RL>(pa.x, pa.y + 1, H, T);
+  const VlTop tb = vl_top_issue<RL>(pb.x, pb.y + 1, H, T);
+  VlState sa = ta.table ? vl_top_finish(ta, pa.x, pa.y + 1, RL)
+                        : vl_start<RL, IDX, true, LevelsLds>(pa.x, pa.y + 1, H, T);
+  VlState sb = tb.table ? vl_top_finish(tb, pb.x, pb.y + 1, RL)
+                        : vl_start<RL, IDX, true, LevelsLds>(pb.x, pb.y + 1, H, T);
+  while (sa.la || sa.ra) vl_step<RL, IDX>(sa, H, T);
+  while (sb.la || sb.ra) vl_step<RL, IDX>(sb, H, T);
+  if (!da) vl_emit<RL, IDX>(sa, q0, H, T, zeros, out_v, out_i);
+  if (!db) vl_emit<RL, IDX>(sb, q0 + 1, H, T, zeros, out_v, out_i);
+}
+
 }  // namespace rmq

@@ -30,6 +30,7 @@ RMQ_BUILD_FULL = 0x2
 RMQ_BUILD_NO_TOP_TABLE = 0x4
 RMQ_BUILD_PACK_DESCENTS = 0x8
 RMQ_BUILD_CHECK_QUERIES = 0x10
+RMQ_BUILD_PAIRED = 0x20
 
 
 def RMQ_BUILD_CHUNK_LOG(k: int) -> int:

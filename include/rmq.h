@@ -82,7 +82,7 @@ typedef struct {
 #define RMQ_BUILD_FULL 0x2u         /* read every piece of the traversal in the reference's
                                        order instead of the pruned order (same answers) */
 /* A build also keeps a sparse table over the top level (m entries): floor(log2 m) rows of m
- * 8-byte {value, position} entries, built when m >= 64 and the table is at most 64 MiB, and not
+ * 8-byte {value, level-0 index} entries, built when m >= 64 and the table is at most 64 MiB, and not
  * with RMQ_BUILD_FULL. A query whose remaining range on the top level is longer than 8 entries
  * reads two table entries instead of scanning it, so large t no longer costs long scans (the
  * reference default t = 2^15 with uniform ranges). The flag below builds none (same answers). */
@@ -101,6 +101,11 @@ typedef struct {
  * stream and uses one device word of the hierarchy, so checked calls on one hierarchy must not
  * overlap (a call on another stream could see the other call's result). */
 #define RMQ_BUILD_CHECK_QUERIES 0x10u
+/* Lane-per-query solver with two adjacent queries per lane: both queries' top-level table reads
+ * are in flight together (queries answered from the table are a chain of two dependent reads).
+ * Same answers. Not with RMQ_BUILD_FULL or RMQ_BUILD_PACK_DESCENTS (which it overrides); the
+ * other solvers and rmq_query64 ignore it. */
+#define RMQ_BUILD_PAIRED 0x20u
 /* Chunk size (log2, 8..30) of the 64-bit form for this build; arrays of more than 2^31 - 1
  * elements use 2^30 without it. Setting it holds even a small array in chunks, which is how the
  * 64-bit form is tested on small inputs. */

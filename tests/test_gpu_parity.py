@@ -48,6 +48,10 @@ VARIANTS = [
     (16, dict(pack=True)),
     (24, dict(group_lanes=1, cg_size_log=3, cg_amount_log=3, pack=True)),
     (16, dict(group_lanes=4, pack=True)),
+    # two queries per lane (their top-level table reads in flight together)
+    (20, dict(cg_amount_log=0, paired=True)),
+    (20, dict(cg_amount_log=1, paired=True)),
+    (16, dict(paired=True)),
 ]
 
 
