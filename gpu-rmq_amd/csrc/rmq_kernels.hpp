@@ -710,25 +710,60 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
       if (right_tail && (ek & 3u) == 0) gR = gb;
       return VlState{l, e, bv, bpay, static_cast<int>(K) - 1, left_tail && gL <= bv, right_tail && gR < bv};
     }
-    float4 x = load_level(vK, p, szK, K, H.tail);
-    if (left_tail) gL = (lk & 3u) ? dev::pick4(x, (lk - 1u) & 3u) : vK[lk - 1u];
-    for (;;) {
-      if constexpr (IDX) {
-        const float xs[4] = {x.x, x.y, x.z, x.w};
+    if constexpr (RL >= 4) {
+      // R >= 16: a final range below the top spans up to 2R - 2 entries (16 vectors at R = 32);
+      // four vectors are loaded before any of them is folded, a quarter of the round trips
+      float4 last = make_float4(RMQ_INF, RMQ_INF, RMQ_INF, RMQ_INF);
+      for (bool first = true;; first = false) {
+        float4 xv[4];
 #pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-          const bool t = dev::in_range(p + j, lk, ek) && xs[j] < bv;
-          bv = t ? xs[j] : bv;
-          bpay = t ? pb + ((p + j) << shK) : bpay;
+        for (uint32_t v = 0; v < 4; ++v)
+          if (p + 4u * v < ek) xv[v] = load_level(vK, p + 4u * v, szK, K, H.tail);
+        if (first && left_tail) gL = (lk & 3u) ? dev::pick4(xv[0], (lk - 1u) & 3u) : vK[lk - 1u];
+#pragma unroll
+        for (uint32_t v = 0; v < 4; ++v) {
+          const uint32_t pv = p + 4u * v;
+          if (pv < ek) {
+            const float4 x = xv[v];
+            if constexpr (IDX) {
+              const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+              for (uint32_t j = 0; j < 4; ++j) {
+                const bool t = dev::in_range(pv + j, lk, ek) && xs[j] < bv;
+                bv = t ? xs[j] : bv;
+                bpay = t ? pb + ((pv + j) << shK) : bpay;
+              }
+            } else {
+              bv = dev::fold4(bv, x, pv, lk, ek);
+            }
+            last = x;
+          }
         }
-      } else {
-        bv = dev::fold4(bv, x, p, lk, ek);
+        p += 16;
+        if (p >= ek) break;
       }
-      p += 4;
-      if (p >= ek) break;
-      x = load_level(vK, p, szK, K, H.tail);
+      if (right_tail) gR = (ek & 3u) ? dev::pick4(last, ek & 3u) : vK[ek];
+    } else {
+      float4 x = load_level(vK, p, szK, K, H.tail);
+      if (left_tail) gL = (lk & 3u) ? dev::pick4(x, (lk - 1u) & 3u) : vK[lk - 1u];
+      for (;;) {
+        if constexpr (IDX) {
+          const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+          for (uint32_t j = 0; j < 4; ++j) {
+            const bool t = dev::in_range(p + j, lk, ek) && xs[j] < bv;
+            bv = t ? xs[j] : bv;
+            bpay = t ? pb + ((p + j) << shK) : bpay;
+          }
+        } else {
+          bv = dev::fold4(bv, x, p, lk, ek);
+        }
+        p += 4;
+        if (p >= ek) break;
+        x = load_level(vK, p, szK, K, H.tail);
+      }
+      if (right_tail) gR = (ek & 3u) ? dev::pick4(x, ek & 3u) : vK[ek];
     }
-    if (right_tail) gR = (ek & 3u) ? dev::pick4(x, ek & 3u) : vK[ek];
   }
   return VlState{l, e, bv, bpay, static_cast<int>(K) - 1, left_tail && gL <= bv, right_tail && gR < bv};
 }
@@ -779,21 +814,54 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
   // its payload formed once at the end of the step
   constexpr uint32_t kNoSlot = 2u * R;
   uint32_t wslot = kNoSlot;
+  // folds one vector of the left piece (slot s: right to left, `<=`) / of the right piece
+  // (left to right, strict `<`), noting the next guard where the vector holds it
+  auto fold_left = [&](const float4 x, uint32_t s, float& g) {
+    const uint32_t p = lbase + 4u * s;
+    if (p <= lo - 1u) g = dev::pick4(x, (lo - 1u) & 3u);  // next guard, lo - 1
+    const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (int j = 3; j >= 0; --j) {
+      const bool t = p + static_cast<uint32_t>(j) >= lo && xs[j] <= bv;
+      bv = t ? xs[j] : bv;
+      if constexpr (IDX) wslot = t ? 4u * s + static_cast<uint32_t>(j) : wslot;
+    }
+  };
+  auto fold_right = [&](const float4 x, uint32_t s, float& g) {
+    const uint32_t p = rlo + 4u * s;
+    if (hi - p < 4u) g = dev::pick4(x, hi & 3u);  // next guard, hi
+    const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      const bool t = p + j < hi && xs[j] < bv;
+      bv = t ? xs[j] : bv;
+      if constexpr (IDX) wslot = t ? R + 4u * s + j : wslot;
+    }
+  };
+  // R >= 16 (V >= 4): a piece's vectors are loaded four at a time before they are folded
+  constexpr bool kGroups = !kEarly && V % 4u == 0;
   if (la) {
     float g = RMQ_INF;
+    if constexpr (kGroups) {
 #pragma unroll
-    for (int s = static_cast<int>(V) - 1; s >= 0; --s) {
-      const uint32_t p = lbase + 4u * static_cast<uint32_t>(s);
-      if ((lo & (R - 1u)) != 0u && p + 4u > lo) {
-        const float4 x = kEarly ? xl[kEarly ? s : 0] : load_level(vk, p, szk, uk, H.tail);
-        if (p <= lo - 1u) g = dev::pick4(x, (lo - 1u) & 3u);  // next guard, lo - 1
-        const float xs[4] = {x.x, x.y, x.z, x.w};
+      for (int s0 = static_cast<int>(V) - 4; s0 >= 0; s0 -= 4) {
+        const uint32_t pg = lbase + 4u * static_cast<uint32_t>(s0);
+        if ((lo & (R - 1u)) != 0u && pg + 16u > lo) {
+          float4 xg[4];
 #pragma unroll
-        for (int j = 3; j >= 0; --j) {
-          const bool t = p + static_cast<uint32_t>(j) >= lo && xs[j] <= bv;
-          bv = t ? xs[j] : bv;
-          if constexpr (IDX) wslot = t ? 4u * static_cast<uint32_t>(s) + static_cast<uint32_t>(j) : wslot;
+          for (uint32_t v = 0; v < 4; ++v)
+            if (pg + 4u * v + 4u > lo) xg[v] = load_level(vk, pg + 4u * v, szk, uk, H.tail);
+#pragma unroll
+          for (int v = 3; v >= 0; --v)
+            if (pg + 4u * static_cast<uint32_t>(v) + 4u > lo) fold_left(xg[v], static_cast<uint32_t>(s0 + v), g);
         }
+      }
+    } else {
+#pragma unroll
+      for (int s = static_cast<int>(V) - 1; s >= 0; --s) {
+        const uint32_t p = lbase + 4u * static_cast<uint32_t>(s);
+        if ((lo & (R - 1u)) != 0u && p + 4u > lo)
+          fold_left(kEarly ? xl[kEarly ? s : 0] : load_level(vk, p, szk, uk, H.tail), static_cast<uint32_t>(s), g);
       }
     }
     if (!lnext) {
@@ -805,19 +873,25 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
   }
   if (ra) {
     float g = RMQ_INF;
+    if constexpr (kGroups) {
 #pragma unroll
-    for (uint32_t s = 0; s < V; ++s) {
-      const uint32_t p = rlo + 4u * s;
-      if (p < hi) {
-        const float4 x = kEarly ? xr[kEarly ? s : 0] : load_level(vk, p, szk, uk, H.tail);
-        if (hi - p < 4u) g = dev::pick4(x, hi & 3u);  // next guard, hi
-        const float xs[4] = {x.x, x.y, x.z, x.w};
+      for (uint32_t s0 = 0; s0 < V; s0 += 4) {
+        const uint32_t pg = rlo + 4u * s0;
+        if (pg < hi) {
+          float4 xg[4];
 #pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-          const bool t = p + j < hi && xs[j] < bv;
-          bv = t ? xs[j] : bv;
-          if constexpr (IDX) wslot = t ? R + 4u * s + j : wslot;
+          for (uint32_t v = 0; v < 4; ++v)
+            if (pg + 4u * v < hi) xg[v] = load_level(vk, pg + 4u * v, szk, uk, H.tail);
+#pragma unroll
+          for (uint32_t v = 0; v < 4; ++v)
+            if (pg + 4u * v < hi) fold_right(xg[v], s0 + v, g);
         }
+      }
+    } else {
+#pragma unroll
+      for (uint32_t s = 0; s < V; ++s) {
+        const uint32_t p = rlo + 4u * s;
+        if (p < hi) fold_right(kEarly ? xr[kEarly ? s : 0] : load_level(vk, p, szk, uk, H.tail), s, g);
       }
     }
     if (!rnext) {
