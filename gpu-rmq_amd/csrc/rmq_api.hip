@@ -259,6 +259,27 @@ rmq_status build_levels(uint32_t R_log, const HierView& H, hipStream_t s) {
   }
 }
 
+template <int RL>
+rmq_status launch_block_scans(const HierView& H, hipStream_t s) {
+  for (uint32_t k = 1; k < H.num_levels; ++k) {
+    const uint32_t blocks = (H.size[k] + (1u << RL) - 1u) >> RL;
+    rmq::k_block_scans<RL><<<(blocks + kBlock - 1) / kBlock, kBlock, 0, s>>>(H, k, const_cast<uint4*>(H.ps));
+  }
+  return last_error();
+}
+
+rmq_status build_block_scans(uint32_t R_log, const HierView& H, hipStream_t s) {
+  switch (R_log) {
+    case 2: return launch_block_scans<2>(H, s);
+    case 3: return launch_block_scans<3>(H, s);
+    case 4: return launch_block_scans<4>(H, s);
+    case 5: return launch_block_scans<5>(H, s);
+    case 6: return launch_block_scans<6>(H, s);
+    case 7: return launch_block_scans<7>(H, s);
+    default: return RMQ_EUNSUPPORTED;
+  }
+}
+
 // Byte offsets of the levels' offset entries (rel_bytes per level, 16-byte aligned, four entries
 // of padding each); returns the total bytes.
 uint64_t rel_layout(uint32_t R_log, uint32_t num_levels, const uint64_t* size, uint32_t* roff) {
@@ -308,30 +329,44 @@ rmq_status launch_cl_g(uint32_t G, bool full, int alg, const uint2* Q, uint32_t 
   }
 }
 
+// The pruned lane kernels, W0: n % 4 == 0, so level-0 vector loads need no tail select; PS: the
+// hierarchy has block scans.
+template <int RL, bool W0, bool PS>
+void launch_vl_pruned(bool packed, bool paired, uint32_t grid, const uint2* Q, uint32_t q, float* ov,
+                      uint32_t* oi, const HierView& H, hipStream_t s) {
+  if (paired) {
+    if (oi) rmq::k_query_vl2<RL, true, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_vl2<RL, false, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  } else if (packed) {
+    if (oi) rmq::k_query_vlc<RL, true, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_vlc<RL, false, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  } else {
+    if (oi) rmq::k_query_vl<RL, true, true, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_vl<RL, true, false, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  }
+}
+
+template <int RL, bool W0>
+void launch_vl_ps(bool packed, bool paired, uint32_t grid, const uint2* Q, uint32_t q, float* ov,
+                  uint32_t* oi, const HierView& H, hipStream_t s) {
+  if (H.ps) launch_vl_pruned<RL, W0, true>(packed, paired, grid, Q, q, ov, oi, H, s);
+  else launch_vl_pruned<RL, W0, false>(packed, paired, grid, Q, q, ov, oi, H, s);
+}
+
 template <int RL>
 rmq_status launch_vl(bool full, bool packed, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
                      const HierView& H, hipStream_t s, bool paired = false) {
-  if (paired && !full) {  // two queries per lane
-    const uint64_t lanes = (static_cast<uint64_t>(q) + 1) / 2;
-    const uint32_t grid = grid_for(lanes);
-    if (static_cast<uint64_t>(grid) * kBlock < lanes) return RMQ_EINVAL;
-    if (oi) rmq::k_query_vl2<RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-    else rmq::k_query_vl2<RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-    return last_error();
-  }
-  const uint32_t grid = grid_for(q);
-  if (static_cast<uint64_t>(grid) * kBlock < q) return RMQ_EINVAL;
+  paired = paired && !full;  // two queries per lane
+  const uint64_t lanes = paired ? (static_cast<uint64_t>(q) + 1) / 2 : q;
+  const uint32_t grid = grid_for(lanes);
+  if (static_cast<uint64_t>(grid) * kBlock < lanes) return RMQ_EINVAL;
   if (full) {
     if (oi) rmq::k_query_vl<RL, false, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
     else rmq::k_query_vl<RL, false, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  } else if ((H.size[0] & 3u) == 0) {
+    launch_vl_ps<RL, true>(packed, paired, grid, Q, q, ov, oi, H, s);
   } else {
-    if (packed) {
-      if (oi) rmq::k_query_vlc<RL, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-      else rmq::k_query_vlc<RL, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-    } else {
-      if (oi) rmq::k_query_vl<RL, true, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-      else rmq::k_query_vl<RL, true, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-    }
+    launch_vl_ps<RL, false>(packed, paired, grid, Q, q, ov, oi, H, s);
   }
   return last_error();
 }
@@ -468,6 +503,7 @@ rmq_status build_chunked(const float* d_array, uint64_t n, uint64_t t, int alg, 
   if (clog < 8 || clog > kChunkLog) return RMQ_EINVAL;
   rmq_build_options o = opt ? *opt : rmq_build_options{0, 0, 0, 0};
   o.flags &= ~(0xFFu << 8);
+  o.flags = (o.flags & ~RMQ_BUILD_BLOCK_SCANS) | RMQ_BUILD_NO_BLOCK_SCANS;  // rmq_query64 reads none
   rmq_hierarchy* h = new (std::nothrow) rmq_hierarchy();
   if (!h) return RMQ_ENOMEM;
   h->alg = alg;
@@ -598,6 +634,12 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   }
   const TopShape ts = top_shape(P, h->flags);
   const uint64_t at_tt = carve(ts.rows * ts.stride * sizeof(uint2));
+  // block scans, one 16-byte entry per level-buffer slot: read by the lane solver where R >= 16;
+  // built on request there, and by default where R >= 32 (tail pieces of up to R - 1 entries)
+  const bool lane = !(h->flags & RMQ_BUILD_FULL) && (alg == RMQ_ALG_VECTOR_LOAD || P.G == 1);
+  const bool scans = lane && P.num_levels > 1 && P.R_log >= 4 && !(h->flags & RMQ_BUILD_NO_BLOCK_SCANS) &&
+                     ((h->flags & RMQ_BUILD_BLOCK_SCANS) || P.R_log >= 5);
+  const uint64_t at_ps = carve(scans ? P.buffer_floats * sizeof(uint4) : 0);
   const uint64_t at_view = carve(sizeof(HierView));  // the view, on the device, for rmq_query64
   if (hipMalloc(reinterpret_cast<void**>(&h->mem), bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -611,6 +653,7 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
     h->lr = h->mem + at_lr;
   }
   if (ts.rows) h->tt = reinterpret_cast<uint2*>(h->mem + at_tt);
+  if (scans) h->view.ps = reinterpret_cast<const uint4*>(h->mem + at_ps);
   h->self_view = reinterpret_cast<HierView*>(h->mem + at_view);
   HierView& H = h->view;
   H.A = d_array;
@@ -640,6 +683,10 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   if (tst != RMQ_OK) {
     rmq_free(h);
     return tst;
+  }
+  if (scans && build_block_scans(P.R_log, H, s) != RMQ_OK) {
+    rmq_free(h);
+    return RMQ_EDEVICE;
   }
   // the finished view (h lives until rmq_free, so the copy's source outlives it)
   if (hipMemcpyAsync(h->self_view, &h->view, sizeof(HierView), hipMemcpyHostToDevice, s) != hipSuccess) {

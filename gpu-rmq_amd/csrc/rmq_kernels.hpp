@@ -184,6 +184,41 @@ __device__ __forceinline__ void top_table_block(const float* __restrict__ top, u
   }
 }
 
+// Block scans of level k >= 1 (RMQ_BUILD_BLOCK_SCANS): thread b takes R-block b of level k,
+// entries [bR, min(bR + R, size_k)), and writes for each entry i the leftmost minimum of
+// [i, block end) (right to left, a left entry wins ties) and of [block start, i] (left to
+// right, strict <), each as {value bits, level-0 index}. Upper levels hold no NaN.
+template <int RL>
+__global__ __launch_bounds__(kBlock) void k_block_scans(HierView H, uint32_t k, uint4* __restrict__ ps) {
+  constexpr uint32_t R = 1u << RL;
+  const uint32_t size = H.size[k];
+  const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= (size + R - 1u) >> RL) return;
+  const uint32_t base = b << RL;
+  const uint32_t cnt = min(R, size - base);
+  const float* __restrict__ v = H.lv + H.off[k];
+  const uint32_t sk = RL * k;
+  uint2* __restrict__ out = reinterpret_cast<uint2*>(ps + H.off[k] + base);
+  float bv = RMQ_INF;
+  uint32_t bi = 0;
+  for (uint32_t j = 0; j < cnt; ++j) {
+    const float x = v[base + j];
+    if (j == 0 || x < bv) {
+      bv = x;
+      bi = level0_index(H.lr, H.roff[k], base + j, sk);
+    }
+    out[2u * j + 1u] = make_uint2(dev::f2u(bv), bi);
+  }
+  for (uint32_t j = cnt; j-- > 0;) {
+    const float x = v[base + j];
+    if (j == cnt - 1u || x <= bv) {
+      bv = x;
+      bi = level0_index(H.lr, H.roff[k], base + j, sk);
+    }
+    out[2u * j] = make_uint2(dev::f2u(bv), bi);
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Query traversal shared by algs 16 and 24 (src/xxx.h:158-187 and 493-521): r exclusive;
 // while the remaining range at this level is longer than 2R, the partial blocks
@@ -218,19 +253,28 @@ __device__ __forceinline__ const float* level_values(const HierView& H, uint32_t
 // HierView itself, or in a copy the kernel placed in LDS.
 // offz(k) is the offset with level 0 read as 0 (it lives in the caller's array).
 struct LevelsOf {
+  static constexpr bool kWhole0 = false;
+  static constexpr bool kBlockScans = false;
   const HierView& H;
   __device__ __forceinline__ uint32_t off(uint32_t k) const { return H.off[k]; }
   __device__ __forceinline__ uint32_t offz(uint32_t k) const { return k == 0 ? 0u : H.off[k]; }
   __device__ __forceinline__ uint32_t roff(uint32_t k) const { return H.roff[k]; }
   __device__ __forceinline__ uint32_t size(uint32_t k) const { return H.size[k]; }
 };
-struct LevelsLds {
+// W0: level 0 (the caller's array) holds whole 16-byte vectors (n % 4 == 0), so its loads need
+// no tail select.
+// PS: the hierarchy has block scans (HierView::ps), which the lane solver reads.
+template <bool W0, bool PS = false>
+struct LevelsLdsT {
+  static constexpr bool kWhole0 = W0;
+  static constexpr bool kBlockScans = PS;
   const uint32_t* meta;  // LDS: off (with off[0] = 0), size, roff; kMaxLevels each (lds_meta)
   __device__ __forceinline__ uint32_t off(uint32_t k) const { return meta[k]; }
   __device__ __forceinline__ uint32_t offz(uint32_t k) const { return meta[k]; }
   __device__ __forceinline__ uint32_t size(uint32_t k) const { return meta[kMaxLevels + k]; }
   __device__ __forceinline__ uint32_t roff(uint32_t k) const { return meta[2 * kMaxLevels + k]; }
 };
+using LevelsLds = LevelsLdsT<false>;
 constexpr uint32_t kMetaWords = 3 * kMaxLevels;
 
 // Entry i (< kMetaWords) of the LDS copy that LevelsLds reads.
@@ -596,9 +640,12 @@ __device__ __forceinline__ GroupAnswer solve_vl(uint32_t l, uint32_t e, const Hi
 // A 16-byte vector of level k at p (p % 4 == 0, p < size), with no branch: upper levels sit in
 // the level buffer, padded to 32 entries past each level's end, so their vectors are whole; on
 // level 0 (the caller's array of exactly n floats) the one vector that would run past n, at
-// p = n & ~3, is read from the hierarchy's copy of it, padded with +inf.
+// p = n & ~3, is read from the hierarchy's copy of it, padded with +inf. Where the level table
+// type says that n % 4 == 0 (LT::kWhole0), there is no such vector and no select.
+template <class LT>
 __device__ __forceinline__ float4 load_level(const float* __restrict__ v, uint32_t p, uint32_t size, uint32_t k,
                                              const float* __restrict__ tail) {
+  if constexpr (LT::kWhole0) return *reinterpret_cast<const float4*>(v + p);
   const float* src = (k != 0 || p + 4u <= size) ? v + p : tail;
   return *reinterpret_cast<const float4*>(src);
 }
@@ -630,6 +677,11 @@ __device__ __forceinline__ uint32_t pay_index(uint32_t pay, const HierView& H, c
   const uint32_t s0 = pay & ~((1u << sk) - 1u);
   return s0 + dev::rel_load(H.lr, T.roff(sk / RL), s0 >> sk, sk);
 }
+
+// Block scans answer a final range only when it is longer than this (shorter ones are one round
+// of four vector loads), and tail pieces only where R >= 16 (a piece of R <= 8 is at most two
+// vectors, whose loads also hold the next guard).
+constexpr uint32_t kScanFinalMin = 16;
 
 struct VlState {
   uint32_t l, e;
@@ -669,6 +721,19 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
     bpay = dev::kDirect | idx0;
     if (left_tail) gL = vK[lk - 1u];
     if (right_tail) gR = vK[ek];
+  } else if (LT::kBlockScans && RL >= 4 && K != 0 && ek - lk > kScanFinalMin &&
+             ((ek - 1u) >> RL) == (lk >> RL) + 1u) {
+    // a final range of more than one round of four vectors that is the rest of one R-block and
+    // the start of the next: the suffix minimum from lk and the prefix minimum up to ek - 1,
+    // with their level-0 indices
+    const uint2* ps = reinterpret_cast<const uint2*>(H.ps + T.offz(K));
+    const uint2 x = ps[2u * lk];
+    const uint2 y = ps[2u * (ek - 1u) + 1u];
+    if (left_tail) gL = vK[lk - 1u];
+    if (right_tail) gR = vK[ek];
+    const bool right = dev::u2f(y.x) < dev::u2f(x.x);
+    bv = dev::u2f(right ? y.x : x.x);
+    bpay = dev::kDirect | (right ? y.y : x.y);
   } else {
     // guards lk - 1 and ek: in the first / last vector the scan reads unless 4-aligned
     uint32_t p = lk & ~3u;
@@ -684,7 +749,7 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
       float4 xv[NV];
 #pragma unroll
       for (uint32_t v = 0; v < NV; ++v)
-        if (p + 4u * v < ek) xv[v] = load_level(vK, p + 4u * v, szK, K, H.tail);
+        if (p + 4u * v < ek) xv[v] = load_level<LT>(vK, p + 4u * v, szK, K, H.tail);
       if (left_tail) gL = (lk & 3u) ? dev::pick4(xv[0], (lk - 1u) & 3u) : ga;
       // index mode: the winner's offset from p (a constant per element), its payload once after
       uint32_t bo = lk - p;
@@ -718,7 +783,7 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
         float4 xv[4];
 #pragma unroll
         for (uint32_t v = 0; v < 4; ++v)
-          if (p + 4u * v < ek) xv[v] = load_level(vK, p + 4u * v, szK, K, H.tail);
+          if (p + 4u * v < ek) xv[v] = load_level<LT>(vK, p + 4u * v, szK, K, H.tail);
         if (first && left_tail) gL = (lk & 3u) ? dev::pick4(xv[0], (lk - 1u) & 3u) : vK[lk - 1u];
 #pragma unroll
         for (uint32_t v = 0; v < 4; ++v) {
@@ -744,7 +809,7 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
       }
       if (right_tail) gR = (ek & 3u) ? dev::pick4(last, ek & 3u) : vK[ek];
     } else {
-      float4 x = load_level(vK, p, szK, K, H.tail);
+      float4 x = load_level<LT>(vK, p, szK, K, H.tail);
       if (left_tail) gL = (lk & 3u) ? dev::pick4(x, (lk - 1u) & 3u) : vK[lk - 1u];
       for (;;) {
         if constexpr (IDX) {
@@ -760,7 +825,7 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
         }
         p += 4;
         if (p >= ek) break;
-        x = load_level(vK, p, szK, K, H.tail);
+        x = load_level<LT>(vK, p, szK, K, H.tail);
       }
       if (right_tail) gR = (ek & 3u) ? dev::pick4(x, ek & 3u) : vK[ek];
     }
@@ -800,14 +865,42 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
   // with them rather than after the fold: same entries, same conditions, one round trip less
   const bool lnext = la && k != 0 && l < (lo << sk);
   const bool rnext = ra && k != 0 && (hi << sk) < e;
+  if constexpr (LT::kBlockScans && RL >= 4) {
+    if (k != 0) {
+      // each piece is one block-scan entry: the left piece [lo, block end) a suffix minimum, the
+      // right piece [block start, hi) a prefix minimum; the next guards are loaded with them
+      const uint2* ps = reinterpret_cast<const uint2*>(H.ps + ok);
+      const bool lp = la && (lo & (R - 1u)) != 0u;
+      const bool rp = ra && (hi & (R - 1u)) != 0u;
+      const uint2 none = make_uint2(dev::f2u(RMQ_INF), 0u);
+      const uint2 x = lp ? ps[2u * lo] : none;
+      const uint2 y = rp ? ps[2u * (hi - 1u) + 1u] : none;
+      const float gl = lnext ? vk[lo - 1u] : RMQ_INF;
+      const float gr = rnext ? vk[hi] : RMQ_INF;
+      if (lp && dev::u2f(x.x) <= bv) {
+        bv = dev::u2f(x.x);
+        bpay = dev::kDirect | x.y;
+      }
+      if (rp && dev::u2f(y.x) < bv) {
+        bv = dev::u2f(y.x);
+        bpay = dev::kDirect | y.y;
+      }
+      st.bv = bv;
+      st.bpay = bpay;
+      st.la = lnext && !(gl > bv);
+      st.ra = rnext && gr < bv;
+      st.k = k - 1;
+      return;
+    }
+  }
   const float gl0 = lnext && (lo & 3u) == 0 ? vk[lo - 1u] : RMQ_INF;
   const float gr0 = rnext && (hi & 3u) == 0 ? vk[hi] : RMQ_INF;
   if constexpr (kEarly) {
 #pragma unroll
     for (uint32_t s = 0; s < V; ++s) {
       const uint32_t pl = lbase + 4u * s, pr = rlo + 4u * s;
-      if (la && (lo & (R - 1u)) != 0u && pl + 4u > lo) xl[s] = load_level(vk, pl, szk, uk, H.tail);
-      if (ra && pr < hi) xr[s] = load_level(vk, pr, szk, uk, H.tail);
+      if (la && (lo & (R - 1u)) != 0u && pl + 4u > lo) xl[s] = load_level<LT>(vk, pl, szk, uk, H.tail);
+      if (ra && pr < hi) xr[s] = load_level<LT>(vk, pr, szk, uk, H.tail);
     }
   }
   // index mode: the winning element's slot (a constant per element; R + slot on the right),
@@ -850,7 +943,7 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
           float4 xg[4];
 #pragma unroll
           for (uint32_t v = 0; v < 4; ++v)
-            if (pg + 4u * v + 4u > lo) xg[v] = load_level(vk, pg + 4u * v, szk, uk, H.tail);
+            if (pg + 4u * v + 4u > lo) xg[v] = load_level<LT>(vk, pg + 4u * v, szk, uk, H.tail);
 #pragma unroll
           for (int v = 3; v >= 0; --v)
             if (pg + 4u * static_cast<uint32_t>(v) + 4u > lo) fold_left(xg[v], static_cast<uint32_t>(s0 + v), g);
@@ -861,7 +954,7 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
       for (int s = static_cast<int>(V) - 1; s >= 0; --s) {
         const uint32_t p = lbase + 4u * static_cast<uint32_t>(s);
         if ((lo & (R - 1u)) != 0u && p + 4u > lo)
-          fold_left(kEarly ? xl[kEarly ? s : 0] : load_level(vk, p, szk, uk, H.tail), static_cast<uint32_t>(s), g);
+          fold_left(kEarly ? xl[kEarly ? s : 0] : load_level<LT>(vk, p, szk, uk, H.tail), static_cast<uint32_t>(s), g);
       }
     }
     if (!lnext) {
@@ -881,7 +974,7 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
           float4 xg[4];
 #pragma unroll
           for (uint32_t v = 0; v < 4; ++v)
-            if (pg + 4u * v < hi) xg[v] = load_level(vk, pg + 4u * v, szk, uk, H.tail);
+            if (pg + 4u * v < hi) xg[v] = load_level<LT>(vk, pg + 4u * v, szk, uk, H.tail);
 #pragma unroll
           for (uint32_t v = 0; v < 4; ++v)
             if (pg + 4u * v < hi) fold_right(xg[v], s0 + v, g);
@@ -891,7 +984,7 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
 #pragma unroll
       for (uint32_t s = 0; s < V; ++s) {
         const uint32_t p = rlo + 4u * s;
-        if (p < hi) fold_right(kEarly ? xr[kEarly ? s : 0] : load_level(vk, p, szk, uk, H.tail), s, g);
+        if (p < hi) fold_right(kEarly ? xr[kEarly ? s : 0] : load_level<LT>(vk, p, szk, uk, H.tail), s, g);
       }
     }
     if (!rnext) {
@@ -942,7 +1035,7 @@ __device__ __forceinline__ bool emit_nan_at_l(uint32_t flags, const float* __res
   return true;
 }
 
-template <int RL, bool P, bool IDX>
+template <int RL, bool P, bool IDX, bool W0 = false, bool PS = false>
 __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q, uint32_t q,
                                                      float* __restrict__ out_v,
                                                      uint32_t* __restrict__ out_i, HierView H) {
@@ -950,7 +1043,7 @@ __global__ __launch_bounds__(kBlock) void k_query_vl(const uint2* __restrict__ Q
   __shared__ uint32_t s_meta[kMetaWords];
   if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
-  const LevelsLds T{s_meta};
+  const LevelsLdsT<W0, PS> T{s_meta};
   const uint32_t zeros = *H.zeros;
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (qi >= q) return;
@@ -987,7 +1080,7 @@ __device__ __forceinline__ void vl_emit(const VlState& st, uint64_t qi, const Hi
 // as soon as it is done. Where few queries descend (uniform lengths) or most descend only a
 // little (short lengths), the parking, the barrier and the parked queries' scattered stores
 // are not repaid: a few percent slower there.
-template <int RL, bool IDX>
+template <int RL, bool IDX, bool W0 = false, bool PS = false>
 __global__ __launch_bounds__(kBlock) void k_query_vlc(const uint2* __restrict__ Q, uint32_t q,
                                                       float* __restrict__ out_v,
                                                       uint32_t* __restrict__ out_i, HierView H) {
@@ -998,7 +1091,7 @@ __global__ __launch_bounds__(kBlock) void k_query_vlc(const uint2* __restrict__ 
   __shared__ uint32_t s_cnt[kWaves];
   if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
-  const LevelsLds T{s_meta};
+  const LevelsLdsT<W0, PS> T{s_meta};
   const uint32_t zeros = *H.zeros;
   const uint32_t w = threadIdx.x >> 6;
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -1113,14 +1206,14 @@ __device__ __forceinline__ VlState vl_top_finish(const VlTop& r, uint32_t l, uin
   return VlState{l, e, bv, bpay, K - 1, r.lt && r.gL <= bv, r.rt && r.gR < bv};
 }
 
-template <int RL, bool IDX>
+template <int RL, bool IDX, bool W0 = false, bool PS = false>
 __global__ __launch_bounds__(kBlock) void k_query_vl2(const uint2* __restrict__ Q, uint32_t q,
                                                       float* __restrict__ out_v,
                                                       uint32_t* __restrict__ out_i, HierView H) {
   __shared__ uint32_t s_meta[kMetaWords];
   if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
-  const LevelsLds T{s_meta};
+  const LevelsLdsT<W0, PS> T{s_meta};
   const uint32_t zeros = *H.zeros;
   const uint64_t q0 = 2 * (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x);
   if (q0 >= q) return;
@@ -1142,9 +1235,9 @@ __global__ __launch_bounds__(kBlock) void k_query_vl2(const uint2* __restrict__ 
   const VlTop ta = vl_top_issue<RL>(pa.x, pa.y + 1, H, T);
   const VlTop tb = vl_top_issue<RL>(pb.x, pb.y + 1, H, T);
   VlState sa = ta.table ? vl_top_finish(ta, pa.x, pa.y + 1, RL)
-                        : vl_start<RL, IDX, true, LevelsLds>(pa.x, pa.y + 1, H, T);
+                        : vl_start<RL, IDX, true, LevelsLdsT<W0, PS>>(pa.x, pa.y + 1, H, T);
   VlState sb = tb.table ? vl_top_finish(tb, pb.x, pb.y + 1, RL)
-                        : vl_start<RL, IDX, true, LevelsLds>(pb.x, pb.y + 1, H, T);
+                        : vl_start<RL, IDX, true, LevelsLdsT<W0, PS>>(pb.x, pb.y + 1, H, T);
   while (sa.la || sa.ra) vl_step<RL, IDX>(sa, H, T);
   while (sb.la || sb.ra) vl_step<RL, IDX>(sb, H, T);
   if (!da) vl_emit<RL, IDX>(sa, q0, H, T, zeros, out_v, out_i);

@@ -52,8 +52,8 @@ inline void variant_build(uint32_t variant, int* alg, rmq_build_options* o) {
     *alg = RMQ_ALG_VECTOR_LOAD;  // R = 4 * 2^1 or 4 * 2^0
     *o = rmq_build_options{0, variant == RMQ_VARIANT_LANE8 ? 1u : 0u, 0, RMQ_BUILD_DEFAULT_CG};
   } else {
-    *alg = RMQ_ALG_XXX;          // R = 2^(3 + 2)
-    *o = rmq_build_options{3, 2, variant == RMQ_VARIANT_LANE ? 0u : variant, 0};
+    *alg = RMQ_ALG_XXX;          // R = 2^(3 + 2); the multi-array kernels read no block scans
+    *o = rmq_build_options{3, 2, variant == RMQ_VARIANT_LANE ? 0u : variant, RMQ_BUILD_NO_BLOCK_SCANS};
   }
 }
 constexpr uint32_t kQueues = 8;          // XCDs of an MI355X

@@ -55,9 +55,9 @@ class Plan:
 
 
 def _options(cg_size_log, cg_amount_log, group_lanes, full=False, chunk_log=0,
-             top_table=True, pack=False, check=False, paired=False) -> Optional[BuildOptions]:
+             top_table=True, pack=False, check=False, paired=False, scans=None) -> Optional[BuildOptions]:
     if (cg_size_log is None and cg_amount_log is None and not group_lanes and not full and not chunk_log
-            and top_table and not pack and not check and not paired):
+            and top_table and not pack and not check and not paired and scans is None):
         return None
     o = BuildOptions()
     o.cg_size_log = 3 if cg_size_log is None else int(cg_size_log)
@@ -67,6 +67,8 @@ def _options(cg_size_log, cg_amount_log, group_lanes, full=False, chunk_log=0,
                (0 if top_table else _lib.RMQ_BUILD_NO_TOP_TABLE) |
                (_lib.RMQ_BUILD_PACK_DESCENTS if pack else 0) |
                (_lib.RMQ_BUILD_CHECK_QUERIES if check else 0) | (_lib.RMQ_BUILD_PAIRED if paired else 0) |
+               (_lib.RMQ_BUILD_BLOCK_SCANS if scans is True else 0) |
+               (_lib.RMQ_BUILD_NO_BLOCK_SCANS if scans is False else 0) |
                _lib.RMQ_BUILD_CHUNK_LOG(chunk_log))
     return o
 
@@ -158,13 +160,15 @@ class Hierarchy:
     reference's traversal order), chunk_log (the 64-bit form's chunk size), top_table=False (no
     sparse table over the top level), pack=True (RMQ_BUILD_PACK_DESCENTS), check_queries=True
     (RMQ_BUILD_CHECK_QUERIES: out-of-range queries raise RMQError instead of being answered),
-    paired=True (RMQ_BUILD_PAIRED: two queries per lane).
+    paired=True (RMQ_BUILD_PAIRED: two queries per lane), scans=True / False (RMQ_BUILD_BLOCK_SCANS /
+    RMQ_BUILD_NO_BLOCK_SCANS: block prefix / suffix minima for the lane solver; None = the
+    library's default, built where R >= 32).
     """
 
     def __init__(self, array, t: int, alg: int = RMQ_ALG_XXX, cg_size_log=None,
                  cg_amount_log=None, group_lanes: int = 0, full: bool = False, chunk_log: int = 0,
                  top_table: bool = True, pack: bool = False, check_queries: bool = False, paired: bool = False,
-                 stream=None):
+                 scans: Optional[bool] = None, stream=None):
         torch = _torch()
         _check_device_tensor(array, "array", (torch.float32,))
         if array.dim() != 1:
@@ -175,7 +179,7 @@ class Hierarchy:
         self.alg = int(alg)
         self._h = ctypes.c_void_p()
         o = _options(cg_size_log, cg_amount_log, group_lanes, full, chunk_log, top_table, pack, check_queries,
-                     paired)
+                     paired, scans)
         st = _lib.lib().rmq_build_ex(array.data_ptr(), self.n, self.t, self.alg, _opt_ptr(o),
                                      _stream_handle(stream), ctypes.byref(self._h))
         _lib.check(st, "rmq_build")

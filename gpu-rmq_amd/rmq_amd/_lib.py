@@ -31,6 +31,8 @@ RMQ_BUILD_NO_TOP_TABLE = 0x4
 RMQ_BUILD_PACK_DESCENTS = 0x8
 RMQ_BUILD_CHECK_QUERIES = 0x10
 RMQ_BUILD_PAIRED = 0x20
+RMQ_BUILD_BLOCK_SCANS = 0x40
+RMQ_BUILD_NO_BLOCK_SCANS = 0x80
 
 
 def RMQ_BUILD_CHUNK_LOG(k: int) -> int:

@@ -106,6 +106,18 @@ typedef struct {
  * Same answers. Not with RMQ_BUILD_FULL or RMQ_BUILD_PACK_DESCENTS (which it overrides); the
  * other solvers and rmq_query64 ignore it. */
 #define RMQ_BUILD_PAIRED 0x20u
+/* Block scans: the build also keeps, for every entry i of levels 1.., the leftmost minimum of
+ * the rest of its R-block [i, block end) and of the block up to it [block start, i], each with
+ * its level-0 index (16 bytes per upper entry: 0.52 n bytes at R = 32). Where R >= 16 the
+ * lane-per-query solver (alg 20; algs 16/24 with group_lanes 1) then reads a tail piece above
+ * level 0 as one entry, and a final range of more than 16 entries that spans two blocks as
+ * two, instead of scanning up to R - 1 (2R - 2) entries. Same answers. Built by default for the
+ * lane solver at R >= 32 (the algs 16/24 default); RMQ_BUILD_BLOCK_SCANS builds them at R = 16
+ * too (measured at n = q = 2^26 there: lognormal-medium and uniform lengths faster, short and
+ * mixed lengths slower) and RMQ_BUILD_NO_BLOCK_SCANS builds none. Ignored at R <= 8 and by the
+ * group solvers and RMQ_BUILD_FULL. */
+#define RMQ_BUILD_BLOCK_SCANS 0x40u
+#define RMQ_BUILD_NO_BLOCK_SCANS 0x80u
 /* Chunk size (log2, 8..30) of the 64-bit form for this build; arrays of more than 2^31 - 1
  * elements use 2^30 without it. Setting it holds even a small array in chunks, which is how the
  * 64-bit form is tested on small inputs. */

@@ -52,6 +52,16 @@ VARIANTS = [
     (20, dict(cg_amount_log=0, paired=True)),
     (20, dict(cg_amount_log=1, paired=True)),
     (16, dict(paired=True)),
+    # block prefix / suffix minima, read by the lane solver where R >= 16 (built by default at
+    # R >= 32, so the defaults above read them) and ignored by the group solvers
+    (20, dict(cg_amount_log=2, scans=True)),
+    (20, dict(cg_amount_log=2, scans=True, pack=True)),
+    (20, dict(cg_amount_log=2, scans=True, paired=True)),
+    (16, dict(scans=False)),
+    (16, dict(paired=True, scans=False)),
+    (20, dict(cg_amount_log=3, scans=False, pack=True)),
+    (24, dict(group_lanes=1, cg_size_log=3, cg_amount_log=3, scans=False)),
+    (16, dict(group_lanes=4, scans=True)),
 ]
 
 
@@ -258,7 +268,8 @@ def test_config2_index_sampled_full_n(device, t_log):
     A = _array_2_26()
     lr = _queries_2_26()[: 1 << 22]
     ref_v, ref_i = oracle.CpuHierarchy(A, 1 << 15, 5, 16).query(lr)
-    for alg, opts in ((16, {}), (24, {}), (20, {}), (20, dict(cg_amount_log=0, pack=True))):
+    for alg, opts in ((16, {}), (24, {}), (20, {}), (20, dict(cg_amount_log=0, pack=True)),
+                      (20, dict(cg_amount_log=2, scans=True)), (16, dict(paired=True)), (16, dict(scans=False))):
         _say(f"config2 alg={alg} opts={opts} t=2^{t_log}")
         v, i = run_gpu(A, lr, alg, 1 << t_log, True, True, **opts)
         assert_exact(A, lr, v, i, ref_i, f"config2 alg={alg} t=2^{t_log}")
