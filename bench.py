@@ -151,7 +151,8 @@ def parse():
     ap.add_argument("--lr", type=int, default=-2)
     ap.add_argument("--alg", default="", help="16, 16/g1, 16/g4, 16/g8, 16/g64, 24/g4, 24/g64, 20, 20/c0, 20/c1, "
                                               "or 20/c0/p, 20/c1/p, 16/p (packed descents), 20/c0/2, 20/c1/2, "
-                                              "16/2 (two queries per lane) (empty = sweep)")
+                                              "16/2 (two queries per lane), 16/n (no block scans), 20/s (block scans at R = 16) "
+                                              "(empty = sweep)")
     ap.add_argument("--t", type=int, default=0, help="scan threshold (0 = sweep)")
     ap.add_argument("--mode", choices=["index", "value", "both"], default="index")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -410,12 +411,12 @@ def uniform_leg(dA, n, steps, warmup, peaks=None, ceilings=None):
     _, bi = rmq_amd.gpu_base(dA, sq, False, True)
     sel = torch.from_numpy(sidx.astype(np.int64)).cuda()
     assert torch.equal(out_i[sel], bi), "uniform-length output differs from GPU BASE"
-    a16 = [r for r in runs if r["alg"] == "16"][0]
+    a16 = max((r for r in runs if r["alg"].split("/")[0] == "16"), key=lambda r: r["gqs"])
     ln, lq = n.bit_length() - 1, q.bit_length() - 1
     key = f"alg={best['alg']} t={best['t']} n=2^{ln} q=2^{lq} lr=-1 index"
     return {"workload": f"n=2^{ln}, q=2^{lq}, lr=-1 (uniform lengths), index mode",
             "gqs": best["gqs"], "alg": best["alg"], "t": best["t"], "ms": best["ms"],
-            "alg16_t2e15_gqs": a16["gqs"], "runs": runs,
+            "alg16_t2e15_gqs": a16["gqs"], "alg16_t2e15_form": a16["alg"], "runs": runs,
             "roofline": roofline_block(key, roofline.total_bytes(lr, 1), q, best["ms"], peaks, ceilings),
             "spot_check": f"{sidx.size} sampled queries shorter than 2^22 equal GPU BASE"}
 
@@ -486,7 +487,9 @@ OPTS_OF = {"16": {}, "16/g4": dict(group_lanes=4), "24/g4": dict(group_lanes=4),
            "16/g1": dict(group_lanes=1), "16/g8": dict(group_lanes=8),
            "20/c0/p": dict(cg_amount_log=0, pack=True), "20/c1/p": dict(cg_amount_log=1, pack=True),
            "16/p": dict(pack=True), "20/c0/2": dict(cg_amount_log=0, paired=True),
-           "20/c1/2": dict(cg_amount_log=1, paired=True), "16/2": dict(paired=True)}
+           "20/c1/2": dict(cg_amount_log=1, paired=True), "16/2": dict(paired=True),
+           # block scans: built by default at R = 32 ("16/n" without them), on request at R = 16
+           "16/n": dict(scans=False), "20/s": dict(scans=True)}
 
 
 def main():
@@ -551,6 +554,8 @@ def main():
         candidates += [(v, 1 << tl) for v in ("20/c0/p", "20/c1/p", "16/p") for tl in (10, 13, 15)]
         # two queries per lane
         candidates += [(v, 1 << tl) for v in ("20/c0/2", "20/c1/2", "16/2") for tl in (10, 13, 15)]
+        # alg 16 at the reference default without its block scans; alg 20 (R = 16) with them
+        candidates += [("16/n", 1 << 15), ("20/s", 1 << 10), ("20/s", 1 << 13)]
     sweep = []
     best = None
     for var, t in candidates:

@@ -146,6 +146,10 @@ constexpr uint64_t kMaxN = 0x7FFFFFFFull;  // int32 n, as the reference's CmdArg
 constexpr uint64_t kMaxN64 = 1ull << 38;    // the 64-bit form (nType = uint64_t, src/tools.h:27-29)
 constexpr uint32_t kChunkLog = 30;          // chunk size of the 64-bit form
 constexpr uint64_t kMaxGrid = 1ull << 20;
+// Most bytes of block scans a hierarchy keeps (RMQ_BUILD_BLOCK_SCANS), from the top level down:
+// at n = 2^30, R = 32 the scans of level 1 (512 MB, cold) were slower than its vectors, those of
+// levels 2.. faster; at n = 2^26 all of them (33 MB) pay.
+constexpr uint64_t kScanBytes = 64ull << 20;
 // Queries one launch of a one-lane-per-query kernel answers (kMaxGrid workgroups of kBlock
 // lanes); rmq_query and rmq_gpu_base split larger sets into launches of this many.
 constexpr uint64_t kLaunchQueries = kMaxGrid * kBlock;
@@ -261,7 +265,7 @@ rmq_status build_levels(uint32_t R_log, const HierView& H, hipStream_t s) {
 
 template <int RL>
 rmq_status launch_block_scans(const HierView& H, hipStream_t s) {
-  for (uint32_t k = 1; k < H.num_levels; ++k) {
+  for (uint32_t k = H.ps_lo; k < H.num_levels; ++k) {
     const uint32_t blocks = (H.size[k] + (1u << RL) - 1u) >> RL;
     rmq::k_block_scans<RL><<<(blocks + kBlock - 1) / kBlock, kBlock, 0, s>>>(H, k, const_cast<uint4*>(H.ps));
   }
@@ -635,11 +639,18 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   const TopShape ts = top_shape(P, h->flags);
   const uint64_t at_tt = carve(ts.rows * ts.stride * sizeof(uint2));
   // block scans, one 16-byte entry per level-buffer slot: read by the lane solver where R >= 16;
-  // built on request there, and by default where R >= 32 (tail pieces of up to R - 1 entries)
+  // built on request there, and by default where R >= 32 (tail pieces of up to R - 1 entries);
+  // for the levels from the top down while they take at most kScanBytes (a cold level's scans
+  // cost more than its vectors)
   const bool lane = !(h->flags & RMQ_BUILD_FULL) && (alg == RMQ_ALG_VECTOR_LOAD || P.G == 1);
-  const bool scans = lane && P.num_levels > 1 && P.R_log >= 4 && !(h->flags & RMQ_BUILD_NO_BLOCK_SCANS) &&
-                     ((h->flags & RMQ_BUILD_BLOCK_SCANS) || P.R_log >= 5);
-  const uint64_t at_ps = carve(scans ? P.buffer_floats * sizeof(uint4) : 0);
+  uint32_t ps_lo = 0;
+  if (lane && P.num_levels > 1 && P.R_log >= 4 && !(h->flags & RMQ_BUILD_NO_BLOCK_SCANS) &&
+      ((h->flags & RMQ_BUILD_BLOCK_SCANS) || P.R_log >= 5)) {
+    for (uint32_t k = 1; k < P.num_levels && !ps_lo; ++k)
+      if ((P.buffer_floats - P.off[k]) * sizeof(uint4) <= kScanBytes) ps_lo = k;
+  }
+  const bool scans = ps_lo != 0;
+  const uint64_t at_ps = carve(scans ? (P.buffer_floats - P.off[ps_lo]) * sizeof(uint4) : 0);
   const uint64_t at_view = carve(sizeof(HierView));  // the view, on the device, for rmq_query64
   if (hipMalloc(reinterpret_cast<void**>(&h->mem), bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -653,7 +664,11 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
     h->lr = h->mem + at_lr;
   }
   if (ts.rows) h->tt = reinterpret_cast<uint2*>(h->mem + at_tt);
-  if (scans) h->view.ps = reinterpret_cast<const uint4*>(h->mem + at_ps);
+  if (scans) {  // indexed from the level buffer's start: ps[off[k] + i] for k >= ps_lo
+    h->view.ps = reinterpret_cast<const uint4*>(reinterpret_cast<uintptr_t>(h->mem + at_ps) -
+                                                P.off[ps_lo] * sizeof(uint4));
+    h->view.ps_lo = ps_lo;
+  }
   h->self_view = reinterpret_cast<HierView*>(h->mem + at_view);
   HierView& H = h->view;
   H.A = d_array;

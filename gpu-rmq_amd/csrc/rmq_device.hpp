@@ -22,10 +22,11 @@ struct HierView {
   const uint32_t* zeros; // which signs of zero level 0 holds (kZeroPos | kZeroNeg), see below
   const float* tail;     // level 0's last 16-byte vector A[n & ~3, n), +inf padded, 16-byte aligned
   const uint2* tt;       // sparse table over the top level (dev::top_lookup), or null
-  // block scans of levels 1.., or null: entry i of level k at ps[off[k] + i] holds {value bits,
-  // level-0 index} of the leftmost minimum of the rest of its R-block [i, block end) in .x/.y
-  // and of the block up to it [block start, i] in .z/.w
+  // block scans of levels ps_lo.., or null: entry i of level k at ps[off[k] + i] holds {value
+  // bits, level-0 index} of the leftmost minimum of the rest of its R-block [i, block end) in
+  // .x/.y and of the block up to it [block start, i] in .z/.w
   const uint4* ps;
+  uint32_t ps_lo;        // the lowest level with block scans
   uint32_t tstride;      // entries per row of tt
   uint32_t num_levels;
   uint32_t size[kMaxLevels];

@@ -721,7 +721,7 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
     bpay = dev::kDirect | idx0;
     if (left_tail) gL = vK[lk - 1u];
     if (right_tail) gR = vK[ek];
-  } else if (LT::kBlockScans && RL >= 4 && K != 0 && ek - lk > kScanFinalMin &&
+  } else if (LT::kBlockScans && RL >= 4 && K >= H.ps_lo && ek - lk > kScanFinalMin &&
              ((ek - 1u) >> RL) == (lk >> RL) + 1u) {
     // a final range of more than one round of four vectors that is the rest of one R-block and
     // the start of the next: the suffix minimum from lk and the prefix minimum up to ek - 1,
@@ -866,7 +866,7 @@ __device__ __forceinline__ void vl_step(VlState& st, const HierView& H, const LT
   const bool lnext = la && k != 0 && l < (lo << sk);
   const bool rnext = ra && k != 0 && (hi << sk) < e;
   if constexpr (LT::kBlockScans && RL >= 4) {
-    if (k != 0) {
+    if (uk >= H.ps_lo) {
       // each piece is one block-scan entry: the left piece [lo, block end) a suffix minimum, the
       // right piece [block start, hi) a prefix minimum; the next guards are loaded with them
       const uint2* ps = reinterpret_cast<const uint2*>(H.ps + ok);

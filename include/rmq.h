@@ -106,9 +106,10 @@ typedef struct {
  * Same answers. Not with RMQ_BUILD_FULL or RMQ_BUILD_PACK_DESCENTS (which it overrides); the
  * other solvers and rmq_query64 ignore it. */
 #define RMQ_BUILD_PAIRED 0x20u
-/* Block scans: the build also keeps, for every entry i of levels 1.., the leftmost minimum of
- * the rest of its R-block [i, block end) and of the block up to it [block start, i], each with
- * its level-0 index (16 bytes per upper entry: 0.52 n bytes at R = 32). Where R >= 16 the
+/* Block scans: the build also keeps, for every entry i of the upper levels, the leftmost minimum
+ * of the rest of its R-block [i, block end) and of the block up to it [block start, i], each
+ * with its level-0 index (16 bytes per entry; levels from the top down while they take at most
+ * 64 MiB: all of them at n = 2^26, R = 32, levels 2.. at n = 2^30). Where R >= 16 the
  * lane-per-query solver (alg 20; algs 16/24 with group_lanes 1) then reads a tail piece above
  * level 0 as one entry, and a final range of more than 16 entries that spans two blocks as
  * two, instead of scanning up to R - 1 (2R - 2) entries. Same answers. Built by default for the
