@@ -36,10 +36,10 @@ for _p in (ROOT, os.path.join(ROOT, "gpu-rmq_amd")):
 METRIC = "Gqueries/s + %HBM roofline at n=2^26,q=2^26; batched vs plain; serve p99"
 SEED = 27722
 LR_NAMES = {-1: "uniform-large", -2: "lognormal-medium", -3: "lognormal-small", -6: "mixed"}
-# The configuration timed when the sweep finds nothing more than 2% faster (lane per query with
-# packed descents, R = 4, t = 2^13): keeps the timed kernel the one whose counters are committed
-# under profiles/.
-DEFAULT_ALG, DEFAULT_T = "20/c0/p", 1 << 13
+# The configuration timed when the sweep finds nothing more than 2% faster (lane solver with two
+# queries per lane and packed descents, R = 4, t = 2^13): keeps the timed kernel the one whose
+# counters are committed under profiles/.
+DEFAULT_ALG, DEFAULT_T = "20/c0/2p", 1 << 13
 COUNTERS_FILE = os.path.join(ROOT, "profiles", "counters.json")
 SERVE_LOAD = os.path.join(ROOT, "gpu-rmq_amd", "serve_load")
 
@@ -151,7 +151,8 @@ def parse():
     ap.add_argument("--lr", type=int, default=-2)
     ap.add_argument("--alg", default="", help="16, 16/g1, 16/g4, 16/g8, 16/g64, 24/g4, 24/g64, 20, 20/c0, 20/c1, "
                                               "or 20/c0/p, 20/c1/p, 16/p (packed descents), 20/c0/2, 20/c1/2, "
-                                              "16/2 (two queries per lane), 16/n (no block scans), 20/s (block scans at R = 16) "
+                                              "16/2 (two queries per lane), 20/c0/2p, 20/c1/2p, 16/2p (two per lane, packed), "
+                                              "16/n (no block scans), 20/s (block scans at R = 16) "
                                               "(empty = sweep)")
     ap.add_argument("--t", type=int, default=0, help="scan threshold (0 = sweep)")
     ap.add_argument("--mode", choices=["index", "value", "both"], default="index")
@@ -392,8 +393,9 @@ def uniform_leg(dA, n, steps, warmup, peaks=None, ceilings=None):
     dQ = torch.from_numpy(lr.view(np.int32)).cuda()
     out_i = torch.empty(q, dtype=torch.int32, device="cuda")
     runs = []
-    for var, t in (("20/c0", 1 << 10), ("20/c0/p", 1 << 10), ("20/c0/2", 1 << 10), ("20/c1", 1 << 13),
-                   ("20/c1/2", 1 << 13), ("16", 1 << 15), ("16/2", 1 << 15)):
+    for var, t in (("20/c0", 1 << 10), ("20/c0/p", 1 << 10), ("20/c0/2", 1 << 10), ("20/c0/2p", 1 << 10),
+                   ("20/c1", 1 << 13), ("20/c1/2", 1 << 13), ("16", 1 << 15), ("16/2", 1 << 15),
+                   ("16/2p", 1 << 15)):
         alg = int(var.split("/")[0])
         with rmq_amd.Hierarchy(dA, t, alg, **OPTS_OF[var]) as h:
             torch.cuda.synchronize()
@@ -489,7 +491,10 @@ OPTS_OF = {"16": {}, "16/g4": dict(group_lanes=4), "24/g4": dict(group_lanes=4),
            "16/p": dict(pack=True), "20/c0/2": dict(cg_amount_log=0, paired=True),
            "20/c1/2": dict(cg_amount_log=1, paired=True), "16/2": dict(paired=True),
            # block scans: built by default at R = 32 ("16/n" without them), on request at R = 16
-           "16/n": dict(scans=False), "20/s": dict(scans=True)}
+           "16/n": dict(scans=False), "20/s": dict(scans=True),
+           # two queries per lane with packed descents
+           "20/c0/2p": dict(cg_amount_log=0, paired=True, pack=True),
+           "20/c1/2p": dict(cg_amount_log=1, paired=True, pack=True), "16/2p": dict(paired=True, pack=True)}
 
 
 def main():
@@ -554,6 +559,9 @@ def main():
         candidates += [(v, 1 << tl) for v in ("20/c0/p", "20/c1/p", "16/p") for tl in (10, 13, 15)]
         # two queries per lane
         candidates += [(v, 1 << tl) for v in ("20/c0/2", "20/c1/2", "16/2") for tl in (10, 13, 15)]
+        # two queries per lane with packed descents
+        candidates += [(v, 1 << tl) for v in ("20/c0/2p", "20/c1/2p") for tl in (6, 8, 10, 13)]
+        candidates += [("16/2p", 1 << 15), ("20/c0/p", 1 << 6)]
         # alg 16 at the reference default without its block scans; alg 20 (R = 16) with them
         candidates += [("16/n", 1 << 15), ("20/s", 1 << 10), ("20/s", 1 << 13)]
     sweep = []

@@ -338,7 +338,10 @@ rmq_status launch_cl_g(uint32_t G, bool full, int alg, const uint2* Q, uint32_t 
 template <int RL, bool W0, bool PS>
 void launch_vl_pruned(bool packed, bool paired, uint32_t grid, const uint2* Q, uint32_t q, float* ov,
                       uint32_t* oi, const HierView& H, hipStream_t s) {
-  if (paired) {
+  if (paired && packed) {
+    if (oi) rmq::k_query_vl2c<RL, true, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_vl2c<RL, false, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+  } else if (paired) {
     if (oi) rmq::k_query_vl2<RL, true, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
     else rmq::k_query_vl2<RL, false, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
   } else if (packed) {

@@ -1244,4 +1244,102 @@ __global__ __launch_bounds__(kBlock) void k_query_vl2(const uint2* __restrict__ 
   if (!db) vl_emit<RL, IDX>(sb, q0 + 1, H, T, zeros, out_v, out_i);
 }
 
+// Two queries per lane with their descents packed (RMQ_BUILD_PAIRED | RMQ_BUILD_PACK_DESCENTS):
+// both queries' final ranges (their table reads in flight together); one level of the tails,
+// for both queries, where more than kInlineMin lanes of the wavefront still descend; then the
+// queries still descending are parked in LDS (both queries of a lane, wavefront by wavefront)
+// and after the workgroup's barrier the first threads walk them. With uniform lengths about one
+// query in a hundred opens a tail after its table lookup, so without the parking most
+// wavefronts of 128 queries would walk several levels for one or two lanes.
+constexpr uint32_t kInlineMin = 8;
+template <int RL, bool IDX, bool W0 = false, bool PS = false>
+__global__ __launch_bounds__(kBlock) void k_query_vl2c(const uint2* __restrict__ Q, uint32_t q,
+                                                       float* __restrict__ out_v,
+                                                       uint32_t* __restrict__ out_i, HierView H) {
+  using LT = LevelsLdsT<W0, PS>;
+  constexpr uint32_t kWaves = kBlock / 64;
+  __shared__ uint32_t s_meta[kMetaWords];
+  __shared__ uint4 s_st[2 * kBlock];   // l, e, best value bits, best payload
+  __shared__ uint2 s_aux[2 * kBlock];  // next level | la << 8 | ra << 9, query index
+  __shared__ uint32_t s_cnt[kWaves];
+  if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
+  __syncthreads();
+  const LT T{s_meta};
+  const uint32_t zeros = *H.zeros;
+  const uint32_t w = threadIdx.x >> 6;
+  const uint64_t q0 = 2 * (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x);
+  bool need_a = false, need_b = false;
+  VlState sa{}, sb{};
+  if (q0 < q) {
+    const bool two = q0 + 1 < q;
+    uint2 pa, pb;
+    if (two) {
+      const uint4 v = (reinterpret_cast<uintptr_t>(Q) & 15u) == 0
+                          ? *reinterpret_cast<const uint4*>(Q + q0)
+                          : make_uint4(Q[q0].x, Q[q0].y, Q[q0 + 1].x, Q[q0 + 1].y);
+      pa = make_uint2(v.x, v.y);
+      pb = make_uint2(v.z, v.w);
+    } else {
+      pa = Q[q0];
+      pb = pa;
+    }
+    const bool da = emit_nan_at_l<IDX>(zeros, H.A, pa.x, q0, out_v, out_i);
+    const bool db = !two || emit_nan_at_l<IDX>(zeros, H.A, pb.x, q0 + 1, out_v, out_i);
+    const VlTop ta = vl_top_issue<RL>(pa.x, pa.y + 1, H, T);
+    const VlTop tb = vl_top_issue<RL>(pb.x, pb.y + 1, H, T);
+    sa = ta.table ? vl_top_finish(ta, pa.x, pa.y + 1, RL) : vl_start<RL, IDX, true, LT>(pa.x, pa.y + 1, H, T);
+    sb = tb.table ? vl_top_finish(tb, pb.x, pb.y + 1, RL) : vl_start<RL, IDX, true, LT>(pb.x, pb.y + 1, H, T);
+    // one level inline where many lanes of the wavefront descend (lognormal lengths), none where
+    // a few do (uniform lengths: their walks are all parked)
+    if (__popcll(__ballot(sa.la || sa.ra)) > kInlineMin && (sa.la || sa.ra)) vl_step<RL, IDX>(sa, H, T);
+    if (__popcll(__ballot(sb.la || sb.ra)) > kInlineMin && (sb.la || sb.ra)) vl_step<RL, IDX>(sb, H, T);
+    need_a = !da && (sa.la || sa.ra);
+    need_b = !db && (sb.la || sb.ra);
+    if (!da && !need_a) vl_emit<RL, IDX>(sa, q0, H, T, zeros, out_v, out_i);
+    if (!db && !need_b) vl_emit<RL, IDX>(sb, q0 + 1, H, T, zeros, out_v, out_i);
+  }
+  // wave w's parked queries take slots w * 128 + (parked queries of its lower lanes), a before b
+  const uint64_t ma = __ballot(need_a), mb = __ballot(need_b);
+  const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(ma >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(ma), 0u)) +
+                         __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mb >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mb), 0u));
+  const uint32_t slot = w * 128u + below;
+  if (need_a) {
+    s_st[slot] = make_uint4(sa.l, sa.e, dev::f2u(sa.bv), sa.bpay);
+    s_aux[slot] = make_uint2(static_cast<uint32_t>(sa.k) | (sa.la ? 0x100u : 0u) | (sa.ra ? 0x200u : 0u),
+                             static_cast<uint32_t>(q0));
+  }
+  if (need_b) {
+    const uint32_t sl = slot + (need_a ? 1u : 0u);
+    s_st[sl] = make_uint4(sb.l, sb.e, dev::f2u(sb.bv), sb.bpay);
+    s_aux[sl] = make_uint2(static_cast<uint32_t>(sb.k) | (sb.la ? 0x100u : 0u) | (sb.ra ? 0x200u : 0u),
+                           static_cast<uint32_t>(q0 + 1));
+  }
+  if ((threadIdx.x & 63u) == 0u) s_cnt[w] = static_cast<uint32_t>(__popcll(ma) + __popcll(mb));
+  __syncthreads();
+  uint32_t c[kWaves], total = 0;
+#pragma unroll
+  for (uint32_t v = 0; v < kWaves; ++v) {
+    c[v] = s_cnt[v];
+    total += c[v];
+  }
+  for (uint32_t j = threadIdx.x; j < total; j += kBlock) {
+    uint32_t r = j, sl = 2 * kBlock;
+#pragma unroll
+    for (uint32_t v = 0; v < kWaves; ++v) {
+      if (sl == 2 * kBlock) {
+        if (r < c[v]) sl = v * 128u + r;
+        else r -= c[v];
+      }
+    }
+    const uint4 a = s_st[sl];
+    const uint2 b = s_aux[sl];
+    VlState ps{a.x, a.y, dev::u2f(a.z), a.w, static_cast<int>(b.x & 0xFFu), (b.x & 0x100u) != 0u,
+               (b.x & 0x200u) != 0u};
+    while (ps.la || ps.ra) vl_step<RL, IDX>(ps, H, T);
+    vl_emit<RL, IDX>(ps, b.y, H, T, zeros, out_v, out_i);
+  }
+}
+
 }  // namespace rmq

@@ -103,8 +103,11 @@ typedef struct {
 #define RMQ_BUILD_CHECK_QUERIES 0x10u
 /* Lane-per-query solver with two adjacent queries per lane: both queries' top-level table reads
  * are in flight together (queries answered from the table are a chain of two dependent reads).
- * Same answers. Not with RMQ_BUILD_FULL or RMQ_BUILD_PACK_DESCENTS (which it overrides); the
- * other solvers and rmq_query64 ignore it. */
+ * With RMQ_BUILD_PACK_DESCENTS as well, both queries read their final ranges, walk one level
+ * of their tails where many lanes of the wavefront descend, and the queries still descending
+ * are regrouped through LDS (the fastest form on lognormal-medium lengths; uniform lengths as
+ * fast as without packing). Same answers. Not with RMQ_BUILD_FULL; the other solvers and
+ * rmq_query64 ignore it. */
 #define RMQ_BUILD_PAIRED 0x20u
 /* Block scans: the build also keeps, for every entry i of the upper levels, the leftmost minimum
  * of the rest of its R-block [i, block end) and of the block up to it [block start, i], each

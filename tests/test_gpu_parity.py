@@ -52,6 +52,8 @@ VARIANTS = [
     (20, dict(cg_amount_log=0, paired=True)),
     (20, dict(cg_amount_log=1, paired=True)),
     (16, dict(paired=True)),
+    (20, dict(cg_amount_log=0, paired=True, pack=True)),   # two per lane, descents packed
+    (16, dict(paired=True, pack=True)),
     # block prefix / suffix minima, read by the lane solver where R >= 16 (built by default at
     # R >= 32, so the defaults above read them) and ignored by the group solvers
     (20, dict(cg_amount_log=2, scans=True)),
