@@ -146,6 +146,10 @@ constexpr uint64_t kMaxN = 0x7FFFFFFFull;  // int32 n, as the reference's CmdArg
 constexpr uint64_t kMaxN64 = 1ull << 38;    // the 64-bit form (nType = uint64_t, src/tools.h:27-29)
 constexpr uint32_t kChunkLog = 30;          // chunk size of the 64-bit form
 constexpr uint64_t kMaxGrid = 1ull << 20;
+#ifndef RMQ_LDS_TOP
+#define RMQ_LDS_TOP 1
+#endif
+constexpr bool kLdsTop = RMQ_LDS_TOP;
 // Most bytes of block scans a hierarchy keeps (RMQ_BUILD_BLOCK_SCANS), from the top level down:
 // at n = 2^30, R = 32 the scans of level 1 (512 MB, cold) were slower than its vectors, those of
 // levels 2.. faster; at n = 2^26 all of them (33 MB) pay.
@@ -335,9 +339,38 @@ rmq_status launch_cl_g(uint32_t G, bool full, int alg, const uint2* Q, uint32_t 
 
 // The pruned lane kernels, W0: n % 4 == 0, so level-0 vector loads need no tail select; PS: the
 // hierarchy has block scans.
+// The LDS-top form of the paired kernel: a persistent grid of kTopBlock-lane workgroups, as many
+// as fit the CUs with the top level's LDS image.
+template <int RL, bool IDX, bool W0, bool PS>
+void launch_vl2t(const uint2* Q, uint32_t q, float* ov, uint32_t* oi, const HierView& H, hipStream_t s) {
+  const uint32_t m = H.size[H.num_levels - 1];
+  const uint32_t rows = 31u - static_cast<uint32_t>(__builtin_clz(m));
+  const uint32_t bytes = rmq::lds_top_bytes(m, rows) + 8;  // + the row start's 8-byte alignment
+  const void* k = reinterpret_cast<const void*>(rmq::k_query_vl2t<RL, IDX, W0, PS>);
+  static uint32_t grid = 0;
+  if (!grid) {
+    (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024 * 2);
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, rmq::kTopBlock, rmq::lds_top_bytes(rmq::kLdsTopMax, 10) + 8) !=
+            hipSuccess || per_cu < 1)
+      per_cu = 1;
+    grid = static_cast<uint32_t>(cus * per_cu);
+  }
+  rmq::k_query_vl2t<RL, IDX, W0, PS><<<grid, rmq::kTopBlock, bytes, s>>>(Q, q, ov, oi, H);
+}
+
 template <int RL, bool W0, bool PS>
 void launch_vl_pruned(bool packed, bool paired, uint32_t grid, const uint2* Q, uint32_t q, float* ov,
                       uint32_t* oi, const HierView& H, hipStream_t s) {
+  const uint32_t m = H.size[H.num_levels - 1];
+  if (RL <= 5 && paired && !packed && H.tt && H.num_levels > 1 && m > rmq::dev::kTopScan && m <= rmq::kLdsTopMax &&
+      kLdsTop) {
+    if (oi) launch_vl2t<RL, true, W0, PS>(Q, q, ov, oi, H, s);
+    else launch_vl2t<RL, false, W0, PS>(Q, q, ov, oi, H, s);
+    return;
+  }
   if (paired && packed) {
     if (oi) rmq::k_query_vl2c<RL, true, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
     else rmq::k_query_vl2c<RL, false, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);

@@ -1342,4 +1342,97 @@ __global__ __launch_bounds__(kBlock) void k_query_vl2c(const uint2* __restrict__
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// Two queries per lane with the top level and its sparse table in LDS (the uniform-length
+// form). A query answered from the table reads its pair, then two table entries and its guards,
+// and writes its answer: with the table in L2 the middle step is a dependent L2 round trip per
+// query. Here every workgroup (kTopBlock lanes, persistent) first copies the top level's values
+// and the table rows a query reads (rows j >= 3: only final ranges of more than kTopScan = 8
+// entries come from the table) into LDS; those reads are then LDS reads. Queries not answered
+// from the table, and the descents, read the hierarchy in HBM as in k_query_vl2. Top levels of
+// 9..kLdsTopMax entries.
+constexpr uint32_t kTopBlock = 1024;
+constexpr uint32_t kLdsTopMax = 1024;
+constexpr uint32_t kLdsTopRow0 = 3;  // the first table row a query reads (2^3 < kTopScan + 1 <= 2^4)
+static_assert(dev::kTopScan + 1 >= (1u << kLdsTopRow0), "table rows below kLdsTopRow0 are read");
+
+// LDS bytes of that image for a top level of m entries (rows = floor(log2 m) table rows).
+__host__ __device__ constexpr uint32_t lds_top_bytes(uint32_t m, uint32_t rows) {
+  return 4u * m + (rows + 1u - kLdsTopRow0) * m * 8u;
+}
+
+// At most 80 scalar registers: above that (91 measured in index mode) a CU holds only one
+// workgroup of 1024 lanes instead of two, and the kernel ran 118 instead of 162 Gq/s (uniform
+// lengths, n = q = 2^26); the compiler's own occupancy estimate does not show it.
+template <int RL, bool IDX, bool W0 = false, bool PS = false>
+__global__ __launch_bounds__(kTopBlock) __attribute__((amdgpu_num_sgpr(80))) void k_query_vl2t(const uint2* __restrict__ Q, uint32_t q,
+                                                          float* __restrict__ out_v,
+                                                          uint32_t* __restrict__ out_i, HierView H) {
+  using LT = LevelsLdsT<W0, PS>;
+  extern __shared__ __align__(16) uint32_t s_dyn[];
+  __shared__ uint32_t s_meta[kMetaWords];
+  const uint32_t top = H.num_levels - 1;
+  const uint32_t m = H.size[top];
+  const uint32_t rows = 31u - static_cast<uint32_t>(__clz(m));
+  float* s_topv = reinterpret_cast<float*>(s_dyn);            // m values
+  uint2* s_tt = reinterpret_cast<uint2*>(s_dyn + ((m + 1u) & ~1u));  // rows kLdsTopRow0.. of the table
+  {
+    const float* tv = H.lv + H.off[top];
+    for (uint32_t i = threadIdx.x; i < m; i += kTopBlock) s_topv[i] = tv[i];
+    const uint32_t cnt = (rows + 1u - kLdsTopRow0) * m;
+    for (uint32_t x = threadIdx.x; x < cnt; x += kTopBlock) {
+      const uint32_t r = x / m, i = x - r * m;
+      s_tt[x] = H.tt[static_cast<size_t>(r + kLdsTopRow0 - 1u) * H.tstride + i];
+    }
+    if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
+  }
+  __syncthreads();
+  const LT T{s_meta};
+  const uint32_t zeros = *H.zeros;
+  // the final range of [l, e) when it lies on the top level and is longer than kTopScan: its
+  // table answer and guards from LDS (false: the ordinary solver)
+  auto from_table = [&](uint32_t l, uint32_t e, VlState& st) {
+    const uint32_t K = min((31u - static_cast<uint32_t>(__clz(e - l))) / RL, top);
+    const uint32_t shK = RL * K;
+    const uint32_t lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
+    const uint32_t ek = e >> shK;
+    if (K != top || lk >= ek || ek - lk <= dev::kTopScan) return false;
+    const uint32_t j = 31u - static_cast<uint32_t>(__clz(ek - lk));
+    const uint2* row = s_tt + (j - kLdsTopRow0) * m;
+    const uint2 a = row[lk], b = row[ek - (1u << j)];
+    const bool right = dev::u2f(b.x) < dev::u2f(a.x);
+    const float bv = dev::u2f(right ? b.x : a.x);
+    const bool lt = l < (lk << shK), rt = (ek << shK) < e;
+    const float gL = lt ? s_topv[lk - 1u] : RMQ_INF;
+    const float gR = rt ? s_topv[ek] : RMQ_INF;
+    st = VlState{l, e, bv, dev::kDirect | (right ? b.y : a.y), static_cast<int>(K) - 1, lt && gL <= bv,
+                 rt && gR < bv};
+    return true;
+  };
+  const uint64_t step = 2ull * kTopBlock * gridDim.x;
+  for (uint64_t q0 = 2ull * (static_cast<uint64_t>(blockIdx.x) * kTopBlock + threadIdx.x); q0 < q; q0 += step) {
+    const bool two = q0 + 1 < q;
+    uint2 pa, pb;
+    if (two) {
+      const uint4 v = (reinterpret_cast<uintptr_t>(Q) & 15u) == 0
+                          ? *reinterpret_cast<const uint4*>(Q + q0)
+                          : make_uint4(Q[q0].x, Q[q0].y, Q[q0 + 1].x, Q[q0 + 1].y);
+      pa = make_uint2(v.x, v.y);
+      pb = make_uint2(v.z, v.w);
+    } else {
+      pa = Q[q0];
+      pb = pa;
+    }
+    const bool da = emit_nan_at_l<IDX>(zeros, H.A, pa.x, q0, out_v, out_i);
+    const bool db = !two || emit_nan_at_l<IDX>(zeros, H.A, pb.x, q0 + 1, out_v, out_i);
+    VlState sa, sb;
+    if (!from_table(pa.x, pa.y + 1, sa)) sa = vl_start<RL, IDX, true, LT>(pa.x, pa.y + 1, H, T);
+    if (!from_table(pb.x, pb.y + 1, sb)) sb = vl_start<RL, IDX, true, LT>(pb.x, pb.y + 1, H, T);
+    while (sa.la || sa.ra) vl_step<RL, IDX>(sa, H, T);
+    while (sb.la || sb.ra) vl_step<RL, IDX>(sb, H, T);
+    if (!da) vl_emit<RL, IDX>(sa, q0, H, T, zeros, out_v, out_i);
+    if (!db) vl_emit<RL, IDX>(sb, q0 + 1, H, T, zeros, out_v, out_i);
+  }
+}
+
 }  // namespace rmq

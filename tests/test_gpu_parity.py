@@ -477,7 +477,8 @@ def test_check_queries_mode(device):
 def test_uniform_lengths_2_26(device):
     """The north-star workload: n = q = 2^26, uniform lengths (lr = -1, the paper's sweep), value
     and index mode. Alg 16 at the reference default t = 2^15, alg 20 at R = 8 and the bench's
-    uniform-length configuration (alg 20, R = 4, t = 2^10), each packed and not: 2^22 queries
+    uniform-length configuration (alg 20, R = 4, t = 2^10), packed and not, one or two queries
+    per lane (two at t = 2^10: the top level and its table in LDS): 2^22 queries
     against the oracle hierarchy (pinned to CPU BASE in test_oracle.py) plus a CPU BASE sample;
     the full 2^26 queries by size-independent properties (every configuration gives the same
     answer, the index lies in [l, r], the value is A[index])."""
@@ -499,7 +500,10 @@ def test_uniform_lengths_2_26(device):
     first = None
     for alg, t, opts in ((16, 1 << 15, {}), (20, 1 << 13, dict(cg_amount_log=1)),
                          (20, 1 << 10, dict(cg_amount_log=0)), (20, 1 << 10, dict(cg_amount_log=0, pack=True)),
-                         (24, 1 << 15, dict(group_lanes=4))):
+                         (24, 1 << 15, dict(group_lanes=4)),
+                         # two per lane: top level and table in LDS (t = 2^10), in L2 (alg 16), packed
+                         (20, 1 << 10, dict(cg_amount_log=0, paired=True)), (16, 1 << 15, dict(paired=True)),
+                         (20, 1 << 10, dict(cg_amount_log=0, paired=True, pack=True))):
         _say(f"uniform 2^26 alg={alg} t={t} opts={opts}")
         with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
             v, i = h.query(dS, True, True)
