@@ -345,7 +345,7 @@ template <int RL, bool IDX, bool W0, bool PS>
 void launch_vl2t(const uint2* Q, uint32_t q, float* ov, uint32_t* oi, const HierView& H, hipStream_t s) {
   const uint32_t m = H.size[H.num_levels - 1];
   const uint32_t rows = 31u - static_cast<uint32_t>(__builtin_clz(m));
-  const uint32_t bytes = rmq::lds_top_bytes(m, rows) + 8;  // + the row start's 8-byte alignment
+  const uint32_t bytes = rmq::lds_top_bytes(m, rows);
   const void* k = reinterpret_cast<const void*>(rmq::k_query_vl2t<RL, IDX, W0, PS>);
   static uint32_t grid = 0;
   if (!grid) {
@@ -353,7 +353,7 @@ void launch_vl2t(const uint2* Q, uint32_t q, float* ov, uint32_t* oi, const Hier
     int dev = 0, cus = 0, per_cu = 0;
     (void)hipGetDevice(&dev);
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, rmq::kTopBlock, rmq::lds_top_bytes(rmq::kLdsTopMax, 10) + 8) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, rmq::kTopBlock, rmq::lds_top_bytes(rmq::kLdsTopMax, 10)) !=
             hipSuccess || per_cu < 1)
       per_cu = 1;
     grid = static_cast<uint32_t>(cus * per_cu);
@@ -365,8 +365,8 @@ template <int RL, bool W0, bool PS>
 void launch_vl_pruned(bool packed, bool paired, uint32_t grid, const uint2* Q, uint32_t q, float* ov,
                       uint32_t* oi, const HierView& H, hipStream_t s) {
   const uint32_t m = H.size[H.num_levels - 1];
-  if (RL <= 5 && paired && !packed && H.tt && H.num_levels > 1 && m > rmq::dev::kTopScan && m <= rmq::kLdsTopMax &&
-      kLdsTop) {
+  if (RL == 2 && paired && !packed && H.tt && H.num_levels > 1 && m >= (1u << rmq::kLdsTopRow0) &&
+      m <= rmq::kLdsTopMax && kLdsTop) {
     if (oi) launch_vl2t<RL, true, W0, PS>(Q, q, ov, oi, H, s);
     else launch_vl2t<RL, false, W0, PS>(Q, q, ov, oi, H, s);
     return;

@@ -1347,35 +1347,45 @@ __global__ __launch_bounds__(kBlock) void k_query_vl2c(const uint2* __restrict__
 // form). A query answered from the table reads its pair, then two table entries and its guards,
 // and writes its answer: with the table in L2 the middle step is a dependent L2 round trip per
 // query. Here every workgroup (kTopBlock lanes, persistent) first copies the top level's values
-// and the table rows a query reads (rows j >= 3: only final ranges of more than kTopScan = 8
-// entries come from the table) into LDS; those reads are then LDS reads. Queries not answered
-// from the table, and the descents, read the hierarchy in HBM as in k_query_vl2. Top levels of
-// 9..kLdsTopMax entries.
+// and the table rows it reads (rows j >= kLdsTopRow0: final ranges of at least 16 top entries;
+// shorter ones take the ordinary solver, which reads the table in HBM above 8 entries) into
+// LDS; those reads are then LDS reads. With uniform lengths about one query in a hundred opens
+// a tail after its table lookup, and a wavefront would walk it for one or two lanes: instead
+// the queries still descending are parked in the wavefront's own LDS queue (kParkCap entries;
+// no barrier), and walked kParkCap at a time on all lanes when it would overflow and at the
+// end. Top levels of 16..kLdsTopMax entries.
 constexpr uint32_t kTopBlock = 1024;
 constexpr uint32_t kLdsTopMax = 1024;
-constexpr uint32_t kLdsTopRow0 = 3;  // the first table row a query reads (2^3 < kTopScan + 1 <= 2^4)
-static_assert(dev::kTopScan + 1 >= (1u << kLdsTopRow0), "table rows below kLdsTopRow0 are read");
+constexpr uint32_t kLdsTopRow0 = 4;
+constexpr uint32_t kParkCap = 64;
 
-// LDS bytes of that image for a top level of m entries (rows = floor(log2 m) table rows).
+// LDS bytes of that image for a top level of m entries (rows = floor(log2 m) table rows), with
+// the wavefronts' park queues after it.
+__host__ __device__ constexpr uint32_t lds_top_image(uint32_t m, uint32_t rows) {
+  return ((4u * m + 15u) & ~15u) + (rows + 1u - kLdsTopRow0) * m * 8u;
+}
 __host__ __device__ constexpr uint32_t lds_top_bytes(uint32_t m, uint32_t rows) {
-  return 4u * m + (rows + 1u - kLdsTopRow0) * m * 8u;
+  return lds_top_image(m, rows) + (kTopBlock / 64u) * kParkCap * 16u;
 }
 
 // At most 80 scalar registers: above that (91 measured in index mode) a CU holds only one
 // workgroup of 1024 lanes instead of two, and the kernel ran 118 instead of 162 Gq/s (uniform
 // lengths, n = q = 2^26); the compiler's own occupancy estimate does not show it.
 template <int RL, bool IDX, bool W0 = false, bool PS = false>
-__global__ __launch_bounds__(kTopBlock) __attribute__((amdgpu_num_sgpr(80))) void k_query_vl2t(const uint2* __restrict__ Q, uint32_t q,
-                                                          float* __restrict__ out_v,
-                                                          uint32_t* __restrict__ out_i, HierView H) {
+__global__ __launch_bounds__(kTopBlock) __attribute__((amdgpu_num_sgpr(80))) void k_query_vl2t(
+    const uint2* __restrict__ Q, uint32_t q, float* __restrict__ out_v, uint32_t* __restrict__ out_i, HierView H) {
   using LT = LevelsLdsT<W0, PS>;
   extern __shared__ __align__(16) uint32_t s_dyn[];
   __shared__ uint32_t s_meta[kMetaWords];
   const uint32_t top = H.num_levels - 1;
   const uint32_t m = H.size[top];
   const uint32_t rows = 31u - static_cast<uint32_t>(__clz(m));
-  float* s_topv = reinterpret_cast<float*>(s_dyn);            // m values
-  uint2* s_tt = reinterpret_cast<uint2*>(s_dyn + ((m + 1u) & ~1u));  // rows kLdsTopRow0.. of the table
+  float* s_topv = reinterpret_cast<float*>(s_dyn);  // m values
+  uint2* s_tt = reinterpret_cast<uint2*>(s_dyn + ((m + 3u) & ~3u));  // rows kLdsTopRow0.. of the table
+  const uint32_t lane = threadIdx.x & 63u;
+  // this wavefront's park queue: {query index, best value bits, best payload, next level | la | ra}
+  uint4* s_park = reinterpret_cast<uint4*>(reinterpret_cast<uint8_t*>(s_dyn) + lds_top_image(m, rows)) +
+                  (threadIdx.x >> 6) * kParkCap;
   {
     const float* tv = H.lv + H.off[top];
     for (uint32_t i = threadIdx.x; i < m; i += kTopBlock) s_topv[i] = tv[i];
@@ -1389,14 +1399,14 @@ __global__ __launch_bounds__(kTopBlock) __attribute__((amdgpu_num_sgpr(80))) voi
   __syncthreads();
   const LT T{s_meta};
   const uint32_t zeros = *H.zeros;
-  // the final range of [l, e) when it lies on the top level and is longer than kTopScan: its
-  // table answer and guards from LDS (false: the ordinary solver)
+  // the final range of [l, e) when it lies on the top level and spans at least 2^kLdsTopRow0
+  // entries: its table answer and guards from LDS (false: the ordinary solver)
   auto from_table = [&](uint32_t l, uint32_t e, VlState& st) {
     const uint32_t K = min((31u - static_cast<uint32_t>(__clz(e - l))) / RL, top);
     const uint32_t shK = RL * K;
     const uint32_t lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
     const uint32_t ek = e >> shK;
-    if (K != top || lk >= ek || ek - lk <= dev::kTopScan) return false;
+    if (K != top || lk >= ek || ek - lk < (1u << kLdsTopRow0)) return false;
     const uint32_t j = 31u - static_cast<uint32_t>(__clz(ek - lk));
     const uint2* row = s_tt + (j - kLdsTopRow0) * m;
     const uint2 a = row[lk], b = row[ek - (1u << j)];
@@ -1409,30 +1419,65 @@ __global__ __launch_bounds__(kTopBlock) __attribute__((amdgpu_num_sgpr(80))) voi
                  rt && gR < bv};
     return true;
   };
+  // the park queue (its count is the same in every lane): walked on the first `parked` lanes
+  uint32_t parked = 0;
+  auto walk_parked = [&]() {
+    __builtin_amdgcn_wave_barrier();
+    if (lane < parked) {
+      const uint4 x = s_park[lane];
+      const uint2 p = Q[x.x];
+      VlState st{p.x, p.y + 1, dev::u2f(x.y), x.z, static_cast<int>(x.w & 0xFFu), (x.w & 0x100u) != 0u,
+                 (x.w & 0x200u) != 0u};
+      while (st.la || st.ra) vl_step<RL, IDX>(st, H, T);
+      vl_emit<RL, IDX>(st, x.x, H, T, zeros, out_v, out_i);
+    }
+    __builtin_amdgcn_wave_barrier();
+    parked = 0;
+  };
+  // a query that still descends is parked (the queue walked first if it would overflow);
+  // true if it was
+  auto park = [&](bool open, const VlState& st, uint32_t qi) {
+    const uint64_t mk = __ballot(open);
+    const uint32_t c = static_cast<uint32_t>(__popcll(mk));
+    if (c == 0u) return false;
+    if (parked + c > kParkCap) walk_parked();
+    if (open) {
+      const uint32_t at = parked + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mk >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mk), 0u));
+      s_park[at] = make_uint4(qi, dev::f2u(st.bv), st.bpay,
+                              static_cast<uint32_t>(st.k) | (st.la ? 0x100u : 0u) | (st.ra ? 0x200u : 0u));
+    }
+    parked += c;
+    return open;
+  };
   const uint64_t step = 2ull * kTopBlock * gridDim.x;
-  for (uint64_t q0 = 2ull * (static_cast<uint64_t>(blockIdx.x) * kTopBlock + threadIdx.x); q0 < q; q0 += step) {
-    const bool two = q0 + 1 < q;
-    uint2 pa, pb;
+  // the wavefront stays in the loop until its last pair (its queue is walked on all its lanes)
+  for (uint64_t w0 = 2ull * (static_cast<uint64_t>(blockIdx.x) * kTopBlock + (threadIdx.x & ~63u)); w0 < q;
+       w0 += step) {
+    const uint64_t q0 = w0 + 2u * lane;
+    const bool one = q0 < q, two = q0 + 1 < q;
+    uint2 pa = make_uint2(0u, 0u), pb = pa;
     if (two) {
       const uint4 v = (reinterpret_cast<uintptr_t>(Q) & 15u) == 0
                           ? *reinterpret_cast<const uint4*>(Q + q0)
                           : make_uint4(Q[q0].x, Q[q0].y, Q[q0 + 1].x, Q[q0 + 1].y);
       pa = make_uint2(v.x, v.y);
       pb = make_uint2(v.z, v.w);
-    } else {
+    } else if (one) {
       pa = Q[q0];
       pb = pa;
     }
-    const bool da = emit_nan_at_l<IDX>(zeros, H.A, pa.x, q0, out_v, out_i);
+    const bool da = !one || emit_nan_at_l<IDX>(zeros, H.A, pa.x, q0, out_v, out_i);
     const bool db = !two || emit_nan_at_l<IDX>(zeros, H.A, pb.x, q0 + 1, out_v, out_i);
-    VlState sa, sb;
-    if (!from_table(pa.x, pa.y + 1, sa)) sa = vl_start<RL, IDX, true, LT>(pa.x, pa.y + 1, H, T);
-    if (!from_table(pb.x, pb.y + 1, sb)) sb = vl_start<RL, IDX, true, LT>(pb.x, pb.y + 1, H, T);
-    while (sa.la || sa.ra) vl_step<RL, IDX>(sa, H, T);
-    while (sb.la || sb.ra) vl_step<RL, IDX>(sb, H, T);
-    if (!da) vl_emit<RL, IDX>(sa, q0, H, T, zeros, out_v, out_i);
-    if (!db) vl_emit<RL, IDX>(sb, q0 + 1, H, T, zeros, out_v, out_i);
+    VlState sa{}, sb{};
+    if (!da && !from_table(pa.x, pa.y + 1, sa)) sa = vl_start<RL, IDX, true, LT>(pa.x, pa.y + 1, H, T);
+    if (!db && !from_table(pb.x, pb.y + 1, sb)) sb = vl_start<RL, IDX, true, LT>(pb.x, pb.y + 1, H, T);
+    const bool pka = park(!da && (sa.la || sa.ra), sa, static_cast<uint32_t>(q0));
+    const bool pkb = park(!db && (sb.la || sb.ra), sb, static_cast<uint32_t>(q0 + 1));
+    if (!da && !pka) vl_emit<RL, IDX>(sa, q0, H, T, zeros, out_v, out_i);
+    if (!db && !pkb) vl_emit<RL, IDX>(sb, q0 + 1, H, T, zeros, out_v, out_i);
   }
+  if (parked) walk_parked();
 }
 
 }  // namespace rmq
