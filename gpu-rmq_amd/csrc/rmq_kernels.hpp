@@ -1358,6 +1358,7 @@ constexpr uint32_t kTopBlock = 1024;
 constexpr uint32_t kLdsTopMax = 1024;
 constexpr uint32_t kLdsTopRow0 = 4;
 constexpr uint32_t kParkCap = 64;
+constexpr uint32_t kUnstarted = 0x400u;  // a parked query whose solve has not started
 
 // LDS bytes of that image for a top level of m entries (rows = floor(log2 m) table rows), with
 // the wavefronts' park queues after it.
@@ -1419,6 +1420,13 @@ __global__ __launch_bounds__(kTopBlock) __attribute__((amdgpu_num_sgpr(80))) voi
                  rt && gR < bv};
     return true;
   };
+  // Both modes track the winner's position: its value is then the bit pattern of A[leftmost
+  // argmin] (the table's entries hold it, and the solver keeps the winning element's own bits),
+  // so value mode needs no second solve for a zero of unknown sign.
+  auto emit = [&](const VlState& st, uint64_t qi) {
+    if (out_v) out_v[qi] = st.bv;
+    if (IDX && out_i) out_i[qi] = pay_index<RL>(st.bpay, H, T);
+  };
   // the park queue (its count is the same in every lane): walked on the first `parked` lanes
   uint32_t parked = 0;
   auto walk_parked = [&]() {
@@ -1426,17 +1434,18 @@ __global__ __launch_bounds__(kTopBlock) __attribute__((amdgpu_num_sgpr(80))) voi
     if (lane < parked) {
       const uint4 x = s_park[lane];
       const uint2 p = Q[x.x];
-      VlState st{p.x, p.y + 1, dev::u2f(x.y), x.z, static_cast<int>(x.w & 0xFFu), (x.w & 0x100u) != 0u,
-                 (x.w & 0x200u) != 0u};
-      while (st.la || st.ra) vl_step<RL, IDX>(st, H, T);
-      vl_emit<RL, IDX>(st, x.x, H, T, zeros, out_v, out_i);
+      VlState st = (x.w & kUnstarted) ? vl_start<RL, true, true, LT>(p.x, p.y + 1, H, T)
+                                      : VlState{p.x, p.y + 1, dev::u2f(x.y), x.z, static_cast<int>(x.w & 0xFFu),
+                                                (x.w & 0x100u) != 0u, (x.w & 0x200u) != 0u};
+      while (st.la || st.ra) vl_step<RL, true>(st, H, T);
+      emit(st, x.x);
     }
     __builtin_amdgcn_wave_barrier();
     parked = 0;
   };
   // a query that still descends is parked (the queue walked first if it would overflow);
   // true if it was
-  auto park = [&](bool open, const VlState& st, uint32_t qi) {
+  auto park = [&](bool open, const VlState& st, uint32_t qi, bool unstarted) {
     const uint64_t mk = __ballot(open);
     const uint32_t c = static_cast<uint32_t>(__popcll(mk));
     if (c == 0u) return false;
@@ -1445,7 +1454,8 @@ __global__ __launch_bounds__(kTopBlock) __attribute__((amdgpu_num_sgpr(80))) voi
       const uint32_t at = parked + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mk >> 32),
                                                              __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mk), 0u));
       s_park[at] = make_uint4(qi, dev::f2u(st.bv), st.bpay,
-                              static_cast<uint32_t>(st.k) | (st.la ? 0x100u : 0u) | (st.ra ? 0x200u : 0u));
+                              unstarted ? kUnstarted
+                                        : static_cast<uint32_t>(st.k) | (st.la ? 0x100u : 0u) | (st.ra ? 0x200u : 0u));
     }
     parked += c;
     return open;
@@ -1469,13 +1479,15 @@ __global__ __launch_bounds__(kTopBlock) __attribute__((amdgpu_num_sgpr(80))) voi
     }
     const bool da = !one || emit_nan_at_l<IDX>(zeros, H.A, pa.x, q0, out_v, out_i);
     const bool db = !two || emit_nan_at_l<IDX>(zeros, H.A, pb.x, q0 + 1, out_v, out_i);
+    // a query not answered from the table is parked unstarted (its whole solve then runs on a
+    // full wavefront), one that still descends is parked mid-walk
     VlState sa{}, sb{};
-    if (!da && !from_table(pa.x, pa.y + 1, sa)) sa = vl_start<RL, IDX, true, LT>(pa.x, pa.y + 1, H, T);
-    if (!db && !from_table(pb.x, pb.y + 1, sb)) sb = vl_start<RL, IDX, true, LT>(pb.x, pb.y + 1, H, T);
-    const bool pka = park(!da && (sa.la || sa.ra), sa, static_cast<uint32_t>(q0));
-    const bool pkb = park(!db && (sb.la || sb.ra), sb, static_cast<uint32_t>(q0 + 1));
-    if (!da && !pka) vl_emit<RL, IDX>(sa, q0, H, T, zeros, out_v, out_i);
-    if (!db && !pkb) vl_emit<RL, IDX>(sb, q0 + 1, H, T, zeros, out_v, out_i);
+    const bool ta = !da && from_table(pa.x, pa.y + 1, sa);
+    const bool tb = !db && from_table(pb.x, pb.y + 1, sb);
+    const bool pka = park(!da && (!ta || sa.la || sa.ra), sa, static_cast<uint32_t>(q0), !ta);
+    const bool pkb = park(!db && (!tb || sb.la || sb.ra), sb, static_cast<uint32_t>(q0 + 1), !tb);
+    if (!da && !pka) emit(sa, q0);
+    if (!db && !pkb) emit(sb, q0 + 1);
   }
   if (parked) walk_parked();
 }

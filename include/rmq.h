@@ -104,10 +104,10 @@ typedef struct {
 /* Lane-per-query solver with two adjacent queries per lane: both queries' top-level table reads
  * are in flight together (queries answered from the table are a chain of two dependent reads).
  * At R = 4 where the top level holds 16..1024 entries, the kernel keeps the top level and the
- * table rows it reads in LDS (60 KiB per 1024-lane workgroup, two per CU) and parks the queries
- * that still descend in a per-wavefront LDS queue, walked 64 at a time: the fastest form on
- * uniform lengths (193 Gq/s at n = q = 2^26), slower than the packed form below on lognormal
- * and short lengths.
+ * table rows it reads in LDS (60 KiB per 1024-lane workgroup, two per CU) and parks every query
+ * the table does not answer outright in a per-wavefront LDS queue, solved 64 at a time: the
+ * fastest form on uniform lengths (257 Gq/s at n = q = 2^26), slower than the packed form below
+ * on lognormal and short lengths.
  * With RMQ_BUILD_PACK_DESCENTS as well, both queries read their final ranges, walk one level
  * of their tails where many lanes of the wavefront descend, and the queries still descending
  * are regrouped through LDS (the fastest form on lognormal-medium lengths; uniform lengths as

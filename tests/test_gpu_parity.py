@@ -638,19 +638,22 @@ def test_nan_random_against_oracle(device, pattern):
     assert np.array_equal(bi.cpu().numpy().view(np.uint32), ref_i[short])
 
 
-@pytest.mark.parametrize("kind", ["uniform", "ties", "nan"])
+@pytest.mark.parametrize("kind", ["uniform", "ties", "zeros", "nan"])
 def test_lds_top_paired_kernel(device, kind):
     """Two queries per lane at R = 4 with the top level and its sparse table copied into LDS and
     the descending queries parked per wavefront (RMQ_BUILD_PAIRED, tops of 16..1024 entries), value
     and index mode, against the oracle's CPU BASE: uniform and mixed lengths, an odd query count,
     tops of 16, 256 and 1024 entries (n = 2^18) or 65 and 257 (n = 2^18 + 3, n % 4 != 0); values
     drawn from {0, 1, 2, 3} (guards equal to the best open most tails, so the park queues fill
-    and are walked many times); NaN planted at 0.5% of the positions."""
-    rng = np.random.default_rng({"uniform": 1, "ties": 2, "nan": 3}[kind])
+    and are walked many times); +0 and -0 among 1 and 2 (value mode must return the leftmost zero's
+    sign); NaN planted at 0.5% of the positions."""
+    rng = np.random.default_rng({"uniform": 1, "ties": 2, "zeros": 4, "nan": 3}[kind])
     n = (1 << 18) + (3 if kind == "nan" else 0)
     A = rmq_amd.gen_array(n, 11)
     if kind == "ties":
         A = np.floor(A * 4).astype(np.float32)
+    if kind == "zeros":
+        A = np.array([0.0, -0.0, 1.0, 2.0], dtype=np.float32)[rng.integers(0, 4, n)]
     if kind == "nan":
         A[rng.choice(n, n // 200, replace=False)] = np.uint32(0x7FC00000 | 5).view(np.float32)
     lr = np.concatenate([rmq_amd.gen_queries((1 << 14) + 1, -1, n, 3), rmq_amd.gen_queries(1 << 12, -6, n, 4)])
