@@ -693,9 +693,13 @@ struct VlState {
 
 // TOP: long final ranges on the top level come from its sparse table when the hierarchy has
 // one; EARLY (R <= 8): a short final range's vectors are all loaded before any is folded.
-template <int RL, bool IDX, bool TOP, class LT, bool EARLY = TOP>
-__device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
-  const uint32_t top = H.num_levels - 1;
+// The level of [l, e)'s final range and its bounds there (lk = ceil(l / R^K), ek = e >> shK).
+struct VlLevel {
+  uint32_t K, shK, lk, ek;
+};
+
+template <int RL>
+__device__ __forceinline__ VlLevel vl_level(uint32_t l, uint32_t e, uint32_t top) {
   uint32_t K = min((31u - static_cast<uint32_t>(__clz(e - l))) / RL, top);
   uint32_t shK = RL * K;
   uint32_t lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
@@ -706,6 +710,14 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
     lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
     ek = e >> shK;
   }
+  return VlLevel{K, shK, lk, ek};
+}
+
+template <int RL, bool IDX, bool TOP, class LT, bool EARLY = TOP>
+__device__ __forceinline__ VlState vl_start_at(uint32_t l, uint32_t e, const VlLevel& lv, const HierView& H,
+                                               const LT& T) {
+  const uint32_t top = H.num_levels - 1;
+  const uint32_t K = lv.K, shK = lv.shK, lk = lv.lk, ek = lv.ek;
   const float* vK = K == 0 ? H.A : H.lv + T.offz(K);
   const uint32_t szK = T.size(K);
   const bool left_tail = l < (lk << shK);
@@ -831,6 +843,11 @@ __device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierVi
     }
   }
   return VlState{l, e, bv, bpay, static_cast<int>(K) - 1, left_tail && gL <= bv, right_tail && gR < bv};
+}
+
+template <int RL, bool IDX, bool TOP, class LT, bool EARLY = TOP>
+__device__ __forceinline__ VlState vl_start(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
+  return vl_start_at<RL, IDX, TOP, LT, EARLY>(l, e, vl_level<RL>(l, e, H.num_levels - 1), H, T);
 }
 
 // One level of both tails. Any interleaving of the two sides is exact: a left element wins ties
@@ -1155,7 +1172,7 @@ __global__ __launch_bounds__(kBlock) void k_query_vlc(const uint2* __restrict__ 
 // top level and is answered from the table, that answer's two entries and the guards, loaded
 // but not yet compared.
 struct VlTop {
-  uint32_t lk, ek, shK;
+  uint32_t K, lk, ek, shK;
   bool table, lt, rt;
   uint2 x, y;
   float gL, gR;
@@ -1165,16 +1182,9 @@ template <int RL, class LT>
 __device__ __forceinline__ VlTop vl_top_issue(uint32_t l, uint32_t e, const HierView& H, const LT& T) {
   VlTop r;
   const uint32_t top = H.num_levels - 1;
-  uint32_t K = min((31u - static_cast<uint32_t>(__clz(e - l))) / RL, top);
-  uint32_t shK = RL * K;
-  uint32_t lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
-  uint32_t ek = e >> shK;
-  if (lk >= ek) {
-    --K;
-    shK -= RL;
-    lk = (l >> shK) + ((l & ((1u << shK) - 1u)) != 0u);
-    ek = e >> shK;
-  }
+  const VlLevel lv = vl_level<RL>(l, e, top);
+  const uint32_t K = lv.K, shK = lv.shK, lk = lv.lk, ek = lv.ek;
+  r.K = K;
   r.lk = lk;
   r.ek = ek;
   r.shK = shK;
@@ -1196,6 +1206,8 @@ __device__ __forceinline__ VlTop vl_top_issue(uint32_t l, uint32_t e, const Hier
   }
   return r;
 }
+
+__device__ __forceinline__ VlLevel vl_level_of(const VlTop& r) { return VlLevel{r.K, r.shK, r.lk, r.ek}; }
 
 // The lane solver's state after its final range, from vl_top_issue's loads (table case).
 __device__ __forceinline__ VlState vl_top_finish(const VlTop& r, uint32_t l, uint32_t e, uint32_t RL) {
@@ -1235,9 +1247,9 @@ __global__ __launch_bounds__(kBlock) void k_query_vl2(const uint2* __restrict__ 
   const VlTop ta = vl_top_issue<RL>(pa.x, pa.y + 1, H, T);
   const VlTop tb = vl_top_issue<RL>(pb.x, pb.y + 1, H, T);
   VlState sa = ta.table ? vl_top_finish(ta, pa.x, pa.y + 1, RL)
-                        : vl_start<RL, IDX, true, LevelsLdsT<W0, PS>>(pa.x, pa.y + 1, H, T);
+                        : vl_start_at<RL, IDX, true, LevelsLdsT<W0, PS>>(pa.x, pa.y + 1, vl_level_of(ta), H, T);
   VlState sb = tb.table ? vl_top_finish(tb, pb.x, pb.y + 1, RL)
-                        : vl_start<RL, IDX, true, LevelsLdsT<W0, PS>>(pb.x, pb.y + 1, H, T);
+                        : vl_start_at<RL, IDX, true, LevelsLdsT<W0, PS>>(pb.x, pb.y + 1, vl_level_of(tb), H, T);
   while (sa.la || sa.ra) vl_step<RL, IDX>(sa, H, T);
   while (sb.la || sb.ra) vl_step<RL, IDX>(sb, H, T);
   if (!da) vl_emit<RL, IDX>(sa, q0, H, T, zeros, out_v, out_i);
@@ -1287,8 +1299,10 @@ __global__ __launch_bounds__(kBlock) void k_query_vl2c(const uint2* __restrict__
     const bool db = !two || emit_nan_at_l<IDX>(zeros, H.A, pb.x, q0 + 1, out_v, out_i);
     const VlTop ta = vl_top_issue<RL>(pa.x, pa.y + 1, H, T);
     const VlTop tb = vl_top_issue<RL>(pb.x, pb.y + 1, H, T);
-    sa = ta.table ? vl_top_finish(ta, pa.x, pa.y + 1, RL) : vl_start<RL, IDX, true, LT>(pa.x, pa.y + 1, H, T);
-    sb = tb.table ? vl_top_finish(tb, pb.x, pb.y + 1, RL) : vl_start<RL, IDX, true, LT>(pb.x, pb.y + 1, H, T);
+    sa = ta.table ? vl_top_finish(ta, pa.x, pa.y + 1, RL)
+                  : vl_start_at<RL, IDX, true, LT>(pa.x, pa.y + 1, vl_level_of(ta), H, T);
+    sb = tb.table ? vl_top_finish(tb, pb.x, pb.y + 1, RL)
+                  : vl_start_at<RL, IDX, true, LT>(pb.x, pb.y + 1, vl_level_of(tb), H, T);
     // one level inline where many lanes of the wavefront descend (lognormal lengths), none where
     // a few do (uniform lengths: their walks are all parked)
     if (__popcll(__ballot(sa.la || sa.ra)) > kInlineMin && (sa.la || sa.ra)) vl_step<RL, IDX>(sa, H, T);
