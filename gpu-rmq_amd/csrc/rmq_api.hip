@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -146,10 +147,8 @@ constexpr uint64_t kMaxN = 0x7FFFFFFFull;  // int32 n, as the reference's CmdArg
 constexpr uint64_t kMaxN64 = 1ull << 38;    // the 64-bit form (nType = uint64_t, src/tools.h:27-29)
 constexpr uint32_t kChunkLog = 30;          // chunk size of the 64-bit form
 constexpr uint64_t kMaxGrid = 1ull << 20;
-#ifndef RMQ_LDS_TOP
-#define RMQ_LDS_TOP 1
-#endif
-constexpr bool kLdsTop = RMQ_LDS_TOP;
+constexpr int kMaxDevices = 64;  // devices a process's per-device launch shapes are kept for
+
 // Most bytes of block scans a hierarchy keeps (RMQ_BUILD_BLOCK_SCANS), from the top level down:
 // at n = 2^30, R = 32 the scans of level 1 (512 MB, cold) were slower than its vectors, those of
 // levels 2.. faster; at n = 2^26 all of them (33 MB) pay.
@@ -347,16 +346,21 @@ void launch_vl2t(const uint2* Q, uint32_t q, float* ov, uint32_t* oi, const Hier
   const uint32_t rows = 31u - static_cast<uint32_t>(__builtin_clz(m));
   const uint32_t bytes = rmq::lds_top_bytes(m, rows);
   const void* k = reinterpret_cast<const void*>(rmq::k_query_vl2t<RL, IDX, W0, PS>);
-  static uint32_t grid = 0;
+  // per device, on its first launch there: the LDS limit raised past 64 KiB, and as many
+  // workgroups as its CUs hold with the largest image (concurrent first calls compute the same)
+  static std::atomic<uint32_t> grids[kMaxDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+  uint32_t grid = grids[dev].load(std::memory_order_relaxed);
   if (!grid) {
     (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024 * 2);
-    int dev = 0, cus = 0, per_cu = 0;
-    (void)hipGetDevice(&dev);
+    int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, rmq::kTopBlock, rmq::lds_top_bytes(rmq::kLdsTopMax, 10)) !=
             hipSuccess || per_cu < 1)
       per_cu = 1;
     grid = static_cast<uint32_t>(cus * per_cu);
+    grids[dev].store(grid, std::memory_order_relaxed);
   }
   rmq::k_query_vl2t<RL, IDX, W0, PS><<<grid, rmq::kTopBlock, bytes, s>>>(Q, q, ov, oi, H);
 }
@@ -366,7 +370,7 @@ void launch_vl_pruned(bool packed, bool paired, uint32_t grid, const uint2* Q, u
                       uint32_t* oi, const HierView& H, hipStream_t s) {
   const uint32_t m = H.size[H.num_levels - 1];
   if (RL == 2 && paired && !packed && H.tt && H.num_levels > 1 && m >= (1u << rmq::kLdsTopRow0) &&
-      m <= rmq::kLdsTopMax && kLdsTop) {
+      m <= rmq::kLdsTopMax) {
     if (oi) launch_vl2t<RL, true, W0, PS>(Q, q, ov, oi, H, s);
     else launch_vl2t<RL, false, W0, PS>(Q, q, ov, oi, H, s);
     return;
