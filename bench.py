@@ -12,9 +12,10 @@ The roofline names the resource that binds the kernel: per-launch VALU / SALU wa
 and HBM bytes come from rocprofv3 counter runs of the same kernel and workload (committed in
 profiles/counters.json), divided by the kernel time measured live with HIP events; the
 reference traversal's algorithmic bytes are reported beside it as `traversal_equiv`. Beside the
-headline the line carries the uniform-length workload (lr = -1), the batched workload (4096
-arrays, one persistent launch, against per-array plain launches) and the serving curve (p50 /
-p99 latency and Gq/s over offered loads, driven by the C++ harness gpu-rmq_amd/serve_load).
+headline the line carries the uniform-length workload (lr = -1), the large workload (n = 2^30,
+q = 2^26, small and mixed lengths), the batched workload (4096 arrays, one persistent launch,
+against per-array plain launches) and the serving curve (p50 / p99 latency and Gq/s over offered
+loads, driven by the C++ harness gpu-rmq_amd/serve_load).
 
   python bench.py [--steps K] [--warmup W] [--alg A --t T] [--lr L] [--no-cpu]
   python bench.py --batched-only     # the batched build + query alone (profiling runs)
@@ -162,6 +163,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-uniform", action="store_true", help="skip the uniform-length leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the n = 2^30 leg")
+    ap.add_argument("--large-log-n", type=int, default=30, help="log2 of the large leg's array size")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched build and its query launch (no per-array baseline, no "
                          "graph, no other leg): the form rocprofv3 counter runs use")
@@ -423,6 +426,62 @@ def uniform_leg(dA, n, steps, warmup, peaks=None, ceilings=None):
             "spot_check": f"{sidx.size} sampled queries shorter than 2^22 equal GPU BASE"}
 
 
+def large_leg(log_n, steps, warmup, peaks=None, ceilings=None):
+    """The large plain workload: n = 2^30 (4 GiB of float32), q = 2^26, index mode, under the
+    small (lr = -3) and mixed (lr = -6) length mixes. At this n the upper levels no longer fit
+    the Infinity Cache and the walks wait on HBM; a few forms that win there (alg 16 with four
+    lanes per query, the lane solver at R = 8 and R = 4) are timed; the lane solver at R = 8,
+    t = 2^6 (the one profiled for both mixes) is reported unless another form is more than 3%
+    faster, with a spot check against GPU BASE."""
+    import numpy as np
+    import torch
+
+    import rmq_amd
+    from rmq_amd import roofline
+
+    n, q = 1 << log_n, 1 << 26
+    t0 = time.perf_counter()
+    A = rmq_amd.gen_array(n, SEED)
+    dA = torch.from_numpy(A).cuda()
+    del A
+    out_i = torch.empty(q, dtype=torch.int32, device="cuda")
+    log(f"large: n=2^{log_n} array generated and copied in {time.perf_counter() - t0:.1f} s")
+    mixes = []
+    for lr_mode in (-3, -6):
+        lr = rmq_amd.gen_queries(q, lr_mode, n, SEED)
+        dQ = torch.from_numpy(lr.view(np.int32)).cuda()
+        runs = []
+        for var, t in (("16/g4", 1 << 10), ("20/c1", 1 << 6), ("20/c1/2", 1 << 10), ("20/c0", 1 << 6)):
+            alg = int(var.split("/")[0])
+            with rmq_amd.Hierarchy(dA, t, alg, **OPTS_OF[var]) as h:
+                torch.cuda.synchronize()
+                wall, ev = time_queries(h, dQ, False, True, None, out_i, steps, warmup)
+            runs.append({"alg": var, "t": t, "ms": round(ev, 4), "gqs": round(q / ev / 1e6, 3)})
+            log(f"large lr={lr_mode} alg={var} t={t}: {ev:.3f} ms/launch, {q / ev / 1e6:.2f} Gq/s")
+        best = min(runs, key=lambda r: r["ms"])
+        if runs[1]["ms"] <= best["ms"] * 1.03:  # keep the profiled form unless another clearly wins
+            best = runs[1]
+        with rmq_amd.Hierarchy(dA, best["t"], int(best["alg"].split("/")[0]), **OPTS_OF[best["alg"]]) as h:
+            h.query(dQ, False, True, None, out_i)
+            torch.cuda.synchronize()
+        sidx = rmq_amd.gen_sample_indices(q, 1 << 12, SEED + 5)
+        sidx = sidx[(lr[sidx, 1].astype(np.int64) - lr[sidx, 0]) < (1 << 22)]
+        sq = torch.from_numpy(np.ascontiguousarray(lr[sidx]).view(np.int32)).cuda()
+        _, bi = rmq_amd.gpu_base(dA, sq, False, True)
+        sel = torch.from_numpy(sidx.astype(np.int64)).cuda()
+        assert torch.equal(out_i[sel], bi), f"large lr={lr_mode} output differs from GPU BASE"
+        key = f"alg={best['alg']} t={best['t']} n=2^{log_n} q=2^26 lr={lr_mode} index"
+        mixes.append({"lr": lr_mode, "lengths": LR_NAMES[lr_mode], "gqs": best["gqs"], "alg": best["alg"],
+                      "t": best["t"], "ms": best["ms"], "runs": runs,
+                      "roofline": roofline_block(key, roofline.total_bytes(lr, 1), q, best["ms"], peaks, ceilings),
+                      "spot_check": f"{sidx.size} sampled queries shorter than 2^22 equal GPU BASE"})
+        del dQ, lr
+    del dA, out_i
+    torch.cuda.empty_cache()
+    return {"workload": f"n=2^{log_n} (uniform [0,1) float32), q=2^26, index mode, small and mixed lengths",
+            "mixes": mixes}
+
+
 def host_cpu():
     model = "unknown"
     try:
@@ -640,6 +699,10 @@ def main():
                                         ceilings=ceilings)
     del dA
     torch.cuda.empty_cache()
+    if not args.no_large:
+        log("large leg")
+        result["large"] = large_leg(args.large_log_n, steps=max(3, args.steps // 2), warmup=1,
+                                    peaks=peaks, ceilings=ceilings)
     if not args.no_batched:
         log("batched leg")
         result["batched"] = batched_leg(args.batch_arrays, peaks=peaks, ceilings=ceilings)

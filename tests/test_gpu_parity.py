@@ -324,7 +324,8 @@ def _queries_2_26():
 
 @pytest.mark.parametrize("lr_mode", [-3, -6])
 def test_config3_large_array(device, lr_mode):
-    """n = 2^30 (4 GiB), lr = -3 and -6, value + index; 2^20 queries vs the oracle hierarchy."""
+    """n = 2^30 (4 GiB), lr = -3 and -6, value + index; 2^20 queries vs the oracle hierarchy, in
+    every form the bench times at this size among others."""
     import torch
 
     n = 1 << 30
@@ -337,7 +338,9 @@ def test_config3_large_array(device, lr_mode):
     dA = torch.from_numpy(A).cuda()
     dQ = torch.from_numpy(lr.view(np.int32)).cuda()
     for alg, t, opts in ((16, 64, {}), (24, 1 << 15, {}), (20, 32, dict(cg_amount_log=1)),
-                         (20, 1 << 13, dict(cg_amount_log=0, pack=True))):
+                         (20, 1 << 13, dict(cg_amount_log=0, pack=True)),
+                         (16, 1 << 10, dict(group_lanes=4)), (20, 64, dict(cg_amount_log=1)),
+                         (20, 1 << 10, dict(cg_amount_log=1, paired=True)), (20, 64, dict(cg_amount_log=0))):
         _say(f"config3 lr={lr_mode} alg={alg} t={t} opts={opts}")
         with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
             v, i = h.query(dQ, True, True)
