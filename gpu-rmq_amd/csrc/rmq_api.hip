@@ -368,12 +368,13 @@ void launch_vl2t(const uint2* Q, uint32_t q, float* ov, uint32_t* oi, const Hier
 template <int RL, bool W0, bool PS>
 void launch_vl_pruned(bool packed, bool paired, uint32_t grid, const uint2* Q, uint32_t q, float* ov,
                       uint32_t* oi, const HierView& H, hipStream_t s) {
-  const uint32_t m = H.size[H.num_levels - 1];
-  if (RL == 2 && paired && !packed && H.tt && H.num_levels > 1 && m >= (1u << rmq::kLdsTopRow0) &&
-      m <= rmq::kLdsTopMax) {
-    if (oi) launch_vl2t<RL, true, W0, PS>(Q, q, ov, oi, H, s);
-    else launch_vl2t<RL, false, W0, PS>(Q, q, ov, oi, H, s);
-    return;
+  if constexpr (RL == 2) {  // the LDS-top form is built for R = 4 only
+    const uint32_t m = H.size[H.num_levels - 1];
+    if (paired && !packed && H.tt && H.num_levels > 1 && m >= (1u << rmq::kLdsTopRow0) && m <= rmq::kLdsTopMax) {
+      if (oi) launch_vl2t<RL, true, W0, PS>(Q, q, ov, oi, H, s);
+      else launch_vl2t<RL, false, W0, PS>(Q, q, ov, oi, H, s);
+      return;
+    }
   }
   if (paired && packed) {
     if (oi) rmq::k_query_vl2c<RL, true, W0, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);

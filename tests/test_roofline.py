@@ -55,3 +55,33 @@ def test_build_bytes_matches_level_plan():
         rls.append(rl)
         want += one
     assert roofline.build_bytes(ns, ts, rls) == want
+
+
+def test_bench_bound_skips_passed_measured_ceilings(monkeypatch):
+    """bench.roofline_block: the bound is the resource nearest its peak, except a measured HBM
+    ceiling that the kernel passes (misses the Infinity Cache serves count in FETCH_SIZE but not
+    in the probes over 4 GiB buffers): that one is listed and skipped. Figures of the n = 2^30
+    small-lengths kernel: 22.3 GB per launch of 3.06 ms."""
+    import bench
+
+    counters = {"bytes_per_launch": 22_295_000_000, "fetch_bytes": 22_027_000_000,
+                "write_bytes": 268_000_000, "valu_insts": 545_300_000, "salu_insts": 253_500_000}
+    monkeypatch.setattr(bench, "measured_counters", lambda key: counters)
+    ceilings = {"read_gbps": 5972.1, "read_nt_gbps": 6456.0, "copy_gbps": 5176.4,
+                "read4w1_gbps": 5255.7, "random128_gsegments_per_s": 48.12}
+    peaks = {"valu_peak_ginst_per_s": 1005.6, "salu_peak_ginst_per_s": 592.2}
+    out = bench.roofline_block("k", 196.0 * 2**26, 2**26, 3.06, peaks, ceilings)
+    assert out["passed_measured_ceilings"] == ["hbm_achievable", "hbm_random_lines"]
+    assert out["bound"] == "hbm" and out["peak"] == roofline.HBM_PEAK_GBPS
+    assert abs(out["frac"] - 22_295_000_000 / 3.06e-3 / 1e9 / 8000.0) < 1e-4
+    assert out["resources"]["hbm_random_lines"]["frac"] > 1
+    # a kernel below every ceiling: the measured stream ceiling binds (0.62 > 0.50 of 8 TB/s)
+    counters.update(bytes_per_launch=1_033_000_000, fetch_bytes=715_000_000, valu_insts=103_200_000,
+                    salu_insts=58_800_000)
+    out = bench.roofline_block("k", 196.0 * 2**26, 2**26, 0.2595, peaks, ceilings)
+    assert "passed_measured_ceilings" not in out
+    assert out["bound"] == "hbm_achievable" and out["frac"] <= 1
+    # nothing measured for the key: unmeasured, no fraction
+    monkeypatch.setattr(bench, "measured_counters", lambda key: None)
+    out = bench.roofline_block("k", 196.0 * 2**26, 2**26, 1.0, peaks, ceilings)
+    assert out["bound"] == "unmeasured" and out["frac"] is None
