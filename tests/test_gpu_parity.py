@@ -667,3 +667,36 @@ def test_lds_top_paired_kernel(device, kind):
         assert_exact(A, lr, v, i, ref_i, f"lds top {kind} t={t}")
         v, _ = run_gpu(A, lr, 20, t, True, False, cg_amount_log=0, paired=True)
         assert_exact(A, lr, v, None, ref_i, f"lds top {kind} t={t} value")
+
+
+@pytest.mark.parametrize("form", ["lane", "packed", "paired", "paired_packed", "lds_top", "alg16"])
+def test_query_buffers_off_16_byte_alignment(device, form):
+    """Query pairs that start 8 bytes past a 16-byte boundary (the two-per-lane kernels then load
+    each pair on its own instead of both in one 16-byte load) and answer buffers 4 bytes past one,
+    against the oracle's CPU BASE, value and index mode, with an odd query count."""
+    import torch
+
+    n = (1 << 18) + 1
+    A = rmq_amd.gen_array(n, 5)
+    lr = np.concatenate([rmq_amd.gen_queries(3001, -1, n, 6), rmq_amd.gen_queries(3000, -6, n, 7)])
+    q = lr.shape[0]
+    ref_i = oracle.cpu_base_index(A, lr)
+    alg, t, opts = {"lane": (20, 1 << 13, dict(cg_amount_log=0)),
+                    "packed": (20, 1 << 13, dict(cg_amount_log=0, pack=True)),
+                    "paired": (20, 1 << 13, dict(cg_amount_log=1, paired=True)),
+                    "paired_packed": (20, 1 << 13, dict(cg_amount_log=0, paired=True, pack=True)),
+                    "lds_top": (20, 1 << 10, dict(cg_amount_log=0, paired=True)),
+                    "alg16": (16, 1 << 15, dict(paired=True))}[form]
+    dA = torch.from_numpy(A).cuda()
+    big = torch.zeros((q + 1, 2), dtype=torch.int32, device="cuda")
+    big[1:] = torch.from_numpy(lr.view(np.int32)).cuda()
+    dQ = big[1:]
+    ov = torch.full((q + 1,), -1.0, dtype=torch.float32, device="cuda")
+    oi = torch.full((q + 1,), -1, dtype=torch.int32, device="cuda")
+    assert dQ.data_ptr() % 16 == 8 and ov[1:].data_ptr() % 8 == 4 and oi[1:].data_ptr() % 8 == 4
+    _say(f"unaligned buffers form={form}")
+    with rmq_amd.Hierarchy(dA, t, alg, **opts) as h:
+        h.query(dQ, True, True, ov[1:], oi[1:])
+        torch.cuda.synchronize()
+    assert ov[0].item() == -1.0 and oi[0].item() == -1  # nothing written before the buffers
+    assert_exact(A, lr, ov[1:].cpu().numpy(), oi[1:].cpu().numpy().view(np.uint32), ref_i, f"unaligned {form}")
