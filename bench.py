@@ -26,6 +26,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import resource
 import sys
 import time
 
@@ -35,6 +36,9 @@ for _p in (ROOT, os.path.join(ROOT, "gpu-rmq_amd")):
         sys.path.insert(0, _p)
 
 METRIC = "Gqueries/s + %HBM roofline at n=2^26,q=2^26; batched vs plain; serve p99"
+# the process's CPU affinity before any OpenMP runtime starts (with OMP_PROC_BIND set, the runtime
+# binds the main thread to one place, and its own affinity then shows one CPU)
+START_AFFINITY = len(os.sched_getaffinity(0))
 SEED = 27722
 LR_NAMES = {-1: "uniform-large", -2: "lognormal-medium", -3: "lognormal-small", -6: "mixed"}
 # The configuration timed when the sweep finds nothing more than 2% faster (lane solver with two
@@ -502,7 +506,7 @@ def host_cpu():
     except OSError:
         pass
     return {"model": model, "os_cpu_count": os.cpu_count(),
-            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "affinity_cpus": START_AFFINITY, "main_thread_affinity_cpus": len(os.sched_getaffinity(0)),
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
             "omp_proc_bind": os.environ.get("OMP_PROC_BIND")}
 
@@ -524,9 +528,13 @@ def cpu_baseline(A, lr, target_s=12.0):
     dt = max(time.perf_counter() - t0, 1e-4)
     k = int(min(q, max(2048, 2048 * target_s / dt)))
     sample = lr[rmq_amd.gen_sample_indices(q, k, SEED + 2)]
+    r0 = resource.getrusage(resource.RUSAGE_SELF)
     t0 = time.perf_counter()
     oracle.cpu_base_index(A, sample)
     dt = time.perf_counter() - t0
+    r1 = resource.getrusage(resource.RUSAGE_SELF)
+    # CPU seconds the process spent per wall second of the scan: the cores actually in use
+    busy = (r1.ru_utime - r0.ru_utime + r1.ru_stime - r0.ru_stime) / dt
     gq = k / dt / 1e9
     cap = (f"; affinity allows {host['affinity_cpus']} of the host's {host['os_cpu_count']} CPUs "
            f"and OMP_NUM_THREADS caps the pool at {threads}"
@@ -541,8 +549,9 @@ def cpu_baseline(A, lr, target_s=12.0):
     return {
         "value": gq, "unit": "Gqueries/s", "cores": threads, "kind": "port",
         "sample": f"CPU BASE IDX (alg 100) over {k} queries drawn from the {q}-query set "
-                  f"({dt:.1f} s wall, {threads} OpenMP threads, OMP_PROC_BIND="
-                  f"{host['omp_proc_bind']}, {host['model']}{cap}; extrapolated per query)",
+                  f"({dt:.1f} s wall, {threads} OpenMP threads keeping {busy:.1f} CPUs busy, "
+                  f"OMP_PROC_BIND={host['omp_proc_bind']}, {host['model']}{cap}; extrapolated per query)",
+        "cpu_seconds_per_wall_second": round(busy, 2),
         "host": host,
     }, {
         "value": q / th / 1e9, "unit": "Gqueries/s", "cores": threads,
