@@ -301,37 +301,41 @@ uint64_t rel_layout(uint32_t R_log, uint32_t num_levels, const uint64_t* size, u
 
 // ---- query dispatch -----------------------------------------------------------------------
 // P = pruned evaluation order (default) or every piece read (RMQ_BUILD_FULL).
-template <int G, int RL, bool P>
+// PS: the hierarchy has block scans (read by the pruned order only).
+template <int G, int RL, bool P, bool PS>
 rmq_status launch_cl(int alg, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
                      const HierView& H, hipStream_t s) {
   const bool idx = oi != nullptr;
   if (alg == RMQ_ALG_XXX) {
     const uint32_t grid = grid_for(q);
     if (static_cast<uint64_t>(grid) * kBlock < q) return RMQ_EINVAL;
-    if (idx) rmq::k_query_xxx<G, RL, P, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-    else rmq::k_query_xxx<G, RL, P, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    if (idx) rmq::k_query_xxx<G, RL, P, true, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_xxx<G, RL, P, false, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
   } else {
     const uint32_t grid = grid_for(static_cast<uint64_t>(q) * G);
-    if (idx) rmq::k_query_multi<G, RL, P, true><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
-    else rmq::k_query_multi<G, RL, P, false><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    if (idx) rmq::k_query_multi<G, RL, P, true, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
+    else rmq::k_query_multi<G, RL, P, false, PS><<<grid, kBlock, 0, s>>>(Q, q, ov, oi, H);
   }
   return last_error();
+}
+
+template <int G, int RL>
+rmq_status launch_cl_order(bool full, int alg, const uint2* Q, uint32_t q, float* ov, uint32_t* oi,
+                           const HierView& H, hipStream_t s) {
+  if (full) return launch_cl<G, RL, false, false>(alg, Q, q, ov, oi, H, s);
+  if (H.ps) return launch_cl<G, RL, true, true>(alg, Q, q, ov, oi, H, s);
+  return launch_cl<G, RL, true, false>(alg, Q, q, ov, oi, H, s);
 }
 
 template <int RL>
 rmq_status launch_cl_g(uint32_t G, bool full, int alg, const uint2* Q, uint32_t q, float* ov,
                        uint32_t* oi, const HierView& H, hipStream_t s) {
   switch (G) {
-    case 4: return full ? launch_cl<4, RL, false>(alg, Q, q, ov, oi, H, s)
-                        : launch_cl<4, RL, true>(alg, Q, q, ov, oi, H, s);
-    case 8: return full ? launch_cl<8, RL, false>(alg, Q, q, ov, oi, H, s)
-                        : launch_cl<8, RL, true>(alg, Q, q, ov, oi, H, s);
-    case 16: return full ? launch_cl<16, RL, false>(alg, Q, q, ov, oi, H, s)
-                         : launch_cl<16, RL, true>(alg, Q, q, ov, oi, H, s);
-    case 32: return full ? launch_cl<32, RL, false>(alg, Q, q, ov, oi, H, s)
-                         : launch_cl<32, RL, true>(alg, Q, q, ov, oi, H, s);
-    case 64: return full ? launch_cl<64, RL, false>(alg, Q, q, ov, oi, H, s)   // a wave per query
-                         : launch_cl<64, RL, true>(alg, Q, q, ov, oi, H, s);
+    case 4: return launch_cl_order<4, RL>(full, alg, Q, q, ov, oi, H, s);
+    case 8: return launch_cl_order<8, RL>(full, alg, Q, q, ov, oi, H, s);
+    case 16: return launch_cl_order<16, RL>(full, alg, Q, q, ov, oi, H, s);
+    case 32: return launch_cl_order<32, RL>(full, alg, Q, q, ov, oi, H, s);
+    case 64: return launch_cl_order<64, RL>(full, alg, Q, q, ov, oi, H, s);  // a wave per query
     default: return RMQ_EUNSUPPORTED;
   }
 }
@@ -679,13 +683,13 @@ rmq_status rmq_build_ex(const float* d_array, uint64_t n, uint64_t t, int alg,
   }
   const TopShape ts = top_shape(P, h->flags);
   const uint64_t at_tt = carve(ts.rows * ts.stride * sizeof(uint2));
-  // block scans, one 16-byte entry per level-buffer slot: read by the lane solver where R >= 16;
-  // built on request there, and by default where R >= 32 (tail pieces of up to R - 1 entries);
-  // for the levels from the top down while they take at most kScanBytes (a cold level's scans
-  // cost more than its vectors)
-  const bool lane = !(h->flags & RMQ_BUILD_FULL) && (alg == RMQ_ALG_VECTOR_LOAD || P.G == 1);
+  // block scans, one 16-byte entry per level-buffer slot: read by the pruned solvers (lane and
+  // group) where R >= 16; built on request there, and by default where R >= 32 (tail pieces of
+  // up to R - 1 entries); for the levels from the top down while they take at most kScanBytes
+  // (a cold level's scans cost more than its vectors)
+  const bool pruned = !(h->flags & RMQ_BUILD_FULL);
   uint32_t ps_lo = 0;
-  if (lane && P.num_levels > 1 && P.R_log >= 4 && !(h->flags & RMQ_BUILD_NO_BLOCK_SCANS) &&
+  if (pruned && P.num_levels > 1 && P.R_log >= 4 && !(h->flags & RMQ_BUILD_NO_BLOCK_SCANS) &&
       ((h->flags & RMQ_BUILD_BLOCK_SCANS) || P.R_log >= 5)) {
     for (uint32_t k = 1; k < P.num_levels && !ps_lo; ++k)
       if ((P.buffer_floats - P.off[k]) * sizeof(uint4) <= kScanBytes) ps_lo = k;

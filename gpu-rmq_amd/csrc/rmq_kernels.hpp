@@ -263,7 +263,7 @@ struct LevelsOf {
 };
 // W0: level 0 (the caller's array) holds whole 16-byte vectors (n % 4 == 0), so its loads need
 // no tail select.
-// PS: the hierarchy has block scans (HierView::ps), which the lane solver reads.
+// PS: the hierarchy has block scans (HierView::ps), which the lane and group solvers read.
 template <bool W0, bool PS = false>
 struct LevelsLdsT {
   static constexpr bool kWhole0 = W0;
@@ -368,6 +368,13 @@ __device__ __forceinline__ void fold_piece(dev::Cand& b, float& bv, const float*
   }
 }
 
+// Block scans answer a final range only when it is longer than this: a shorter one is one round
+// of four vector loads in the lane solver, and at most two rounds of a G-lane group's (G = 8:
+// 16 measured 10% ahead of 4G = 32, G = 4 even; 4 lost 4% on short lengths). In the lane
+// solver they answer tail pieces only where R >= 16 (a piece of R <= 8 is at most two vectors,
+// whose loads also hold the next guard).
+constexpr uint32_t kScanFinalMin = 16;
+
 // TOP = false leaves the top level's sparse table out (for kernels short of registers).
 template <int G, int RL, bool IDX, bool TOP = true, class LT>
 __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, uint32_t gl,
@@ -403,7 +410,21 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
   uint32_t bpos = dev::kNoIdx;
   const bool table = TOP && K == top && H.tt != nullptr && ek - lk > dev::kTopScan;
   if (table && gl == 0) dev::top_lookup(H.tt, H.tstride, lk, ek, bv, bpos);
-  for (uint32_t p = (lk & ~(STEP - 1)) + 4 * gl; !table && p < ek; p += STEP) {
+  // block scans: a final range of more than kScanFinalMin entries that is the rest of one
+  // R-block and the start of the next is the first's suffix minimum or the second's prefix
+  // minimum (one lane reads both, with their level-0 indices)
+  const bool scans = LT::kBlockScans && !table && K >= H.ps_lo && ek - lk > kScanFinalMin &&
+                     ((ek - 1u) >> RL) == (lk >> RL) + 1u;
+  if (scans && gl == 0) {
+    const uint2* ps = reinterpret_cast<const uint2*>(H.ps + offK);
+    const uint2 x = ps[2u * lk];
+    const uint2 y = ps[2u * (ek - 1u) + 1u];
+    const bool right = dev::u2f(y.x) < dev::u2f(x.x);
+    bv = dev::u2f(right ? y.x : x.x);
+    bpos = right ? y.y : x.y;
+  }
+  const bool direct = table || scans;  // bpos is then a level-0 index
+  for (uint32_t p = (lk & ~(STEP - 1)) + 4 * gl; !direct && p < ek; p += STEP) {
     if (p + 4 > lk) {
       const float4 x = dev::load4(vK, p, szK);
       if constexpr (IDX) {
@@ -424,10 +445,10 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
   dev::Cand b;
   if constexpr (IDX) {
     b.v = bv;
-    // a table answer is a level-0 index already (a key inside the entry's span, so candidates
-    // keep their order); a scanned one is a level-K position
-    b.key = bpos == dev::kNoIdx ? dev::kNoIdx : (table ? bpos : bpos << shK);
-    b.pay = bpos == dev::kNoIdx ? dev::kNoIdx : (table ? dev::kDirect : payK);
+    // a table or block-scan answer is a level-0 index already (a key inside the entry's span,
+    // so candidates keep their order); a scanned one is a level-K position
+    b.key = bpos == dev::kNoIdx ? dev::kNoIdx : (direct ? bpos : bpos << shK);
+    b.pay = bpos == dev::kNoIdx ? dev::kNoIdx : (direct ? dev::kDirect : payK);
     dev::group_cand<G>(b);
     bv = b.v;
   } else {
@@ -449,8 +470,26 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
     const uint32_t pay = k == 0 ? dev::kDirect : uk;
     const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);
     const uint32_t hi = e >> sk;
-    if (la) fold_piece<G, RL, IDX>(b, bv, vk, szk, lo, (lo + R - 1) & ~(R - 1), gl, sk, pay);
-    if (ra) fold_piece<G, RL, IDX>(b, bv, vk, szk, hi & ~(R - 1), hi, gl, sk, pay);
+    if (LT::kBlockScans && uk >= H.ps_lo) {
+      // a piece is the rest of its block (left) or the block's start (right): one block-scan
+      // entry each, read by one lane (an empty piece, on a block boundary, reads nothing)
+      if (gl == 0) {
+        const uint2* ps = reinterpret_cast<const uint2*>(H.ps + ok);
+        if (la && (lo & (R - 1u)) != 0u) {
+          const uint2 x = ps[2u * lo];
+          if constexpr (IDX) dev::cand_take(b, dev::u2f(x.x), x.y, dev::kDirect, true);
+          else bv = dev::fmin_sel(bv, dev::u2f(x.x));
+        }
+        if (ra && (hi & (R - 1u)) != 0u) {
+          const uint2 y = ps[2u * (hi - 1u) + 1u];
+          if constexpr (IDX) dev::cand_take(b, dev::u2f(y.x), y.y, dev::kDirect, true);
+          else bv = dev::fmin_sel(bv, dev::u2f(y.x));
+        }
+      }
+    } else {
+      if (la) fold_piece<G, RL, IDX>(b, bv, vk, szk, lo, (lo + R - 1) & ~(R - 1), gl, sk, pay);
+      if (ra) fold_piece<G, RL, IDX>(b, bv, vk, szk, hi & ~(R - 1), hi, gl, sk, pay);
+    }
     if constexpr (IDX) {
       dev::group_cand<G>(b);
       bv = b.v;
@@ -488,14 +527,14 @@ __device__ __forceinline__ float value_exact(uint32_t l, uint32_t e, uint32_t gl
 // its members' queries one after another (the reference's ballot work queue, here a fixed
 // round over the group's lanes with a ds_bpermute broadcast). Queries are read and answers
 // written with one coalesced access per wave.
-template <int G, int RL, bool P, bool IDX>
+template <int G, int RL, bool P, bool IDX, bool PS = false>
 __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ Q, uint32_t q,
                                                       float* __restrict__ out_v,
                                                       uint32_t* __restrict__ out_i, HierView H) {
   __shared__ uint32_t s_meta[kMetaWords];
   if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
-  const LevelsLds T{s_meta};
+  const LevelsLdsT<false, PS> T{s_meta};
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t gl = lane & (G - 1);
   const uint64_t qi = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
@@ -541,7 +580,7 @@ __global__ __launch_bounds__(kBlock) void k_query_xxx(const uint2* __restrict__ 
 // Alg 24 (xxx_query_multi_load, src/xxx.h:470-528): one G-lane group per query, every lane of
 // the group reads the same query; grid-stride over queries in units of groups (the
 // reference strides by the thread count, Appendix-A quirk not kept).
-template <int G, int RL, bool P, bool IDX>
+template <int G, int RL, bool P, bool IDX, bool PS = false>
 __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict__ Q, uint32_t q,
                                                         float* __restrict__ out_v,
                                                         uint32_t* __restrict__ out_i,
@@ -549,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void k_query_multi(const uint2* __restrict_
   __shared__ uint32_t s_meta[kMetaWords];
   if (threadIdx.x < kMetaWords) s_meta[threadIdx.x] = lds_meta(H, threadIdx.x);
   __syncthreads();
-  const LevelsLds T{s_meta};
+  const LevelsLdsT<false, PS> T{s_meta};
   const uint32_t gl = threadIdx.x & (G - 1);
   const uint64_t grp = (static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / G;
   const uint64_t ngrp = static_cast<uint64_t>(gridDim.x) * kBlock / G;
@@ -678,10 +717,6 @@ __device__ __forceinline__ uint32_t pay_index(uint32_t pay, const HierView& H, c
   return s0 + dev::rel_load(H.lr, T.roff(sk / RL), s0 >> sk, sk);
 }
 
-// Block scans answer a final range only when it is longer than this (shorter ones are one round
-// of four vector loads), and tail pieces only where R >= 16 (a piece of R <= 8 is at most two
-// vectors, whose loads also hold the next guard).
-constexpr uint32_t kScanFinalMin = 16;
 
 struct VlState {
   uint32_t l, e;

@@ -117,14 +117,14 @@ typedef struct {
 /* Block scans: the build also keeps, for every entry i of the upper levels, the leftmost minimum
  * of the rest of its R-block [i, block end) and of the block up to it [block start, i], each
  * with its level-0 index (16 bytes per entry; levels from the top down while they take at most
- * 64 MiB: all of them at n = 2^26, R = 32, levels 2.. at n = 2^30). Where R >= 16 the
- * lane-per-query solver (alg 20; algs 16/24 with group_lanes 1) then reads a tail piece above
- * level 0 as one entry, and a final range of more than 16 entries that spans two blocks as
- * two, instead of scanning up to R - 1 (2R - 2) entries. Same answers. Built by default for the
- * lane solver at R >= 32 (the algs 16/24 default); RMQ_BUILD_BLOCK_SCANS builds them at R = 16
- * too (measured at n = q = 2^26 there: lognormal-medium and uniform lengths faster, short and
- * mixed lengths slower) and RMQ_BUILD_NO_BLOCK_SCANS builds none. Ignored at R <= 8 and by the
- * group solvers and RMQ_BUILD_FULL. */
+ * 64 MiB: all of them at n = 2^26, R = 32, levels 2.. at n = 2^30). Where R >= 16 the pruned
+ * solvers (the lane-per-query one: alg 20, algs 16/24 with group_lanes 1; and the G-lane groups
+ * of algs 16/24) then read a tail piece above level 0 as one entry, and a final range of more
+ * than 16 entries that spans two blocks as two, instead of scanning up to R - 1 (2R - 2)
+ * entries. Same answers. Built by default at R >= 32 (the algs 16/24 default);
+ * RMQ_BUILD_BLOCK_SCANS builds them at R = 16 too (measured at n = q = 2^26 there for the lane
+ * solver: lognormal-medium and uniform lengths faster, short and mixed lengths slower) and
+ * RMQ_BUILD_NO_BLOCK_SCANS builds none. Ignored at R <= 8 and by RMQ_BUILD_FULL. */
 #define RMQ_BUILD_BLOCK_SCANS 0x40u
 #define RMQ_BUILD_NO_BLOCK_SCANS 0x80u
 /* Chunk size (log2, 8..30) of the 64-bit form for this build; arrays of more than 2^31 - 1

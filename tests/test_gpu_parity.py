@@ -54,8 +54,13 @@ VARIANTS = [
     (16, dict(paired=True)),
     (20, dict(cg_amount_log=0, paired=True, pack=True)),   # two per lane, descents packed
     (16, dict(paired=True, pack=True)),
-    # block prefix / suffix minima, read by the lane solver where R >= 16 (built by default at
-    # R >= 32, so the defaults above read them) and ignored by the group solvers
+    # block prefix / suffix minima, read by the lane and group solvers where R >= 16 (built by
+    # default at R >= 32, so the defaults above read them)
+    (16, dict(group_lanes=8, scans=False)),
+    (24, dict(group_lanes=4, scans=False)),
+    (16, dict(group_lanes=64, scans=False)),
+    (24, dict(group_lanes=4, cg_size_log=2, cg_amount_log=2, scans=True)),  # R = 16
+    (16, dict(group_lanes=16, cg_size_log=4, cg_amount_log=2, scans=True)),  # R = 64
     (20, dict(cg_amount_log=2, scans=True)),
     (20, dict(cg_amount_log=2, scans=True, pack=True)),
     (20, dict(cg_amount_log=2, scans=True, paired=True)),
