@@ -313,6 +313,26 @@ __device__ __forceinline__ void step_cand(Cand& b) {
   b.pay = t ? op : b.pay;
 }
 
+// The value of the first lane of each aligned group of W lanes (all W lanes active), in every
+// lane of the group: a quad permute (W = 4), a ds_swizzle and-mask (W = 8 .. 32), or the
+// wavefront's first lane (W = 64).
+template <int W>
+__device__ __forceinline__ uint32_t group_first(uint32_t x) {
+  static_assert(W == 4 || W == 8 || W == 16 || W == 32 || W == 64, "group of 4 .. 64 lanes");
+  if constexpr (W == 4) return dpp_mov<0x00>(x);  // quad_perm [0,0,0,0]
+  else if constexpr (W == 64) return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(x)));
+  else return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(x), 0x1F & ~(W - 1)));
+}
+
+// A group's candidate when only its first lane can hold a better one than the others (which
+// then all hold the same): that lane's, in every lane, without the reduction.
+template <int W>
+__device__ __forceinline__ void group_first_cand(Cand& b) {
+  b.v = u2f(group_first<W>(f2u(b.v)));
+  b.key = group_first<W>(b.key);
+  b.pay = group_first<W>(b.pay);
+}
+
 template <int W>
 __device__ __forceinline__ void group_cand(Cand& b) {
   if constexpr (W >= 2) step_cand<1>(b);

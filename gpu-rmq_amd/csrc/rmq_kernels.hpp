@@ -449,10 +449,12 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
     // so candidates keep their order); a scanned one is a level-K position
     b.key = bpos == dev::kNoIdx ? dev::kNoIdx : (direct ? bpos : bpos << shK);
     b.pay = bpos == dev::kNoIdx ? dev::kNoIdx : (direct ? dev::kDirect : payK);
-    dev::group_cand<G>(b);
+    // a direct answer is the first lane's alone: broadcast it instead of reducing
+    if (direct) dev::group_first_cand<G>(b);
+    else dev::group_cand<G>(b);
     bv = b.v;
   } else {
-    bv = dev::group_min<G>(bv);
+    bv = direct ? dev::u2f(dev::group_first<G>(dev::f2u(bv))) : dev::group_min<G>(bv);
   }
 
   // Both tails walk down together (rare for random data): per level the group folds the left
@@ -470,7 +472,8 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
     const uint32_t pay = k == 0 ? dev::kDirect : uk;
     const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);
     const uint32_t hi = e >> sk;
-    if (LT::kBlockScans && uk >= H.ps_lo) {
+    const bool by_scans = LT::kBlockScans && uk >= H.ps_lo;
+    if (by_scans) {
       // a piece is the rest of its block (left) or the block's start (right): one block-scan
       // entry each, read by one lane (an empty piece, on a block boundary, reads nothing)
       if (gl == 0) {
@@ -490,11 +493,13 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
       if (la) fold_piece<G, RL, IDX>(b, bv, vk, szk, lo, (lo + R - 1) & ~(R - 1), gl, sk, pay);
       if (ra) fold_piece<G, RL, IDX>(b, bv, vk, szk, hi & ~(R - 1), hi, gl, sk, pay);
     }
+    // the group held one best before this level; by block scans only the first lane moved
     if constexpr (IDX) {
-      dev::group_cand<G>(b);
+      if (by_scans) dev::group_first_cand<G>(b);
+      else dev::group_cand<G>(b);
       bv = b.v;
     } else {
-      bv = dev::group_min<G>(bv);
+      bv = by_scans ? dev::u2f(dev::group_first<G>(dev::f2u(bv))) : dev::group_min<G>(bv);
     }
     if (la) la = k != 0 && l < (lo << sk) && !(vk[lo - 1] > bv);  // deeper left pieces can still win
     if (ra) ra = k != 0 && (hi << sk) < e && vk[hi] < bv;
