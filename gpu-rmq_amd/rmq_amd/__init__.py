@@ -161,7 +161,7 @@ class Hierarchy:
     sparse table over the top level), pack=True (RMQ_BUILD_PACK_DESCENTS), check_queries=True
     (RMQ_BUILD_CHECK_QUERIES: out-of-range queries raise RMQError instead of being answered),
     paired=True (RMQ_BUILD_PAIRED: two queries per lane), scans=True / False (RMQ_BUILD_BLOCK_SCANS /
-    RMQ_BUILD_NO_BLOCK_SCANS: block prefix / suffix minima for the lane solver; None = the
+    RMQ_BUILD_NO_BLOCK_SCANS: block prefix / suffix minima for the lane and group solvers; None = the
     library's default, built where R >= 32).
     """
 
