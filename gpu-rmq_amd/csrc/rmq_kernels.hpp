@@ -472,6 +472,11 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
     const uint32_t pay = k == 0 ? dev::kDirect : uk;
     const uint32_t lo = (l >> sk) + ((l & ((1u << sk) - 1u)) != 0u);
     const uint32_t hi = e >> sk;
+    // the guards of the level below, loaded with this level's pieces rather than after them
+    const bool nl = la && k != 0 && l < (lo << sk);
+    const bool nr = ra && k != 0 && (hi << sk) < e;
+    const float g_next_l = nl ? vk[lo - 1] : RMQ_INF;
+    const float g_next_r = nr ? vk[hi] : RMQ_INF;
     const bool by_scans = LT::kBlockScans && uk >= H.ps_lo;
     if (by_scans) {
       // a piece is the rest of its block (left) or the block's start (right): one block-scan
@@ -501,8 +506,8 @@ __device__ __forceinline__ GroupAnswer solve_cl_pruned(uint32_t l, uint32_t e, u
     } else {
       bv = by_scans ? dev::u2f(dev::group_first<G>(dev::f2u(bv))) : dev::group_min<G>(bv);
     }
-    if (la) la = k != 0 && l < (lo << sk) && !(vk[lo - 1] > bv);  // deeper left pieces can still win
-    if (ra) ra = k != 0 && (hi << sk) < e && vk[hi] < bv;
+    la = nl && !(g_next_l > bv);  // deeper left pieces can still win
+    ra = nr && g_next_r < bv;
   }
   if constexpr (IDX) return GroupAnswer{b.v, cand_index<RL>(b, H, T)};
   else return GroupAnswer{bv, dev::kNoIdx};
