@@ -276,7 +276,9 @@ def test_config2_index_sampled_full_n(device, t_log):
     lr = _queries_2_26()[: 1 << 22]
     ref_v, ref_i = oracle.CpuHierarchy(A, 1 << 15, 5, 16).query(lr)
     for alg, opts in ((16, {}), (24, {}), (20, {}), (20, dict(cg_amount_log=0, pack=True)),
-                      (20, dict(cg_amount_log=2, scans=True)), (16, dict(paired=True)), (16, dict(scans=False))):
+                      (20, dict(cg_amount_log=2, scans=True)), (16, dict(paired=True)), (16, dict(scans=False)),
+                      # the cooperative forms, with block scans at R = 32
+                      (16, dict(group_lanes=4)), (16, dict(group_lanes=8)), (24, dict(group_lanes=4))):
         _say(f"config2 alg={alg} opts={opts} t=2^{t_log}")
         v, i = run_gpu(A, lr, alg, 1 << t_log, True, True, **opts)
         assert_exact(A, lr, v, i, ref_i, f"config2 alg={alg} t=2^{t_log}")
