@@ -202,9 +202,10 @@ __device__ __forceinline__ void queue_shape(const uint32_t* __restrict__ qstart,
 
 // Eight wavefronts per SIMD (64 VGPRs): with every variant's solver (LANES = false) the
 // occupancy is worth more than the few registers the compiler then spills (measured); LANES =
-// true, for launches whose arrays all run lane variants, leaves the group solvers out and fits
-// in 60 registers without spills, with the sparse table over each array's top level (the
-// batched build makes one for top levels of 64..8192 entries). The kernel with every variant
+// true, for launches whose arrays all run lane variants, leaves the group solvers out and, with
+// the sparse table over each array's top level (the batched build makes one for top levels of
+// 64..8192 entries), fits 64 registers with 16 bytes per lane spilled in index mode; held to 7
+// workgroups per CU it needs 67 and spills nothing, but ran 4% slower (4096-array workload). The kernel with every variant
 // scans the top level instead: the table path would spill there.
 template <bool IDX, bool LANES>
 __global__ __launch_bounds__(kBlock, 8) void k_query_batch(
