@@ -269,8 +269,12 @@ rmq_status build_levels(uint32_t R_log, const HierView& H, hipStream_t s) {
 template <int RL>
 rmq_status launch_block_scans(const HierView& H, hipStream_t s) {
   for (uint32_t k = H.ps_lo; k < H.num_levels; ++k) {
-    const uint32_t blocks = (H.size[k] + (1u << RL) - 1u) >> RL;
-    rmq::k_block_scans<RL><<<(blocks + kBlock - 1) / kBlock, kBlock, 0, s>>>(H, k, const_cast<uint4*>(H.ps));
+    if constexpr (RL <= 6) {  // a lane per entry
+      rmq::k_block_scans_lanes<RL><<<(H.size[k] + kBlock - 1) / kBlock, kBlock, 0, s>>>(H, k, const_cast<uint4*>(H.ps));
+    } else {  // a thread per block
+      const uint32_t blocks = (H.size[k] + (1u << RL) - 1u) >> RL;
+      rmq::k_block_scans<RL><<<(blocks + kBlock - 1) / kBlock, kBlock, 0, s>>>(H, k, const_cast<uint4*>(H.ps));
+    }
   }
   return last_error();
 }

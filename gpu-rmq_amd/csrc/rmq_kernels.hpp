@@ -219,6 +219,42 @@ __global__ __launch_bounds__(kBlock) void k_block_scans(HierView H, uint32_t k, 
   }
 }
 
+// The same scans with one lane per entry (R <= 64: a block is an aligned segment of R lanes of a
+// wavefront): log-step scans across the segment, leftwards for the prefix (a left partner wins
+// ties) and rightwards for the suffix (a right partner must be strictly smaller), then one
+// 16-byte store per lane, so a wavefront writes 1 KiB of consecutive entries. Lanes past the
+// level hold +inf and store nothing; no entry of the level reads them (a prefix looks left,
+// and a right partner of +inf never replaces an entry).
+template <int RL>
+__global__ __launch_bounds__(kBlock) void k_block_scans_lanes(HierView H, uint32_t k, uint4* __restrict__ ps) {
+  constexpr int R = 1 << RL;
+  static_assert(R <= 64, "a block within one wavefront");
+  const uint32_t size = H.size[k];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t j = i & static_cast<uint32_t>(R - 1);
+  const bool live = i < size;
+  const float x = live ? H.lv[H.off[k] + i] : RMQ_INF;
+  const uint32_t xi = live ? level0_index(H.lr, H.roff[k], i, RL * k) : 0u;
+  float pv = x, sv = x;
+  uint32_t pi = xi, si = xi;
+#pragma unroll
+  for (int d = 1; d < R; d <<= 1) {
+    const float lv = dev::u2f(static_cast<uint32_t>(__shfl_up(static_cast<int>(dev::f2u(pv)), d, R)));
+    const uint32_t li = static_cast<uint32_t>(__shfl_up(static_cast<int>(pi), d, R));
+    const float rv = dev::u2f(static_cast<uint32_t>(__shfl_down(static_cast<int>(dev::f2u(sv)), d, R)));
+    const uint32_t ri = static_cast<uint32_t>(__shfl_down(static_cast<int>(si), d, R));
+    if (j >= static_cast<uint32_t>(d) && lv <= pv) {
+      pv = lv;
+      pi = li;
+    }
+    if (j + static_cast<uint32_t>(d) < static_cast<uint32_t>(R) && rv < sv) {
+      sv = rv;
+      si = ri;
+    }
+  }
+  if (live) ps[H.off[k] + i] = make_uint4(dev::f2u(sv), si, dev::f2u(pv), pi);
+}
+
 // ---------------------------------------------------------------------------------------
 // Query traversal shared by algs 16 and 24 (src/xxx.h:158-187 and 493-521): r exclusive;
 // while the remaining range at this level is longer than 2R, the partial blocks

@@ -8,6 +8,9 @@
         [--counters-json profiles/counters.json --counters-key K] \
         --command "..." > profiles/<kernel>_<workload>.md
 
+--build-kernels k1,k2,... (with --bytes-per-build) summarises the build instead: every dispatch of
+those kernels, summed and divided by the number of builds (dispatches of --build-count-kernel).
+
 --counters-json records, under key K (bench.py's counters_key for the same kernel and workload),
 the measured HBM bytes per dispatch and, with --insts (a pass over SQ_INSTS_VALU SQ_INSTS_SALU
 SQ_WAVES), the wave-instructions per dispatch, so that bench.py reports roofline.traffic and the
@@ -55,11 +58,61 @@ def _stats(path, kernel):
     raise SystemExit(f"kernel {kernel!r} not in stats")
 
 
+def _all_stats(path):
+    with open(path) as f:
+        return [(row["Name"], int(row["Calls"]), float(row["TotalDurationNs"])) for row in csv.DictReader(f)]
+
+
+def _sum_counter(path, counter, names):
+    total = 0.0
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] == counter and any(k in row["Kernel_Name"] for k in names):
+                total += float(row["Counter_Value"])
+    return total
+
+
+def build_summary(a):
+    """Per-build figures of the build kernels (--build-kernels, comma list of name parts), summed
+    over all their dispatches and divided by the number of builds (the dispatches of
+    --build-count-kernel, one per build)."""
+    names = [k for k in a.build_kernels.split(",") if k]
+    stats = _all_stats(a.stats)
+    builds = sum(c for n, c, _ in stats if a.build_count_kernel in n)
+    if builds == 0:
+        raise SystemExit(f"no dispatches of {a.build_count_kernel!r}")
+    rows = [(n, c, t) for n, c, t in stats if any(k in n for k in names)]
+    ns = sum(t for _, _, t in rows) / builds
+    fetch = _sum_counter(a.fetch, "FETCH_SIZE", names) / builds
+    write = _sum_counter(a.write, "WRITE_SIZE", names) / builds
+    _, cf, _ = _match(_per_kernel(a.copy_fetch, "FETCH_SIZE"), "copyBuffer")
+    _, cw, _ = _match(_per_kernel(a.copy_write, "WRITE_SIZE"), "copyBuffer")
+    gib_kib = float(1 << 20)
+    fetch_b = fetch * 1024 * gib_kib / cf
+    write_b = write * 1024 * gib_kib / cw
+    algo = a.bytes_per_build
+    out = [f"# {a.title}\n", "Commands (run from /tmp with TMPDIR=/tmp, repo at $R):\n"]
+    out += [f"    {c}" for c in a.command]
+    out += ["", "| kernel | dispatches per build | time per build |", "|---|---|---|"]
+    for n, c, t in rows:
+        out.append(f"| `{n}` | {c / builds:g} | {t / builds / 1e3:.2f} us |")
+    out += ["", "| quantity | value |", "|---|---|",
+            f"| builds traced | {builds} |",
+            f"| kernel time per build | {ns / 1e3:.2f} us |",
+            f"| algorithmic bytes per build (level 0 read once, every upper entry and offset written once) | {algo / 1e6:.1f} MB |",
+            f"| algorithmic GB/s at kernel time | {algo / ns:.1f} ({algo / ns / roofline.HBM_PEAK_GBPS:.3f} of 8 TB/s) |",
+            f"| measured traffic per build | {(fetch_b + write_b) / 1e6:.1f} MB (fetch {fetch_b / 1e6:.1f} + write {write_b / 1e6:.1f}) |",
+            f"| measured GB/s at kernel time | {(fetch_b + write_b) / ns:.1f} ({(fetch_b + write_b) / ns / roofline.HBM_PEAK_GBPS:.3f} of 8 TB/s) |"]
+    for line in a.note:
+        out += ["", line]
+    print("\n".join(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--kernel", required=True)
-    ap.add_argument("--queries", type=int, required=True)
-    ap.add_argument("--bytes-per-query", type=float, required=True)
+    ap.add_argument("--kernel", default="")
+    ap.add_argument("--queries", type=int, default=0)
+    ap.add_argument("--bytes-per-query", type=float, default=0.0)
     ap.add_argument("--stats", required=True)
     ap.add_argument("--fetch", required=True)
     ap.add_argument("--write", required=True)
@@ -73,7 +126,13 @@ def main():
     ap.add_argument("--note", action="append", default=[], help="a line printed after the table")
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("--title", default="")
+    ap.add_argument("--build-kernels", help="build mode: comma list of kernel name parts summed per build")
+    ap.add_argument("--build-count-kernel", default="k_init_words", help="one dispatch per build")
+    ap.add_argument("--bytes-per-build", type=float, default=0.0)
     a = ap.parse_args()
+    if a.build_kernels:
+        build_summary(a)
+        return
 
     name, calls, avg_ns, min_ns, max_ns = _stats(a.stats, a.kernel)
     _, fetch_kib, _ = _match(_per_kernel(a.fetch, "FETCH_SIZE"), a.kernel)
