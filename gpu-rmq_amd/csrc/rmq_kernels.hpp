@@ -33,6 +33,9 @@ struct PassShape {
   static constexpr uint32_t SLOG = kPassTileLog;
   static constexpr uint32_t M = SLOG / RL;    // levels per pass: 6 at R = 4, 2 at R = 32
   static constexpr uint32_t LDS = 1u << (SLOG - RL);  // level-(k+1) entries of one tile
+  // the LDS levels give one thread each entry of level k+2 and up (tiles of 2^13 entries at R = 4
+  // would leave half of a tile's level-(k+2) entries unmade)
+  static_assert(M < 2 || LDS / (1u << RL) <= kBlock, "level k+2 of a tile must fit one workgroup");
 };
 
 // Tile `tile` of the pass from level k of H making levels k+1 .. k+m (m <= M). s_v / s_i hold
